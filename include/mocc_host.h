@@ -1,0 +1,71 @@
+/*
+ * mocc_host.h -- C ABI of the host front-end: read a mocc input deck (XML
+ * plus XSL), build the 2-D core mesh, the modularized angular quadrature and
+ * the ray trace, and expose the result as a mocgpu_problem.
+ *
+ * Replaces the set-up half of mocc's start-up path: InputProcessor and
+ * CoreMesh (src/input_proc.cpp:51-213, src/core/core_mesh.cpp:28-206), the
+ * AngularQuadrature (src/core/angular_quadrature.cpp:47-113) and RayData
+ * (src/sweepers/moc/ray_data.cpp:67-343). Errors are reported through int
+ * status codes and mocc_host_last_error() (thread-local).
+ */
+#ifndef MOCC_HOST_H
+#define MOCC_HOST_H
+
+#include <stdint.h>
+
+#include "mocgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mocc_host_problem mocc_host_problem;
+
+typedef struct mocc_host_params {
+    double k_tol, psi_tol;       /* <solver k_tol psi_tol>            */
+    int32_t max_iter, min_iter;  /* <solver max_iter min_iter>        */
+    int32_t n_inner;             /* <sweeper n_inner>                 */
+    int32_t gs_boundary;         /* <sweeper boundary_update> (1 = gs) */
+    int32_t cmfd;                /* <solver cmfd>                     */
+    int32_t ndir_oct;
+} mocc_host_params;
+
+const char *mocc_host_last_error(void);
+
+/* Read and trace a deck from a file, or from a string whose relative
+ * material_lib path is resolved against base_dir. */
+int mocc_host_load(const char *xml_path, mocc_host_problem **out);
+int mocc_host_load_string(const char *xml, const char *base_dir, mocc_host_problem **out);
+void mocc_host_free(mocc_host_problem *h);
+
+const mocgpu_problem *mocc_host_view(const mocc_host_problem *h);
+int mocc_host_get_params(const mocc_host_problem *h, mocc_host_params *out);
+int mocc_host_get_options(const mocc_host_problem *h, mocgpu_options *out);
+
+/* Modularized quadrature, all 8 octants: rows of (alpha, theta, ox, oy, oz,
+ * weight, rsintheta). Returns the number of angles, or < 0. */
+int mocc_host_quadrature(const mocc_host_problem *h, double *out, int32_t cap);
+
+/* Unmodified quadrature from an <ang_quad .../> string (AngularQuadrature
+ * constructor), same row layout. */
+int mocc_host_quadrature_xml(const char *ang_quad_xml, double *out, int32_t cap);
+
+/* Trace one ray between two boundary points of the core of h (moc::Ray).
+ * seg_* receive up to cap segments; cm_head = {cell_fw, surf_fw, cell_bw,
+ * surf_bw, n_cm}; cm receives 4*n_cm ints. Returns nseg or < 0. */
+int mocc_host_trace_ray(const mocc_host_problem *h, double x1, double y1, double x2,
+                        double y2, int32_t *seg_fsr, double *seg_len, int32_t cap,
+                        int32_t *cm_head, int32_t *cm, int32_t cm_cap);
+
+/* Geometry-only decks (no <sweeper>): build only the core mesh, for ray tests. */
+int mocc_host_load_mesh_string(const char *xml, const char *base_dir, mocc_host_problem **out);
+
+/* Total FSR area per region of the plane, [n_fsr] (the true volumes the
+ * FLAT correction preserves). */
+int mocc_host_fsr_areas(const mocc_host_problem *h, double *out, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOCC_HOST_H */

@@ -1,0 +1,240 @@
+/*
+ * mocgpu.h -- C ABI of the MI355X (gfx950) 2-D Method-of-Characteristics
+ * sweeper, source update and pin-cell CMFD solve.
+ *
+ * This is the drop-in boundary for mocc's transport-sweeper plugin
+ * interface. Every entry point below replaces one member of the C++ class
+ * mocc::TransportSweeper / mocc::moc::MoCSweeper / mocc::CMFD /
+ * mocc::EigenSolver (file:line cited per function). The signatures use only
+ * plain C types and pointers so that any host language can bind them; a
+ * handle is not thread-safe, distinct handles are independent.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = OK, < 0 = error. The message
+ *    of the last error on a handle is returned by mocgpu_last_error().
+ *  - Scalar flux arrays are [n_fsr][n_group] (group fastest), FP64, exactly
+ *    the layout of TransportSweeper::flux_ (src/core/transport_sweeper.cpp:81).
+ *  - Angle index iang follows mocc::AngularQuadrature: octant-major, then
+ *    azimuth, then polar (src/core/product_quadrature.hpp:150-157).
+ */
+#ifndef MOCGPU_H
+#define MOCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (mapped from the reference's exception sites). */
+#define MOCGPU_OK 0
+#define MOCGPU_ERR_ARG -1        /* invalid argument / size mismatch            */
+#define MOCGPU_ERR_DEVICE -2     /* HIP runtime error                           */
+#define MOCGPU_ERR_NAN -3        /* keff not a number (eigen_solver.cpp:205-208) */
+#define MOCGPU_ERR_NONPOS_FLUX -4/* set_pin_flux_1g <= 0 (moc_sweeper.cpp:367-375) */
+#define MOCGPU_ERR_MAXITER -5    /* max_iter reached (eigen_solver.cpp:219-221)  */
+#define MOCGPU_ERR_UNSUPPORTED -6/* option not implemented on the GPU path       */
+#define MOCGPU_ERR_CAPACITY -7   /* request larger than an admission limit      */
+
+/* Boundary condition kinds, same numbering as mocc::Boundary
+ * (src/core/constants.hpp). */
+#define MOCGPU_BC_VACUUM 0
+#define MOCGPU_BC_REFLECT 1
+
+/* Surface numbering, same as mocc::Surface (src/core/constants.hpp). */
+#define MOCGPU_EAST 0
+#define MOCGPU_NORTH 1
+#define MOCGPU_WEST 2
+#define MOCGPU_SOUTH 3
+#define MOCGPU_TOP 4
+#define MOCGPU_BOTTOM 5
+#define MOCGPU_INVALID 7
+
+/*
+ * One 2-D MoC problem as emitted by the host ray tracer, in
+ * structure-of-arrays form. All real data is FP64 (the reference's
+ * real_t); the GPU library converts what its kernels read to FP32.
+ *
+ * Rays are shared by the polar angles of an azimuth (modularization depends
+ * on the azimuth only, src/sweepers/moc/ray_data.cpp:491-527), so segment
+ * data is stored once per azimuth. Azimuth a in [0, n_azi) is octant
+ * (a / n_azi_oct) + 1 and azimuthal index (a % n_azi_oct) in that octant;
+ * the reference angle index of (a, p) is iang = a * n_polar + p.
+ */
+typedef struct mocgpu_problem {
+    /* --- sizes ---------------------------------------------------------- */
+    int32_t n_fsr;       /* F: flat source regions                        */
+    int32_t n_group;     /* G                                             */
+    int32_t n_polar;     /* P: polar angles per azimuth                   */
+    int32_t n_azi;       /* A: azimuths swept (octants 1 and 2)           */
+    int32_t n_ray;       /* R: rays over all azimuths                     */
+    int32_t n_mat;       /* XS regions (materials)                        */
+    int64_t n_seg;       /* S: 2-D segments over all rays                 */
+
+    /* --- angles: index a*P+p over octants 1-2 (2*ndir_oct entries) ---- */
+    const double *ang_weight;    /* quadrature weight (sum 8 over sphere)  */
+    const double *ang_sin_theta; /* sin(theta) used by sweep1g             */
+    const double *ang_rsin_theta;/* 1/sin(theta) (Direction::rsintheta)    */
+    const double *ang_ox;        /* direction cosines (Current weights)    */
+    const double *ang_oy;
+    const double *ang_alpha;     /* azimuthal angle                        */
+
+    /* --- azimuths ------------------------------------------------------- */
+    const int32_t *azi_ray_off;  /* [A+1] rays of azimuth a               */
+    const int32_t *azi_nx;       /* [A] rays on the y-normal faces        */
+    const int32_t *azi_ny;       /* [A] rays on the x-normal faces        */
+    const double *azi_spacing;   /* [A] ray spacing after modularization  */
+
+    /* --- rays ----------------------------------------------------------- */
+    const int64_t *ray_seg_off;  /* [R+1] segments of ray r               */
+    const int32_t *ray_bc;       /* [2R] entry/exit boundary index (bc(0), bc(1)) */
+    const int32_t *seg_fsr;      /* [S] FSR index of each segment         */
+    const double *seg_len;       /* [S] volume-corrected 2-D length       */
+
+    /* --- coarse (pin-cell) crossings, as moc::Ray coarse data ----------- */
+    const int32_t *ray_cm_cell;  /* [2R] first coarse cell fw, bw        */
+    const int32_t *ray_cm_surf;  /* [2R] first coarse surface fw, bw     */
+    const int32_t *ray_cm_off;   /* [R+1] entries of ray r in cm_data    */
+    const int32_t *cm_data;      /* [4*n_cm] (fw surf, bw surf, nseg_fw, nseg_bw) */
+
+    /* --- regions and cross sections -------------------------------------- */
+    const double *fsr_vol;       /* [F] area * plane height               */
+    const double *fsr_area;      /* [F] 2-D area (PinMesh::areas())       */
+    const int32_t *fsr_mat;      /* [F] XS region of each FSR             */
+    const double *xs_tr;         /* [n_mat*G] transport XS                */
+    const double *xs_nf;         /* [n_mat*G] nu-fission                  */
+    const double *xs_f;          /* [n_mat*G] fission                     */
+    const double *xs_ch;         /* [n_mat*G] chi (normalized)            */
+    const double *xs_sc;         /* [n_mat*G*G] scattering [to][from]     */
+
+    /* --- pin (coarse) mesh ---------------------------------------------- */
+    int32_t nx, ny;              /* pins along x, y                       */
+    const double *x_vec;         /* [nx+1] pin boundaries                 */
+    const double *y_vec;         /* [ny+1]                                */
+    double hz;                   /* plane (macroplane) height             */
+    const int32_t *pin_fsr_off;  /* [nx*ny+1] FSRs of pin i (FSR order)   */
+    const int32_t *pin_cell;     /* [nx*ny] coarse cell of pin i          */
+    const double *pin_area;      /* [nx*ny] pin mesh area (pitch_x*pitch_y) */
+    int32_t bc[6];               /* mocc::Surface-indexed boundary kinds  */
+    int32_t n_fuel_pins;         /* CoreMesh::n_fuel_2d()                 */
+} mocgpu_problem;
+
+/* Iteration options, the attributes of <sweeper> / <solver> / <cmfd>. */
+typedef struct mocgpu_options {
+    int32_t n_inner;        /* <sweeper n_inner>                           */
+    int32_t energy_mode;    /* 0: Gauss-Seidel over groups (FixedSourceSolver::step),
+                               1: Jacobi (all groups from the previous outer) */
+    int32_t lane_map;       /* -1 autotune; else a ray/polar/group -> lane mapping id */
+    int32_t band_rays;      /* rays per band (0 = library default)         */
+    int32_t use_cmfd;       /* <solver cmfd="t">                           */
+    int32_t cmfd_max_iter;  /* <cmfd max_iter>, default 100                */
+    double cmfd_k_tol;      /* <cmfd k_tol>, default 1e-6                  */
+    double cmfd_psi_tol;    /* <cmfd psi_tol>, default 1e-5                */
+    double cmfd_resid_reduction; /* <cmfd residual_reduction>, default 1e-3 */
+    int32_t cmfd_solver;    /* 0 BiCGSTAB (Eigen default), 1 GMRES         */
+    int32_t reserved;
+} mocgpu_options;
+
+typedef struct mocgpu_handle mocgpu_handle;
+
+/* Fill *opt with the reference defaults. */
+void mocgpu_default_options(mocgpu_options *opt);
+
+/* Build the device-resident sweeper for one problem.
+ * Replaces moc::MoCSweeper::MoCSweeper (src/sweepers/moc/moc_sweeper.cpp:62-187)
+ * for the data the sweep reads; ray tracing stays on the host. */
+int mocgpu_create(const mocgpu_problem *prob, const mocgpu_options *opt,
+                  mocgpu_handle **out);
+int mocgpu_destroy(mocgpu_handle *h);
+const char *mocgpu_last_error(const mocgpu_handle *h);
+
+/* MoCSweeper::initialize (moc_sweeper.cpp:234-253): phi = 1, psi_bc = 1/(4 pi)
+ * on non-vacuum faces, coarse flux = 1. */
+int mocgpu_initialize(mocgpu_handle *h);
+
+/* Copy the scalar flux [F][G] in / out (TransportSweeper::flux()). */
+int mocgpu_set_flux(mocgpu_handle *h, const double *phi);
+int mocgpu_get_flux(mocgpu_handle *h, double *phi);
+
+/* Incoming boundary angular flux for all groups, [G][4*ndir_oct][face values]
+ * in BoundaryCondition::data_ order (src/core/boundary_condition.cpp:33-78). */
+int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n);
+int mocgpu_set_boundary(mocgpu_handle *h, const double *bc, int64_t n);
+int64_t mocgpu_boundary_size(const mocgpu_handle *h);
+
+/* TransportSweeper::store_old_flux (transport_sweeper.hpp:271-275). */
+int mocgpu_store_old_flux(mocgpu_handle *h);
+
+/* TransportSweeper::calc_fission_source (transport_sweeper.cpp:119-134):
+ * device fs = (1/k) sum_g nuSigf phi. If fs_out != NULL it is copied out [F]. */
+int mocgpu_calc_fission_source(mocgpu_handle *h, double k, double *fs_out);
+/* Upload an explicit fission source [F] (FixedSourceSolver::set_fission_source). */
+int mocgpu_set_fission_source(mocgpu_handle *h, const double *fs);
+
+/* Source::initialize_group + fission + in_scatter for group g
+ * (src/core/source.cpp:41-112), into the device one-group source. */
+int mocgpu_build_source(mocgpu_handle *h, int32_t g);
+
+/* MoCSweeper::sweep(g) (moc_sweeper.cpp:189-225): n_inner x {self_scatter;
+ * sweep1g}; on the last inner, if with_current, the moc::Current tallies
+ * (moc_current_worker.hpp:137-330). The boundary update is Jacobi in angle
+ * (the reference's boundary_update="jacobi", moc_sweeper_kernel.inc.hpp:146-150). */
+int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current);
+
+/* FixedSourceSolver::step (src/solvers/fixed_source_solver.cpp:102-117) with
+ * the handle's energy_mode: store old flux; for each group build the source
+ * and sweep. */
+int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current);
+
+/* TransportSweeper::total_fission(old) (transport_sweeper.cpp:104-117), FP64. */
+int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out);
+
+/* TransportSweeper::flux_residual (transport_sweeper.cpp:210-223). */
+int mocgpu_flux_residual(mocgpu_handle *h, double *out);
+
+/* MoCSweeper::get_pin_flux_1g / set_pin_flux_1g with MeshTreatment::PIN_PLANE
+ * (moc_sweeper.cpp:293-433). pin arrays are [nx*ny] in coarse-cell order. */
+int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin);
+int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin,
+                           double *resid);
+
+/* CoarseData current / surface_flux of group g after the last sweep with
+ * currents, [n_surf] each in mocc::Mesh surface order (mesh.cpp:99-147). */
+int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current,
+                        double *surface_flux);
+int32_t mocgpu_n_surf(const mocgpu_handle *h);
+
+/* CMFD::solve (src/core/cmfd.cpp:171-279) on the device in FP64, with the
+ * coarse flux taken from the current fine flux (EigenSolver::do_cmfd,
+ * eigen_solver.cpp:263-291: get_pin_flux, solve, set_pin_flux). */
+int mocgpu_cmfd_step(mocgpu_handle *h, double *k);
+
+/* EigenSolver::step (eigen_solver.cpp:225-253), device-resident: optional CMFD,
+ * fission source, FSS step, k update. err_psi follows eigen_solver.cpp:171-184. */
+typedef struct mocgpu_eigen_state {
+    double k, k_prev, err_k, err_psi;
+    int32_t iterations;
+    int32_t converged;
+} mocgpu_eigen_state;
+
+/* EigenSolver::solve (eigen_solver.cpp:131-223). */
+int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol,
+                       int32_t max_iter, int32_t min_iter,
+                       mocgpu_eigen_state *state);
+/* One outer iteration (EigenSolver::step + the convergence bookkeeping of solve). */
+int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
+
+/* Number of (2-D segment, polar, direction, group) updates per inner sweep of
+ * all groups: 2 * P * S * G. */
+int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h);
+
+/* Time the sweep kernel: run n_sweeps all-group inner sweeps (Jacobi energy,
+ * no currents) and report the mean kernel duration measured with HIP events
+ * on the library's stream. */
+int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n_sweeps, double *ms_per_sweep,
+                       double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOCGPU_H */

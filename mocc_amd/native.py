@@ -1,0 +1,202 @@
+"""ctypes bindings of the two product libraries.
+
+* ``libmocc_host.so`` -- input deck reader, core mesh, quadrature and ray
+  tracer (include/mocc_host.h).
+* ``libmocgpu.so`` -- the gfx950 sweeper / source / CMFD library
+  (include/mocgpu.h).
+
+Both are built in-tree into ``mocc_amd/lib`` by ``mocc_amd.build``. Loading
+fails loudly when a library is missing: there is no CPU fallback for the
+device path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+i32 = C.c_int32
+i64 = C.c_int64
+f64 = C.c_double
+P = C.c_void_p
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+lp = C.POINTER(C.c_int64)
+
+
+class MocgpuProblem(C.Structure):
+    """Mirror of ``mocgpu_problem`` (include/mocgpu.h)."""
+
+    _fields_ = [
+        ("n_fsr", i32), ("n_group", i32), ("n_polar", i32), ("n_azi", i32),
+        ("n_ray", i32), ("n_mat", i32), ("n_seg", i64),
+        ("ang_weight", dp), ("ang_sin_theta", dp), ("ang_rsin_theta", dp),
+        ("ang_ox", dp), ("ang_oy", dp), ("ang_alpha", dp),
+        ("azi_ray_off", ip), ("azi_nx", ip), ("azi_ny", ip), ("azi_spacing", dp),
+        ("ray_seg_off", lp), ("ray_bc", ip), ("seg_fsr", ip), ("seg_len", dp),
+        ("ray_cm_cell", ip), ("ray_cm_surf", ip), ("ray_cm_off", ip), ("cm_data", ip),
+        ("fsr_vol", dp), ("fsr_area", dp), ("fsr_mat", ip),
+        ("xs_tr", dp), ("xs_nf", dp), ("xs_f", dp), ("xs_ch", dp), ("xs_sc", dp),
+        ("nx", i32), ("ny", i32), ("x_vec", dp), ("y_vec", dp), ("hz", f64),
+        ("pin_fsr_off", ip), ("pin_cell", ip), ("pin_area", dp),
+        ("bc", i32 * 6), ("n_fuel_pins", i32),
+    ]
+
+
+class MocgpuOptions(C.Structure):
+    _fields_ = [
+        ("n_inner", i32), ("energy_mode", i32), ("lane_map", i32), ("band_rays", i32),
+        ("use_cmfd", i32), ("cmfd_max_iter", i32), ("cmfd_k_tol", f64),
+        ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("cmfd_solver", i32),
+        ("reserved", i32),
+    ]
+
+
+class EigenState(C.Structure):
+    _fields_ = [("k", f64), ("k_prev", f64), ("err_k", f64), ("err_psi", f64),
+                ("iterations", i32), ("converged", i32)]
+
+
+class HostParams(C.Structure):
+    _fields_ = [("k_tol", f64), ("psi_tol", f64), ("max_iter", i32), ("min_iter", i32),
+                ("n_inner", i32), ("gs_boundary", i32), ("cmfd", i32), ("ndir_oct", i32)]
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(P)
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run mocc_amd.build.build_all() first")
+    return C.CDLL(path)
+
+
+_host = None
+_gpu = None
+
+
+def host() -> C.CDLL:
+    global _host
+    if _host is None:
+        L = _load("libmocc_host.so")
+        L.mocc_host_last_error.restype = C.c_char_p
+        L.mocc_host_view.restype = C.POINTER(MocgpuProblem)
+        L.mocc_host_view.argtypes = [P]
+        L.mocc_host_load.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.mocc_host_load_string.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(P)]
+        L.mocc_host_load_mesh_string.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(P)]
+        L.mocc_host_free.argtypes = [P]
+        L.mocc_host_get_params.argtypes = [P, C.POINTER(HostParams)]
+        L.mocc_host_get_options.argtypes = [P, C.POINTER(MocgpuOptions)]
+        L.mocc_host_quadrature.argtypes = [P, P, i32]
+        L.mocc_host_quadrature_xml.argtypes = [C.c_char_p, P, i32]
+        L.mocc_host_trace_ray.argtypes = [P, f64, f64, f64, f64, P, P, i32, P, P, i32]
+        L.mocc_host_fsr_areas.argtypes = [P, P, i32]
+        _host = L
+    return _host
+
+
+def gpu() -> C.CDLL:
+    global _gpu
+    if _gpu is None:
+        L = _load("libmocgpu.so")
+        L.mocgpu_last_error.restype = C.c_char_p
+        L.mocgpu_last_error.argtypes = [P]
+        L.mocgpu_default_options.argtypes = [C.POINTER(MocgpuOptions)]
+        L.mocgpu_create.argtypes = [C.POINTER(MocgpuProblem), C.POINTER(MocgpuOptions), C.POINTER(P)]
+        L.mocgpu_destroy.argtypes = [P]
+        for fn in ("mocgpu_initialize", "mocgpu_store_old_flux"):
+            getattr(L, fn).argtypes = [P]
+        L.mocgpu_set_flux.argtypes = [P, P]
+        L.mocgpu_get_flux.argtypes = [P, P]
+        L.mocgpu_boundary_size.restype = i64
+        L.mocgpu_boundary_size.argtypes = [P]
+        L.mocgpu_get_boundary.argtypes = [P, P, i64]
+        L.mocgpu_set_boundary.argtypes = [P, P, i64]
+        L.mocgpu_calc_fission_source.argtypes = [P, f64, P]
+        L.mocgpu_set_fission_source.argtypes = [P, P]
+        L.mocgpu_build_source.argtypes = [P, i32]
+        L.mocgpu_sweep.argtypes = [P, i32, i32]
+        L.mocgpu_fss_step.argtypes = [P, i32]
+        L.mocgpu_total_fission.argtypes = [P, i32, P]
+        L.mocgpu_flux_residual.argtypes = [P, P]
+        L.mocgpu_get_pin_flux_1g.argtypes = [P, i32, P]
+        L.mocgpu_set_pin_flux_1g.argtypes = [P, i32, P, P]
+        L.mocgpu_get_currents.argtypes = [P, i32, P, P]
+        L.mocgpu_n_surf.restype = i32
+        L.mocgpu_n_surf.argtypes = [P]
+        L.mocgpu_cmfd_step.argtypes = [P, P]
+        L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
+        L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
+        L.mocgpu_updates_per_sweep.restype = i64
+        L.mocgpu_updates_per_sweep.argtypes = [P]
+        L.mocgpu_time_sweeps.argtypes = [P, i32, P, P]
+        _gpu = L
+    return _gpu
+
+
+class HostProblem:
+    """A traced deck: owns the host library handle and its mocgpu_problem view."""
+
+    def __init__(self, handle: P):
+        self.h = handle
+        self.view = host().mocc_host_view(self.h)
+        if not self.view:
+            raise RuntimeError("deck has no sweeper")
+        self.p = self.view.contents
+
+    @classmethod
+    def load(cls, path: str) -> "HostProblem":
+        h = P()
+        if host().mocc_host_load(path.encode(), C.byref(h)) != 0:
+            raise RuntimeError(host().mocc_host_last_error().decode())
+        return cls(h)
+
+    @classmethod
+    def from_string(cls, xml: str, base_dir: str = ".") -> "HostProblem":
+        h = P()
+        if host().mocc_host_load_string(xml.encode(), base_dir.encode(), C.byref(h)) != 0:
+            raise RuntimeError(host().mocc_host_last_error().decode())
+        return cls(h)
+
+    def params(self) -> HostParams:
+        out = HostParams()
+        host().mocc_host_get_params(self.h, C.byref(out))
+        return out
+
+    def options(self) -> MocgpuOptions:
+        out = MocgpuOptions()
+        host().mocc_host_get_options(self.h, C.byref(out))
+        return out
+
+    # numpy views of the traced arrays (copies)
+    def array(self, name: str, n: int, dtype=np.float64) -> np.ndarray:
+        p = getattr(self.p, name)
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+
+    @property
+    def n_fsr(self):
+        return self.p.n_fsr
+
+    @property
+    def n_group(self):
+        return self.p.n_group
+
+    def updates_per_sweep(self) -> int:
+        return 2 * self.p.n_polar * self.p.n_seg * self.p.n_group
+
+    def close(self):
+        if self.h:
+            host().mocc_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
