@@ -1,0 +1,74 @@
+"""In-tree build of the product libraries (mocc_amd/lib) and the oracle.
+
+``build_all()`` compiles:
+  * libmocc_host.so  -- C++17 host front-end (g++),
+  * libmocgpu.so     -- HIP for gfx950 (hipcc),
+  * oracle/_build/liboracle.so -- the FP64 checker (test infrastructure).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "mocc_amd")
+LIB = os.path.join(PKG, "lib")
+HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
+GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# No packed FP32 instructions (v_pk_*_f32) anywhere in device code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build_host():
+    os.makedirs(LIB, exist_ok=True)
+    src = [os.path.join(PKG, "csrc", "host", s) for s in HOST_SRC]
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
+          "-o", os.path.join(LIB, "libmocc_host.so.tmp")] + src)
+    os.replace(os.path.join(LIB, "libmocc_host.so.tmp"), os.path.join(LIB, "libmocc_host.so"))
+
+
+def build_gpu():
+    os.makedirs(LIB, exist_ok=True)
+    objdir = os.path.join(PKG, "csrc", "gpu", "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for s in GPU_SRC:
+        src = os.path.join(PKG, "csrc", "gpu", s)
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        if s.endswith(".hip"):
+            _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        else:
+            _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-c", src, "-o", obj])
+        objs.append(obj)
+    out = os.path.join(LIB, "libmocgpu.so")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
+    os.replace(out + ".tmp", out)
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all():
+    build_host()
+    build_gpu()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["all"]
+    for w in which:
+        {"host": build_host, "gpu": build_gpu, "oracle": build_oracle, "all": build_all}[w]()
+    print("built:", ", ".join(which))

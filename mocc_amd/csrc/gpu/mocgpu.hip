@@ -1,0 +1,948 @@
+// C ABI of the gfx950 MoC sweeper (include/mocgpu.h): device state, the
+// source / fission / reduction kernels and the host driver of the sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mocgpu.h"
+#include "cmfd.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace mocc_amd::gpu;
+
+namespace {
+
+constexpr double PI = 3.1415926535897932;
+constexpr double FPI = 4.0 * PI;
+constexpr int kRedBlocks = 256;  // fixed stage-1 grid: reductions are bitwise reproducible
+constexpr int kRedThreads = 256;
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define HIP_CHECK(expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+template <typename T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        free();
+        n = count;
+        if (count) HIP_CHECK(hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s)
+    {
+        alloc(v.size());
+        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void free()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// Source::initialize_group + fission + in_scatter (source.cpp:41-112):
+// S = chi*fs + sum_{g' != g} Ss[g'->g] phi_g'
+__global__ void k_build_source(int F, int G, int g_fixed, int has_fs, const int32_t *fsr_mat,
+                               const double *mat_ch, const double *mat_sc, const double *fs,
+                               const double *phi_src, double *src)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int f, g;
+    if (g_fixed >= 0) {
+        f = idx;
+        g = g_fixed;
+    } else {
+        f = idx / G;
+        g = idx % G;
+    }
+    if (f >= F) return;
+    const int m = fsr_mat[f];
+    double s = 0.0;
+    if (has_fs) s += mat_ch[m * G + g] * fs[f];
+    const double *row = mat_sc + ((size_t)m * G + g) * G;
+    for (int gg = 0; gg < G; gg++) {
+        if (gg == g) continue;
+        s += row[gg] * phi_src[(size_t)f * G + gg];
+    }
+    src[(size_t)f * G + g] = s;
+}
+
+// TransportSweeper::calc_fission_source (transport_sweeper.cpp:119-134)
+__global__ void k_fission_source(int F, int G, const int32_t *fsr_mat, const double *mat_nf, const double *phi,
+                                 const double *kptr, double kval, double *fs)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const double rk = 1.0 / (kptr ? kptr[0] : kval);
+    const int m = fsr_mat[f];
+    double s = 0.0;
+    for (int g = 0; g < G; g++) s += rk * mat_nf[m * G + g] * phi[(size_t)f * G + g];
+    fs[f] = s;
+}
+
+enum RedMode { RED_FISSION = 0, RED_NORM = 1, RED_PSIERR = 2, RED_RESID = 3 };
+
+struct RedArgs {
+    int F, G;
+    const int32_t *fsr_mat;
+    const double *mat_nf, *vol, *phi, *phi_old, *fs, *fs_prev;
+    double *scal;  // device scalars (see Scal)
+};
+
+// device scalar slots
+enum Scal { S_K = 0, S_KPREV, S_TF_NEW, S_TF_OLD, S_ERRK, S_ERRPSI, S_FA, S_FB, S_NFISS, S_RESID, S_RESERVED, S_COUNT };
+
+__device__ __forceinline__ void red_values(const RedArgs &a, int mode, int f, double v[4])
+{
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+    const int G = a.G;
+    if (mode == RED_FISSION) {
+        const int m = a.fsr_mat[f];
+        for (int g = 0; g < G; g++) {
+            v[0] += a.phi[(size_t)f * G + g] * a.vol[f] * a.mat_nf[m * G + g];
+            v[1] += a.phi_old[(size_t)f * G + g] * a.vol[f] * a.mat_nf[m * G + g];
+        }
+    } else if (mode == RED_NORM) {
+        v[0] = a.fs[f] > 0.0 ? 1.0 : 0.0;
+        v[1] = a.fs[f];
+        v[2] = a.fs_prev[f] > 0.0 ? 1.0 : 0.0;
+        v[3] = a.fs_prev[f];
+    } else if (mode == RED_PSIERR) {
+        const double e = a.fs[f] * a.scal[S_FA] - a.fs_prev[f] * a.scal[S_FB];
+        v[0] = e * e;
+    } else {
+        for (int g = 0; g < G; g++) {
+            const double e = a.phi[(size_t)f * G + g] - a.phi_old[(size_t)f * G + g];
+            v[0] += e * e;
+        }
+    }
+}
+
+__device__ __forceinline__ void block_sum4(double v[4], double (*sh)[4])
+{
+    const int t = threadIdx.x;
+    for (int q = 0; q < 4; q++) sh[t][q] = v[q];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 4; q++) sh[t][q] += sh[t + s][q];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_red_stage1(RedArgs a, int mode, double *partial)
+{
+    __shared__ double sh[kRedThreads][4];
+    double acc[4] = {0, 0, 0, 0};
+    const int stride = gridDim.x * blockDim.x;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < a.F; f += stride) {
+        double v[4];
+        red_values(a, mode, f, v);
+        for (int q = 0; q < 4; q++) acc[q] += v[q];
+    }
+    block_sum4(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) partial[blockIdx.x * 4 + q] = sh[0][q];
+}
+
+// Stage 2 (one block) and the epilogues of EigenSolver::step / solve.
+__global__ __launch_bounds__(kRedThreads) void k_red_stage2(RedArgs a, int mode, const double *partial, int nb,
+                                                            int update_k, double *out)
+{
+    __shared__ double sh[kRedThreads][4];
+    double acc[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int q = 0; q < 4; q++) acc[q] += partial[b * 4 + q];
+    block_sum4(acc, sh);
+    if (threadIdx.x != 0) return;
+    double *s = a.scal;
+    if (mode == RED_FISSION) {
+        s[S_TF_NEW] = sh[0][0];
+        s[S_TF_OLD] = sh[0][1];
+        if (update_k) {
+            // k <- k * tfis(new) / tfis(old) (eigen_solver.cpp:242-247)
+            s[S_KPREV] = s[S_K];
+            s[S_K] = s[S_K] * sh[0][0] / sh[0][1];
+            s[S_ERRK] = fabs(s[S_K] - s[S_KPREV]);
+        }
+    } else if (mode == RED_NORM) {
+        // Normalize: scale to sum = number of positive entries (util/utils.hpp:28-51)
+        s[S_FA] = sh[0][0] / sh[0][1];
+        s[S_FB] = sh[0][2] / sh[0][3];
+    } else if (mode == RED_PSIERR) {
+        s[S_ERRPSI] = sqrt(sh[0][0] / s[S_NFISS]);
+    } else {
+        s[S_RESID] = sqrt(sh[0][0]);
+    }
+    if (out)
+        for (int q = 0; q < 4; q++) out[q] = sh[0][q];
+}
+
+__global__ void k_copy(const double *a, double *b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_fill_f(float *a, float v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Currents of the coarse surfaces: sum the band slots, normalize by area
+// (moc::Current::post_sweep, moc_current_worker.hpp:272-318). Surfaces no ray
+// crosses end up zero, as after CoarseData::zero_data_radial.
+__global__ void k_reduce_current(int n_surf_xy0, int n_surf_xy1, int G, int g_fixed, const int32_t *ptr,
+                                 const int32_t *idx, const double *csum, const double *ssum, const double *area,
+                                 double *cur, double *sfl)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int s, g;
+    const int n = n_surf_xy1 - n_surf_xy0;
+    if (g_fixed >= 0) {
+        s = t;
+        g = g_fixed;
+    } else {
+        s = t / G;
+        g = t % G;
+    }
+    if (s >= n) return;
+    s += n_surf_xy0;
+    double c = 0.0, f = 0.0;
+    for (int k = ptr[s]; k < ptr[s + 1]; k++) {
+        c += csum[(size_t)idx[k] * G + g];
+        f += ssum[(size_t)idx[k] * G + g];
+    }
+    cur[(size_t)s * G + g] = c / area[s];
+    sfl[(size_t)s * G + g] = f / area[s];
+}
+
+// MoCSweeper::get_pin_flux_1g, PIN_PLANE (moc_sweeper.cpp:293-326)
+__global__ void k_pin_flux(int npin, int G, int g, const int32_t *pin_off, const int32_t *pin_cell,
+                           const double *vol, const double *phi, double *pin)
+{
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= npin) return;
+    double v = 0.0, pf = 0.0;
+    for (int r = pin_off[ip]; r < pin_off[ip + 1]; r++) {
+        v += vol[r];
+        pf += phi[(size_t)r * G + g] * vol[r];
+    }
+    pin[pin_cell[ip]] = pf / v;
+}
+
+// MoCSweeper::set_pin_flux_1g, PIN_PLANE (moc_sweeper.cpp:396-426): scale the
+// pin's fine fluxes by new/old pin average; e^2 per pin for the residual.
+__global__ void k_set_pin_flux(int npin, int G, int g, const int32_t *pin_off, const int32_t *pin_cell,
+                               const double *area, const double *pin_area, const double *pin_new, double *phi,
+                               double *e2)
+{
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= npin) return;
+    const int ic = pin_cell[ip];
+    double fm = 0.0;
+    for (int r = pin_off[ip]; r < pin_off[ip + 1]; r++) fm += phi[(size_t)r * G + g] * area[r];
+    fm /= pin_area[ip];
+    const double e = pin_new[ic] - fm;
+    const double fct = pin_new[ic] / fm;
+    for (int r = pin_off[ip]; r < pin_off[ip + 1]; r++) phi[(size_t)r * G + g] *= fct;
+    e2[ip] = e * e;
+}
+
+inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct mocgpu_handle {
+    std::string err;
+    mocgpu_options opt{};
+    SweepPlan plan;
+    int F = 0, G = 0, NP = 0, A = 0, nd = 0, nx = 0, ny = 0, npin = 0, n_surf = 0, n_mat = 0;
+    int n_fissile = 0;
+    int64_t n_seg = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t parity = 0;
+    bool has_fs = false;
+    bool has_radial = false;
+
+    // plan
+    DBuf<BandInfo> d_bands;
+    DBuf<int32_t> d_order, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
+    DBuf<uint16_t> d_seg_loc;
+    DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw, d_exptab;
+    DBuf<RayInfo> d_rays;
+    DBuf<AziInfo> d_azi;
+    DBuf<uint32_t> d_ev;
+    DBuf<double> d_surf_area;
+    // cross sections and mesh
+    DBuf<int32_t> d_fsr_mat, d_pin_off, d_pin_cell;
+    DBuf<double> d_mat_tr, d_mat_ss, d_mat_nf, d_mat_ch, d_mat_sc, d_mat_f, d_vol, d_area, d_pin_area;
+    // state
+    DBuf<double> d_phi, d_phi_old, d_src, d_fs, d_fs_prev, d_psum, d_csum, d_ssum, d_cur, d_sfl;
+    DBuf<float> d_psi;
+    DBuf<double> d_partial, d_scal, d_red_out, d_pin, d_e2;
+    std::unique_ptr<CmfdSolver> cmfd;
+
+    // host copies
+    std::vector<float> bc_template;  // one group of incoming values after initialize
+    std::vector<int32_t> h_fsr_mat;
+
+    ~mocgpu_handle()
+    {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    SweepArgs sweep_args(int g_fixed) const
+    {
+        SweepArgs a{};
+        a.bands = d_bands.p;
+        a.order = d_order.p;
+        a.n_bands = (int)plan.bands.size();
+        a.g_fixed = g_fixed;
+        a.G = G;
+        a.F = F;
+        a.bcpg = plan.bcpg;
+        a.parity = parity;
+        a.seg_loc = d_seg_loc.p;
+        a.seg_len = d_seg_len.p;
+        a.loc_fsr = d_loc_fsr.p;
+        a.rays = d_rays.p;
+        a.azi = d_azi.p;
+        a.ang_rsin = d_ang_rsin.p;
+        a.ang_wt = d_ang_wt.p;
+        a.ang_cw = d_ang_cw.p;
+        a.ang_fw = d_ang_fw.p;
+        a.exptab = (const float2 *)d_exptab.p;
+        a.fsr_mat = d_fsr_mat.p;
+        a.mat_tr = d_mat_tr.p;
+        a.mat_ss = d_mat_ss.p;
+        a.src = d_src.p;
+        a.phi = d_phi.p;
+        a.psi = d_psi.p;
+        a.psum = d_psum.p;
+        a.ev_off = d_ev_off.p;
+        a.ev = d_ev.p;
+        a.csum = d_csum.p;
+        a.ssum = d_ssum.p;
+        return a;
+    }
+
+    ReduceArgs reduce_args(int g_fixed) const
+    {
+        ReduceArgs r{};
+        r.F = F;
+        r.G = G;
+        r.g_fixed = g_fixed;
+        r.slot_ptr = d_slot_ptr.p;
+        r.slot_idx = d_slot_idx.p;
+        r.psum = d_psum.p;
+        r.fsr_mat = d_fsr_mat.p;
+        r.mat_tr = d_mat_tr.p;
+        r.mat_ss = d_mat_ss.p;
+        r.vol = d_vol.p;
+        r.src = d_src.p;
+        r.phi = d_phi.p;
+        return r;
+    }
+
+    RedArgs red_args() const
+    {
+        RedArgs a{};
+        a.F = F;
+        a.G = G;
+        a.fsr_mat = d_fsr_mat.p;
+        a.mat_nf = d_mat_nf.p;
+        a.vol = d_vol.p;
+        a.phi = d_phi.p;
+        a.phi_old = d_phi_old.p;
+        a.fs = d_fs.p;
+        a.fs_prev = d_fs_prev.p;
+        a.scal = d_scal.p;
+        return a;
+    }
+
+    void reduce(int mode, int update_k)
+    {
+        RedArgs a = red_args();
+        hipLaunchKernelGGL(k_red_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, mode, d_partial.p);
+        hipLaunchKernelGGL(k_red_stage2, dim3(1), dim3(kRedThreads), 0, stream, a, mode, d_partial.p, kRedBlocks,
+                           update_k, d_red_out.p);
+    }
+
+    void build_source(int g_fixed, bool jacobi)
+    {
+        size_t n = g_fixed >= 0 ? (size_t)F : (size_t)F * G;
+        hipLaunchKernelGGL(k_build_source, dim3(nblk(n)), dim3(256), 0, stream, F, G, g_fixed, has_fs ? 1 : 0,
+                           d_fsr_mat.p, d_mat_ch.p, d_mat_sc.p, d_fs.p, jacobi ? d_phi_old.p : d_phi.p,
+                           d_src.p);
+    }
+
+    // one inner iteration of sweep1g for group g (g < 0: all groups)
+    void sweep_once(int g, bool with_current)
+    {
+        SweepArgs a = sweep_args(g);
+        int items = (int)plan.bands.size() * (g >= 0 ? 1 : G);
+        launch_sweep(a, NP, with_current, items, stream);
+        launch_reduce_flux(reduce_args(g), stream);
+        if (with_current) {
+            int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
+            size_t n = (size_t)(n1 - n0) * (g >= 0 ? 1 : G);
+            hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, d_surf_ptr.p,
+                               d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
+            has_radial = true;
+        }
+        if (g >= 0) parity ^= (1ull << g);
+        else parity ^= (G >= 64 ? ~0ull : ((1ull << G) - 1ull));
+    }
+
+    // MoCSweeper::sweep (moc_sweeper.cpp:189-225)
+    void mocsweep(int g, bool with_current)
+    {
+        for (int inner = 0; inner < opt.n_inner; inner++)
+            sweep_once(g, with_current && inner == opt.n_inner - 1);
+    }
+
+    void copy_flux_to_old()
+    {
+        size_t n = (size_t)F * G;
+        hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, stream, d_phi.p, d_phi_old.p, n);
+    }
+
+    // FixedSourceSolver::step (fixed_source_solver.cpp:102-117)
+    void fss_step(bool with_current)
+    {
+        copy_flux_to_old();
+        if (opt.energy_mode == 1) {
+            build_source(-1, true);
+            mocsweep(-1, with_current);
+        } else {
+            for (int g = 0; g < G; g++) {
+                build_source(g, false);
+                mocsweep(g, with_current);
+            }
+        }
+    }
+
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+template <typename F> int guard(mocgpu_handle *h, F &&f)
+{
+    try {
+        int rc = f();
+        return rc;
+    } catch (const HipError &e) {
+        if (h) h->err = e.what();
+        return MOCGPU_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        if (h) h->err = e.what();
+        return MOCGPU_ERR_ARG;
+    }
+}
+
+thread_local std::string g_create_err;
+}  // namespace
+
+extern "C" {
+
+void mocgpu_default_options(mocgpu_options *o)
+{
+    *o = mocgpu_options{};
+    o->n_inner = 1;
+    o->energy_mode = 0;
+    o->lane_map = -1;
+    o->band_rays = 0;
+    o->use_cmfd = 0;
+    o->cmfd_max_iter = 100;
+    o->cmfd_k_tol = 1.0e-6;
+    o->cmfd_psi_tol = 1.0e-5;
+    o->cmfd_resid_reduction = 0.001;
+    o->cmfd_solver = 0;
+}
+
+const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
+{
+    if (!p || !out) return MOCGPU_ERR_ARG;
+    auto h = std::make_unique<mocgpu_handle>();
+    int rc = guard(h.get(), [&] {
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw HipError("no HIP device");
+        if (opt) h->opt = *opt;
+        else mocgpu_default_options(&h->opt);
+        if (p->n_group > 64) throw std::runtime_error("at most 64 energy groups are supported");
+        h->plan = build_plan(*p, h->opt.band_rays);
+        const SweepPlan &pl = h->plan;
+        h->F = p->n_fsr;
+        h->G = p->n_group;
+        h->NP = p->n_polar;
+        h->A = p->n_azi;
+        h->nd = pl.nd;
+        h->nx = p->nx;
+        h->ny = p->ny;
+        h->npin = p->nx * p->ny;
+        h->n_surf = pl.n_surf;
+        h->n_mat = p->n_mat;
+        h->n_seg = p->n_seg;
+        HIP_CHECK(hipStreamCreate(&h->stream));
+        HIP_CHECK(hipEventCreate(&h->ev0));
+        HIP_CHECK(hipEventCreate(&h->ev1));
+        hipStream_t s = h->stream;
+        h->d_bands.upload(pl.bands, s);
+        h->d_order.upload(pl.band_order, s);
+        h->d_seg_loc.upload(pl.seg_loc, s);
+        h->d_seg_len.upload(pl.seg_len, s);
+        h->d_loc_fsr.upload(pl.loc_fsr, s);
+        h->d_rays.upload(pl.rays, s);
+        h->d_azi.upload(pl.azi, s);
+        h->d_ang_rsin.upload(pl.ang_rsin, s);
+        h->d_ang_wt.upload(pl.ang_wt, s);
+        h->d_ang_cw.upload(pl.ang_cw, s);
+        h->d_ang_fw.upload(pl.ang_fw, s);
+        h->d_slot_ptr.upload(pl.fsr_slot_ptr, s);
+        h->d_slot_idx.upload(pl.fsr_slot_idx, s);
+        h->d_ev_off.upload(pl.ev_off, s);
+        h->d_ev.upload(pl.ev, s);
+        h->d_surf_ptr.upload(pl.surf_slot_ptr, s);
+        h->d_surf_idx.upload(pl.surf_slot_idx, s);
+        h->d_surf_area.upload(pl.surf_area, s);
+        std::vector<float> tab;
+        exp_table_fill(tab);
+        h->d_exptab.upload(tab, s);
+        const int F = h->F, G = h->G;
+        std::vector<int32_t> fm(p->fsr_mat, p->fsr_mat + F);
+        h->h_fsr_mat = fm;
+        h->d_fsr_mat.upload(fm, s);
+        auto vecd = [](const double *x, size_t n) { return std::vector<double>(x, x + n); };
+        h->d_mat_tr.upload(vecd(p->xs_tr, (size_t)p->n_mat * G), s);
+        h->d_mat_nf.upload(vecd(p->xs_nf, (size_t)p->n_mat * G), s);
+        h->d_mat_f.upload(vecd(p->xs_f, (size_t)p->n_mat * G), s);
+        h->d_mat_ch.upload(vecd(p->xs_ch, (size_t)p->n_mat * G), s);
+        h->d_mat_sc.upload(vecd(p->xs_sc, (size_t)p->n_mat * G * G), s);
+        std::vector<double> ss((size_t)p->n_mat * G);
+        for (int m = 0; m < p->n_mat; m++)
+            for (int g = 0; g < G; g++) ss[m * G + g] = p->xs_sc[((size_t)m * G + g) * G + g];
+        h->d_mat_ss.upload(ss, s);
+        h->d_vol.upload(vecd(p->fsr_vol, F), s);
+        h->d_area.upload(vecd(p->fsr_area, F), s);
+        h->d_pin_off.upload(std::vector<int32_t>(p->pin_fsr_off, p->pin_fsr_off + h->npin + 1), s);
+        h->d_pin_cell.upload(std::vector<int32_t>(p->pin_cell, p->pin_cell + h->npin), s);
+        h->d_pin_area.upload(vecd(p->pin_area, h->npin), s);
+        // state
+        h->d_phi.alloc((size_t)F * G);
+        h->d_phi_old.alloc((size_t)F * G);
+        h->d_src.alloc((size_t)F * G);
+        h->d_fs.alloc(F);
+        h->d_fs_prev.alloc(F);
+        h->d_psi.alloc((size_t)G * 2 * pl.bcpg);
+        h->d_psum.alloc(pl.loc_fsr.size() * (size_t)G);
+        h->d_csum.alloc(std::max<size_t>(1, pl.lsurf_glob.size()) * (size_t)G);
+        h->d_ssum.alloc(std::max<size_t>(1, pl.lsurf_glob.size()) * (size_t)G);
+        h->d_cur.alloc((size_t)h->n_surf * G);
+        h->d_sfl.alloc((size_t)h->n_surf * G);
+        h->d_partial.alloc(kRedBlocks * 4);
+        h->d_scal.alloc(S_COUNT);
+        h->d_red_out.alloc(4);
+        h->d_pin.alloc(h->npin);
+        h->d_e2.alloc(h->npin);
+        HIP_CHECK(hipMemsetAsync(h->d_src.p, 0, sizeof(double) * F * G, s));
+        HIP_CHECK(hipMemsetAsync(h->d_fs.p, 0, sizeof(double) * F, s));
+        HIP_CHECK(hipMemsetAsync(h->d_cur.p, 0, sizeof(double) * h->n_surf * G, s));
+        HIP_CHECK(hipMemsetAsync(h->d_sfl.p, 0, sizeof(double) * h->n_surf * G, s));
+        HIP_CHECK(hipMemsetAsync(h->d_psi.p, 0, sizeof(float) * G * 2 * pl.bcpg, s));
+        // number of fissile FSRs (eigen_solver.cpp:97-111)
+        int nfis = 0;
+        for (int f = 0; f < F; f++) {
+            int m = fm[f];
+            for (int g = 0; g < G; g++)
+                if (p->xs_nf[m * G + g] > 0.0) {
+                    nfis++;
+                    break;
+                }
+        }
+        h->n_fissile = nfis;
+        std::vector<double> sc(S_COUNT, 0.0);
+        sc[S_K] = 1.0;
+        sc[S_KPREV] = 1.0;
+        sc[S_NFISS] = (double)nfis;
+        h->d_scal.upload(sc, s);
+        // template of the initial incoming flux: 1/(4 pi) on non-vacuum faces
+        // (MoCSweeper::initialize + BoundaryCondition::initialize_scalar)
+        h->bc_template.assign(pl.bcpg, 0.0f);
+        const float val = (float)(1.0 / FPI);
+        for (int iang = 0; iang < 4 * pl.nd; iang++) {
+            int ioct = iang / pl.nd;
+            int i1 = iang < 2 * pl.nd ? iang : iang - 2 * pl.nd;
+            int a = i1 / pl.NP;
+            int ny_ = p->azi_ny[a], nx_ = p->azi_nx[a];
+            bool ox_pos = (ioct == 0 || ioct == 3), oy_pos = (ioct == 0 || ioct == 1);
+            int sx = ox_pos ? MOCGPU_WEST : MOCGPU_EAST;
+            int sy = oy_pos ? MOCGPU_SOUTH : MOCGPU_NORTH;
+            if (p->bc[sx] == MOCGPU_BC_REFLECT)
+                for (int i = 0; i < ny_; i++) h->bc_template[pl.ang_off[iang] + i] = val;
+            if (p->bc[sy] == MOCGPU_BC_REFLECT)
+                for (int i = 0; i < nx_; i++) h->bc_template[pl.ang_off[iang] + ny_ + i] = val;
+        }
+        h->cmfd = std::make_unique<CmfdSolver>();
+        h->cmfd->setup(*p, h->opt, s);
+        h->sync();
+        return 0;
+    });
+    if (rc != 0) {
+        g_create_err = h->err;
+        return rc;
+    }
+    *out = h.release();
+    return 0;
+}
+
+int mocgpu_destroy(mocgpu_handle *h)
+{
+    delete h;
+    return 0;
+}
+
+int mocgpu_initialize(mocgpu_handle *h)
+{
+    return guard(h, [&] {
+        std::vector<double> ones((size_t)h->F * h->G, 1.0);
+        HIP_CHECK(hipMemcpyAsync(h->d_phi.p, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(hipMemcpyAsync(h->d_phi_old.p, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, h->stream));
+        for (int g = 0; g < h->G; g++)
+            for (int b = 0; b < 2; b++)
+                HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + b) * h->plan.bcpg, h->bc_template.data(),
+                                         h->bc_template.size() * 4, hipMemcpyHostToDevice, h->stream));
+        h->parity = 0;
+        h->cmfd->initialize(h->stream);
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_set_flux(mocgpu_handle *h, const double *phi)
+{
+    return guard(h, [&] {
+        HIP_CHECK(hipMemcpyAsync(h->d_phi.p, phi, sizeof(double) * h->F * h->G, hipMemcpyHostToDevice, h->stream));
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_get_flux(mocgpu_handle *h, double *phi)
+{
+    return guard(h, [&] {
+        HIP_CHECK(hipMemcpyAsync(phi, h->d_phi.p, sizeof(double) * h->F * h->G, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        return 0;
+    });
+}
+
+int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return (int64_t)h->G * h->plan.bcpg; }
+
+int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
+{
+    return guard(h, [&] {
+        if (n != (int64_t)h->G * h->plan.bcpg) throw std::runtime_error("boundary size mismatch");
+        std::vector<float> tmp(h->plan.bcpg);
+        for (int g = 0; g < h->G; g++) {
+            int par = (int)((h->parity >> g) & 1ull);
+            HIP_CHECK(hipMemcpyAsync(tmp.data(), h->d_psi.p + ((size_t)g * 2 + par) * h->plan.bcpg, tmp.size() * 4,
+                                     hipMemcpyDeviceToHost, h->stream));
+            h->sync();
+            for (int i = 0; i < h->plan.bcpg; i++) bc[(size_t)g * h->plan.bcpg + i] = tmp[i];
+        }
+        return 0;
+    });
+}
+
+int mocgpu_set_boundary(mocgpu_handle *h, const double *bc, int64_t n)
+{
+    return guard(h, [&] {
+        if (n != (int64_t)h->G * h->plan.bcpg) throw std::runtime_error("boundary size mismatch");
+        std::vector<float> tmp(h->plan.bcpg);
+        for (int g = 0; g < h->G; g++) {
+            for (int i = 0; i < h->plan.bcpg; i++) tmp[i] = (float)bc[(size_t)g * h->plan.bcpg + i];
+            int par = (int)((h->parity >> g) & 1ull);
+            HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + par) * h->plan.bcpg, tmp.data(), tmp.size() * 4,
+                                     hipMemcpyHostToDevice, h->stream));
+            h->sync();
+        }
+        return 0;
+    });
+}
+
+int mocgpu_store_old_flux(mocgpu_handle *h)
+{
+    return guard(h, [&] {
+        h->copy_flux_to_old();
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_calc_fission_source(mocgpu_handle *h, double k, double *fs_out)
+{
+    return guard(h, [&] {
+        hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G,
+                           h->d_fsr_mat.p, h->d_mat_nf.p, h->d_phi.p, (const double *)nullptr, k, h->d_fs.p);
+        h->has_fs = true;
+        if (fs_out)
+            HIP_CHECK(hipMemcpyAsync(fs_out, h->d_fs.p, sizeof(double) * h->F, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_set_fission_source(mocgpu_handle *h, const double *fs)
+{
+    return guard(h, [&] {
+        HIP_CHECK(hipMemcpyAsync(h->d_fs.p, fs, sizeof(double) * h->F, hipMemcpyHostToDevice, h->stream));
+        h->has_fs = true;
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_build_source(mocgpu_handle *h, int32_t g)
+{
+    return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        h->build_source(g, h->opt.energy_mode == 1);
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current)
+{
+    return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        h->mocsweep(g, with_current != 0);
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current)
+{
+    return guard(h, [&] {
+        h->fss_step(with_current != 0);
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out)
+{
+    return guard(h, [&] {
+        h->reduce(RED_FISSION, 0);
+        double v[4];
+        HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        *out = old ? v[1] : v[0];
+        return 0;
+    });
+}
+
+int mocgpu_flux_residual(mocgpu_handle *h, double *out)
+{
+    return guard(h, [&] {
+        h->reduce(RED_RESID, 0);
+        double v[4];
+        HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        *out = std::sqrt(v[0]);
+        return 0;
+    });
+}
+
+int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
+{
+    return guard(h, [&] {
+        hipLaunchKernelGGL(k_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
+                           h->d_pin_off.p, h->d_pin_cell.p, h->d_vol.p, h->d_phi.p, h->d_pin.p);
+        HIP_CHECK(hipMemcpyAsync(pin, h->d_pin.p, sizeof(double) * h->npin, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin, double *resid)
+{
+    for (int i = 0; i < h->npin; i++)
+        if (!(pin[i] > 0.0)) {
+            h->err = "Negative or zero input flux: " + std::to_string(pin[i]);
+            return MOCGPU_ERR_NONPOS_FLUX;
+        }
+    return guard(h, [&] {
+        HIP_CHECK(hipMemcpyAsync(h->d_pin.p, pin, sizeof(double) * h->npin, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_set_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
+                           h->d_pin_off.p, h->d_pin_cell.p, h->d_area.p, h->d_pin_area.p, h->d_pin.p, h->d_phi.p,
+                           h->d_e2.p);
+        std::vector<double> e2(h->npin);
+        HIP_CHECK(hipMemcpyAsync(e2.data(), h->d_e2.p, sizeof(double) * h->npin, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        double r = 0.0;
+        for (double v : e2) r += v;
+        if (resid) *resid = std::sqrt(r) / h->npin;
+        return 0;
+    });
+}
+
+int32_t mocgpu_n_surf(const mocgpu_handle *h) { return h->n_surf; }
+
+int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *surface_flux)
+{
+    return guard(h, [&] {
+        std::vector<double> c((size_t)h->n_surf * h->G), s((size_t)h->n_surf * h->G);
+        HIP_CHECK(hipMemcpyAsync(c.data(), h->d_cur.p, c.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(hipMemcpyAsync(s.data(), h->d_sfl.p, s.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        for (int i = 0; i < h->n_surf; i++) {
+            if (current) current[i] = c[(size_t)i * h->G + g];
+            if (surface_flux) surface_flux[i] = s[(size_t)i * h->G + g];
+        }
+        return 0;
+    });
+}
+
+int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return 2LL * h->NP * h->n_seg * h->G; }
+
+int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double *kernel_ms)
+{
+    return guard(h, [&] {
+        SweepArgs a = h->sweep_args(-1);
+        int items = (int)h->plan.bands.size() * h->G;
+        // warm-up
+        launch_sweep(a, h->NP, false, items, h->stream);
+        launch_reduce_flux(h->reduce_args(-1), h->stream);
+        h->sync();
+        float ksum = 0.0f;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < n; i++) {
+            a.parity = h->parity;
+            HIP_CHECK(hipEventRecord(h->ev0, h->stream));
+            launch_sweep(a, h->NP, false, items, h->stream);
+            HIP_CHECK(hipEventRecord(h->ev1, h->stream));
+            launch_reduce_flux(h->reduce_args(-1), h->stream);
+            HIP_CHECK(hipEventSynchronize(h->ev1));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+            ksum += ms;
+        }
+        h->sync();
+        auto t1 = std::chrono::steady_clock::now();
+        *ms_per_sweep = std::chrono::duration<double, std::milli>(t1 - t0).count() / n;
+        *kernel_ms = ksum / n;
+        return 0;
+    });
+}
+
+// EigenSolver::step + convergence bookkeeping (eigen_solver.cpp:162-253),
+// device-resident: k lives in device memory, the host reads the scalars back.
+int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
+{
+    return guard(h, [&] {
+        if (h->opt.use_cmfd) {
+            double k = st->k;
+            int rc = h->cmfd->step(*h, k);
+            if (rc) return rc;
+            HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, &k, 8, hipMemcpyHostToDevice, h->stream));
+        }
+        hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G, h->d_fsr_mat.p,
+                           h->d_mat_nf.p, h->d_phi.p, h->d_scal.p + S_K, 0.0, h->d_fs.p);
+        h->has_fs = true;
+        hipLaunchKernelGGL(k_copy, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->d_fs.p, h->d_fs_prev.p,
+                           (size_t)h->F);
+        h->fss_step(h->opt.use_cmfd != 0);
+        h->reduce(RED_FISSION, 1);
+        hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G, h->d_fsr_mat.p,
+                           h->d_mat_nf.p, h->d_phi.p, h->d_scal.p + S_K, 0.0, h->d_fs.p);
+        h->reduce(RED_NORM, 0);
+        h->reduce(RED_PSIERR, 0);
+        double sc[S_COUNT];
+        HIP_CHECK(hipMemcpyAsync(sc, h->d_scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        st->k = sc[S_K];
+        st->k_prev = sc[S_KPREV];
+        st->err_k = sc[S_ERRK];
+        st->err_psi = sc[S_ERRPSI];
+        st->iterations++;
+        if (st->k != st->k) {
+            h->err = "Eigenvalue is not a number. Giving up.";
+            return MOCGPU_ERR_NAN;
+        }
+        return 0;
+    });
+}
+
+// EigenSolver::solve (eigen_solver.cpp:131-223)
+int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter, int32_t min_iter,
+                       mocgpu_eigen_state *st)
+{
+    int rc = mocgpu_initialize(h);
+    if (rc) return rc;
+    *st = mocgpu_eigen_state{};
+    st->k = 1.0;
+    st->k_prev = 1.0;
+    rc = guard(h, [&] {
+        std::vector<double> sc(S_COUNT, 0.0);
+        sc[S_K] = 1.0;
+        sc[S_KPREV] = 1.0;
+        sc[S_NFISS] = (double)h->n_fissile;
+        HIP_CHECK(hipMemcpyAsync(h->d_scal.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, h->stream));
+        h->sync();
+        return 0;
+    });
+    if (rc) return rc;
+    for (int n = 0; n < max_iter; n++) {
+        rc = mocgpu_eigen_iterate(h, st);
+        if (rc) return rc;
+        if (st->err_k < k_tol && st->err_psi < psi_tol && n >= min_iter) {
+            st->converged = 1;
+            return 0;
+        }
+    }
+    h->err = "Maximum number of iterations reached!";
+    return MOCGPU_ERR_MAXITER;
+}
+
+int mocgpu_cmfd_step(mocgpu_handle *h, double *k)
+{
+    return guard(h, [&] { return h->cmfd->step(*h, *k); });
+}
+
+}  // extern "C"

@@ -1,0 +1,385 @@
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+namespace mocc_amd {
+namespace gpu {
+
+namespace {
+constexpr double PI = 3.1415926535897932;
+enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5, INVALID = 7 };
+
+int reflect(int iang, int normal, int nd)
+{
+    static const int refl[3][8] = {{1, 0, 3, 2, 5, 4, 7, 6}, {3, 2, 1, 0, 7, 6, 5, 4}, {4, 5, 6, 7, 0, 1, 2, 3}};
+    int ioct = iang / nd;
+    return iang + (refl[normal][ioct] - ioct) * nd;
+}
+
+// Mesh::coarse_surf / coarse_neighbor / surface_normal for one plane
+struct Coarse {
+    int nx, ny;
+    int surf(int cell, int s) const
+    {
+        int ix = cell % nx, iy = cell / nx;
+        switch (s) {
+        case EAST: return (nx + 1) * iy + ix + 1 + nx * ny;
+        case WEST: return (nx + 1) * iy + ix + nx * ny;
+        case NORTH: return nx * ny + (nx + 1) * ny + (ny + 1) * ix + iy + 1;
+        case SOUTH: return (nx + 1) * ny + (ny + 1) * ix + iy + nx * ny;
+        }
+        throw std::runtime_error("coarse crossing on a z-normal surface");
+    }
+    int neighbor(int cell, int s) const
+    {
+        int ix = cell % nx, iy = cell / nx;
+        switch (s) {
+        case NORTH: return (iy < ny - 1) ? cell + nx : -1;
+        case SOUTH: return (iy > 0) ? cell - nx : -1;
+        case EAST: return (ix < nx - 1) ? cell + 1 : -1;
+        case WEST: return (ix > 0) ? cell - 1 : -1;
+        }
+        return -5;
+    }
+    int normal(int s) const
+    {
+        int nsurfz = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
+        int r = s % nsurfz;
+        if (r < nx * ny) return 2;
+        if (r < nx * ny + (nx + 1) * ny) return 0;
+        return 1;
+    }
+};
+
+struct Event {
+    int step, surf, normal;
+};
+
+// moc::Current::post_ray (moc_current_worker.hpp:202-264) flattened: the
+// (segment position, surface, normal) of every tally, in the order the
+// direction meets them.
+void ray_events(const mocgpu_problem &p, const Coarse &cm, int r, int nseg, std::vector<Event> &fw,
+                std::vector<Event> &bw)
+{
+    int cell_fw = p.ray_cm_cell[2 * r], cell_bw = p.ray_cm_cell[2 * r + 1];
+    int surf_fw = p.ray_cm_surf[2 * r], surf_bw = p.ray_cm_surf[2 * r + 1];
+    int iseg_fw = 0, iseg_bw = nseg;
+    fw.push_back({iseg_fw, surf_fw, cm.normal(surf_fw)});
+    bw.push_back({nseg - iseg_bw, surf_bw, cm.normal(surf_bw)});
+    for (int c = p.ray_cm_off[r]; c < p.ray_cm_off[r + 1]; c++) {
+        const int *crd = &p.cm_data[4 * c];
+        if (crd[0] != INVALID) {
+            iseg_fw += crd[2];
+            int s = cm.surf(cell_fw, crd[0]);
+            fw.push_back({iseg_fw, s, (crd[0] == EAST || crd[0] == WEST) ? 0 : 1});
+        }
+        if (crd[1] != INVALID) {
+            iseg_bw -= crd[3];
+            int s = cm.surf(cell_bw, crd[1]);
+            bw.push_back({nseg - iseg_bw, s, (crd[1] == EAST || crd[1] == WEST) ? 0 : 1});
+        }
+        cell_fw = cm.neighbor(cell_fw, crd[0]);
+        cell_bw = cm.neighbor(cell_bw, crd[1]);
+    }
+}
+}  // namespace
+
+// Exponential_Linear<10000> (exponential.hpp:56-112) interpolates exp on
+// the nodes x_i = -10 + i/1000 and the sweep uses e = 1 - exp(-tau). Writing
+// the same piecewise-linear interpolant in terms of w = 1000*tau with nodes
+// H_j = 1 - exp(x_{N-j}) gives e = H_j + (H_{j+1} - H_j) * (w - j), j = floor(w),
+// which FP32 evaluates without the cancellation of 1 - (value near 1).
+void exp_table_fill(std::vector<float> &t)
+{
+    const int N = kExpN;
+    const double mn = -10.0, mx = 0.0;
+    const double space = (mx - mn) / (double)N;
+    std::vector<double> d(N + 1);
+    for (int i = 0; i <= N; i++) d[i] = std::exp(mn + i * space);
+    t.assign(2 * N, 0.0f);
+    for (int j = 0; j < N; j++) {
+        double h0 = 1.0 - d[N - j];
+        double h1 = 1.0 - d[N - j - 1];
+        t[2 * j] = (float)h0;
+        t[2 * j + 1] = (float)(h1 - h0);
+    }
+}
+
+SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
+{
+    if (max_rays <= 0 || max_rays > kMaxRaysPerBand) max_rays = kMaxRaysPerBand;
+    SweepPlan pl;
+    pl.F = p.n_fsr;
+    pl.G = p.n_group;
+    pl.NP = p.n_polar;
+    pl.A = p.n_azi;
+    pl.nd = p.n_azi * p.n_polar / 2;
+    if (p.n_azi % 2 != 0 || pl.nd <= 0) throw std::runtime_error("n_azi must be even and positive");
+    if (p.n_polar < 1 || p.n_polar > 4) throw std::runtime_error("1 to 4 polar angles per azimuth are supported");
+    for (int s = 0; s < 4; s++)
+        if (p.bc[s] != MOCGPU_BC_VACUUM && p.bc[s] != MOCGPU_BC_REFLECT)
+            throw std::runtime_error("only vacuum and reflective radial boundaries are supported");
+    const int nd = pl.nd, NP = pl.NP, A = pl.A, n_az_oct = A / 2;
+    const double hx = p.x_vec[p.nx], hy = p.y_vec[p.ny];
+
+    // boundary layout (moc_sweeper.cpp:38-52, boundary_condition.cpp:33-78)
+    pl.ang_off.assign(4 * nd, 0);
+    {
+        int acc = 0;
+        for (int iang = 0; iang < 4 * nd; iang++) {
+            int i1 = iang < 2 * nd ? iang : iang - 2 * nd;
+            int a = i1 / NP;
+            pl.ang_off[iang] = acc;
+            acc += p.azi_ny[a] + p.azi_nx[a];
+        }
+        pl.bcpg = acc;
+    }
+    pl.azi.resize(A);
+    for (int a = 0; a < A; a++) {
+        int i0 = a * NP;
+        AziInfo &z = pl.azi[a];
+        z.in_off[0] = pl.ang_off[i0];
+        z.in_off[1] = pl.ang_off[i0 + 2 * nd];
+        for (int n = 0; n < 2; n++) {
+            z.out_off[0][n] = pl.ang_off[reflect(i0, n, nd)];
+            z.out_off[1][n] = pl.ang_off[reflect(i0 + 2 * nd, n, nd)];
+        }
+        z.stride = p.azi_nx[a] + p.azi_ny[a];
+        z.pad = 0;
+    }
+    pl.ang_rsin.resize(A * NP);
+    pl.ang_wt.resize(A * NP);
+    pl.ang_cw.resize(A * NP * 2);
+    pl.ang_fw.resize(A * NP * 2);
+    for (int a = 0; a < A; a++) {
+        for (int pp = 0; pp < NP; pp++) {
+            int iang = a * NP + pp;
+            double spacing = p.azi_spacing[a];
+            // sweep1g: wt_v_st = weight * spacing * height * sin(theta) * PI
+            double wt = p.ang_weight[iang] * spacing * p.hz * p.ang_sin_theta[iang] * PI;
+            pl.ang_rsin[iang] = (float)p.ang_rsin_theta[iang];
+            pl.ang_wt[iang] = (float)wt;
+            // moc::Current::set_angle
+            double w = p.ang_weight[iang] * PI;
+            double al = p.ang_alpha[iang];
+            double cw0 = w * p.ang_ox[iang] * spacing / std::abs(std::cos(al)) * p.hz;
+            double cw1 = w * p.ang_oy[iang] * spacing / std::abs(std::sin(al)) * p.hz;
+            double fw0 = w * spacing / std::abs(std::cos(al)) * p.hz;
+            double fw1 = w * spacing / std::abs(std::sin(al)) * p.hz;
+            pl.ang_cw[2 * iang] = (float)cw0;
+            pl.ang_cw[2 * iang + 1] = (float)cw1;
+            pl.ang_fw[2 * iang] = (float)fw0;
+            pl.ang_fw[2 * iang + 1] = (float)fw1;
+        }
+    }
+
+    Coarse cm{p.nx, p.ny};
+    pl.n_surf = (p.nx + 1) * p.ny + (p.ny + 1) * p.nx + 2 * p.nx * p.ny;
+    pl.surf_area.assign(pl.n_surf, 0.0);
+    for (int s = 0; s < pl.n_surf; s++) {
+        int nsp = p.nx * p.ny + (p.nx + 1) * p.ny + (p.ny + 1) * p.nx;
+        if (s >= nsp) continue;
+        int n = cm.normal(s);
+        if (n == 0) {
+            int iy = (s - p.nx * p.ny) / (p.nx + 1);
+            pl.surf_area[s] = (p.y_vec[iy + 1] - p.y_vec[iy]) * p.hz;
+        } else if (n == 1) {
+            int r = s - p.nx * p.ny - (p.nx + 1) * p.ny;
+            int ix = r / (p.ny + 1);
+            pl.surf_area[s] = (p.x_vec[ix + 1] - p.x_vec[ix]) * p.hz;
+        }
+    }
+
+    std::vector<int> stamp(pl.F, -1), g2l(pl.F, -1);
+    std::vector<int> sstamp(pl.n_surf, -1), s2l(pl.n_surf, -1);
+    std::vector<int32_t> fsr_cnt(pl.F, 0), surf_cnt(pl.n_surf, 0);
+    std::vector<std::vector<Event>> evf, evb;
+    pl.ev_off.push_back(0);
+    int band_id = 0;
+    for (int a = 0; a < A; a++) {
+        const int oct = a / n_az_oct;  // 0: octant 1, 1: octant 2
+        const int r0 = p.azi_ray_off[a], r1 = p.azi_ray_off[a + 1];
+        const int Ny = p.azi_ny[a], Nx = p.azi_nx[a];
+        const double alpha = p.ang_alpha[a * NP];
+        const double sx = hx / Nx, sy = hy / Ny;
+        // spatial order: distance of the entry point along the normal
+        std::vector<std::pair<double, int>> order;
+        for (int r = r0; r < r1; r++) {
+            int b0 = p.ray_bc[2 * r];
+            double x, y;
+            if (b0 < Ny) {
+                x = oct == 0 ? 0.0 : hx;
+                y = (0.5 + b0) * sy;
+            } else {
+                x = (0.5 + (b0 - Ny)) * sx;
+                y = 0.0;
+            }
+            order.push_back({-std::sin(alpha) * x + std::cos(alpha) * y, r});
+        }
+        std::sort(order.begin(), order.end());
+        size_t i = 0;
+        while (i < order.size()) {
+            // grow a band
+            std::vector<int> brays;
+            std::vector<int> locs;
+            int n_local = 0;
+            while (i < order.size() && (int)brays.size() < max_rays) {
+                int r = order[i].second;
+                int new_count = 0;
+                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                    int f = p.seg_fsr[s];
+                    if (stamp[f] != band_id && g2l[f] != -2) {
+                        g2l[f] = -2;  // counted for this ray
+                        new_count++;
+                    }
+                }
+                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                    int f = p.seg_fsr[s];
+                    if (g2l[f] == -2) g2l[f] = -1;
+                }
+                if (!brays.empty() && n_local + new_count > kMaxLocal) break;
+                if (n_local + new_count > kMaxLocal)
+                    throw std::runtime_error("a single ray touches more FSRs than a band holds");
+                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                    int f = p.seg_fsr[s];
+                    if (stamp[f] != band_id) {
+                        stamp[f] = band_id;
+                        locs.push_back(f);
+                        n_local++;
+                    }
+                }
+                brays.push_back(r);
+                i++;
+            }
+            std::sort(locs.begin(), locs.end());
+            for (size_t l = 0; l < locs.size(); l++) g2l[locs[l]] = (int)l;
+            BandInfo b{};
+            b.azi = a;
+            b.n_rays = (int)brays.size();
+            b.n_local = n_local;
+            b.loc_base = (int32_t)pl.loc_fsr.size();
+            b.ray_base = (int32_t)pl.rays.size();
+            b.seg_base = (int64_t)pl.seg_loc.size();
+            b.nmax = 0;
+            for (int r : brays) b.nmax = std::max(b.nmax, (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]));
+            if (b.nmax > 65535) throw std::runtime_error("ray longer than 65535 segments");
+            for (int f : locs) pl.loc_fsr.push_back(f);
+            // contributions per local FSR: every segment, both directions
+            b.max_contrib = 1;
+            for (int r : brays)
+                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) fsr_cnt[p.seg_fsr[s]] += 2;
+            for (int f : locs) {
+                b.max_contrib = std::max(b.max_contrib, fsr_cnt[f]);
+                fsr_cnt[f] = 0;
+            }
+            // interleaved segments
+            for (int k = 0; k < b.nmax; k++)
+                for (int j = 0; j < b.n_rays; j++) {
+                    int r = brays[j];
+                    int64_t s0 = p.ray_seg_off[r];
+                    int n = (int)(p.ray_seg_off[r + 1] - s0);
+                    if (k < n) {
+                        pl.seg_loc.push_back((uint16_t)g2l[p.seg_fsr[s0 + k]]);
+                        pl.seg_len.push_back((float)p.seg_len[s0 + k]);
+                    } else {
+                        pl.seg_loc.push_back((uint16_t)kMaxLocal);
+                        pl.seg_len.push_back(0.0f);
+                    }
+                }
+            // ray records and coarse events
+            b.lsurf_base = (int32_t)pl.lsurf_glob.size();
+            std::vector<int> lsurfs;
+            for (int j = 0; j < b.n_rays; j++) {
+                int r = brays[j];
+                int n = (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]);
+                RayInfo ri{};
+                ri.bc0 = p.ray_bc[2 * r];
+                ri.bc1 = p.ray_bc[2 * r + 1];
+                ri.nseg = n;
+                int fw_y = ri.bc1 >= Ny ? 1 : 0;
+                int bw_y = ri.bc0 >= Ny ? 1 : 0;
+                // exit faces: octant 1 fw leaves E/N, octant 2 fw leaves W/N,
+                // their reverses leave W/S and E/S
+                int fw_face = fw_y ? NORTH : (oct == 0 ? EAST : WEST);
+                int bw_face = bw_y ? SOUTH : (oct == 0 ? WEST : EAST);
+                ri.flags = fw_y | (bw_y << 1) | ((p.bc[fw_face] == MOCGPU_BC_VACUUM) << 2) |
+                           ((p.bc[bw_face] == MOCGPU_BC_VACUUM) << 3);
+                pl.rays.push_back(ri);
+                std::vector<Event> ef, eb;
+                ray_events(p, cm, r, n, ef, eb);
+                for (int d = 0; d < 2; d++) {
+                    for (const Event &e : (d == 0 ? ef : eb)) {
+                        if (sstamp[e.surf] != band_id) {
+                            sstamp[e.surf] = band_id;
+                            lsurfs.push_back(e.surf);
+                        }
+                        surf_cnt[e.surf]++;
+                    }
+                }
+                evf.push_back(std::move(ef));
+                evb.push_back(std::move(eb));
+            }
+            b.max_surf_contrib = 1;
+            for (int s : lsurfs) {
+                b.max_surf_contrib = std::max(b.max_surf_contrib, surf_cnt[s]);
+                surf_cnt[s] = 0;
+            }
+            std::sort(lsurfs.begin(), lsurfs.end());
+            if (lsurfs.size() > (size_t)kMaxLSurf) throw std::runtime_error("too many coarse surfaces in one band");
+            for (size_t l = 0; l < lsurfs.size(); l++) s2l[lsurfs[l]] = (int)l;
+            b.n_lsurf = (int)lsurfs.size();
+            for (int s : lsurfs) pl.lsurf_glob.push_back(s);
+            for (int j = 0; j < b.n_rays; j++) {
+                size_t gi = evf.size() - b.n_rays + j;
+                for (int d = 0; d < 2; d++) {
+                    for (const Event &e : (d == 0 ? evf[gi] : evb[gi])) {
+                        if (e.step > 65535) throw std::runtime_error("event step out of range");
+                        pl.ev.push_back((uint32_t)e.step | ((uint32_t)s2l[e.surf] << 16) |
+                                        ((uint32_t)e.normal << 31));
+                    }
+                    pl.ev_off.push_back((int32_t)pl.ev.size());
+                }
+            }
+            pl.bands.push_back(b);
+            band_id++;
+        }
+    }
+    pl.n_seg_padded = (int64_t)pl.seg_loc.size();
+
+    // FSR -> partial slots, surface -> current slots
+    {
+        std::vector<int32_t> cnt(pl.F + 1, 0);
+        for (int32_t f : pl.loc_fsr) cnt[f + 1]++;
+        for (int f = 0; f < pl.F; f++) cnt[f + 1] += cnt[f];
+        pl.fsr_slot_ptr = cnt;
+        pl.fsr_slot_idx.assign(pl.loc_fsr.size(), 0);
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t s = 0; s < pl.loc_fsr.size(); s++) pl.fsr_slot_idx[fill[pl.loc_fsr[s]]++] = (int32_t)s;
+        for (int f = 0; f < pl.F; f++)
+            if (pl.fsr_slot_ptr[f + 1] == pl.fsr_slot_ptr[f])
+                throw std::runtime_error("No rays passed through FSR " + std::to_string(f) +
+                                         "; try a finer ray spacing");
+    }
+    {
+        std::vector<int32_t> cnt(pl.n_surf + 1, 0);
+        for (int32_t s : pl.lsurf_glob) cnt[s + 1]++;
+        for (int s = 0; s < pl.n_surf; s++) cnt[s + 1] += cnt[s];
+        pl.surf_slot_ptr = cnt;
+        pl.surf_slot_idx.assign(pl.lsurf_glob.size(), 0);
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t s = 0; s < pl.lsurf_glob.size(); s++) pl.surf_slot_idx[fill[pl.lsurf_glob[s]]++] = (int32_t)s;
+    }
+    // longest bands first
+    pl.band_order.resize(pl.bands.size());
+    std::iota(pl.band_order.begin(), pl.band_order.end(), 0);
+    std::stable_sort(pl.band_order.begin(), pl.band_order.end(), [&](int x, int y) {
+        return (int64_t)pl.bands[x].nmax > (int64_t)pl.bands[y].nmax;
+    });
+    return pl;
+}
+
+}  // namespace gpu
+}  // namespace mocc_amd

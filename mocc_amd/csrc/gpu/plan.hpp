@@ -1,0 +1,94 @@
+// Host-side sweep plan: turns the ray-major SoA problem into the band-major
+// layout the gfx950 sweep kernel streams.
+//
+//  * A band is a run of spatially adjacent parallel rays of one azimuth whose
+//    touched FSRs fit a band-local index space of at most kMaxLocal entries,
+//    so one workgroup can tally them in LDS.
+//  * Segments of a band are stored interleaved, element (k, j) at
+//    seg_base + k * n_rays + j, so the lanes of a wave read consecutive
+//    addresses at every step; short rays are padded with the band's dummy
+//    local slot and zero length.
+//  * Every band-local FSR owns one partial slot; fsr_slot_ptr/idx list the
+//    slots of each FSR in increasing order for the final integer reduction.
+//  * Coarse-surface crossings (moc::Current) are flattened into per-ray,
+//    per-direction event lists of (step, band-local surface, normal).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/mocgpu.h"
+
+namespace mocc_amd {
+namespace gpu {
+
+constexpr int kMaxLocal = 2047;  // band-local FSRs (slot kMaxLocal is the dummy)
+constexpr int kMaxRaysPerBand = 64;
+constexpr int kExpN = 10000;      // Exponential_Linear<10000> on [-10, 0]
+constexpr int kMaxLSurf = 1024;   // band-local coarse surfaces held in LDS
+
+struct BandInfo {
+    int32_t azi;        // azimuth index
+    int32_t n_rays;     // rays in the band
+    int32_t nmax;       // longest ray (segments)
+    int32_t n_local;    // band-local FSRs
+    int64_t seg_base;   // first interleaved element
+    int32_t loc_base;   // first local FSR / partial slot
+    int32_t ray_base;   // first entry in ray_info
+    int32_t n_lsurf;    // band-local coarse surfaces
+    int32_t lsurf_base; // first local surface / current slot
+    int32_t max_contrib;      // max tally contributions into one local FSR
+    int32_t max_surf_contrib; // max current tallies into one local surface
+};
+
+// Ray as the kernel needs it: entry/exit boundary indices, length and the
+// exit faces of both directions.
+struct RayInfo {
+    int32_t bc0, bc1, nseg, flags;  // flags: bit0 fw exit on a y-normal face,
+                                    // bit1 bw exit on a y-normal face,
+                                    // bit2 fw exit vacuum, bit3 bw exit vacuum
+};
+
+struct AziInfo {
+    int32_t in_off[2];      // boundary offset of (a, p=0) entry angle, fw / bw
+    int32_t out_off[2][2];  // [dir][normal] offset of the reflected angle
+    int32_t stride;         // Nx + Ny (offset step between polar angles)
+    int32_t pad;
+};
+
+struct SweepPlan {
+    int F = 0, G = 0, NP = 0, A = 0, nd = 0;
+    int n_surf = 0;
+    int64_t n_seg_padded = 0;
+    std::vector<BandInfo> bands;
+    std::vector<int32_t> band_order;        // longest-first
+    std::vector<uint16_t> seg_loc;          // interleaved band-local FSR id
+    std::vector<float> seg_len;             // interleaved length
+    std::vector<int32_t> loc_fsr;           // [total local] global FSR
+    std::vector<RayInfo> rays;              // [sum n_rays]
+    std::vector<int32_t> fsr_slot_ptr, fsr_slot_idx;
+    std::vector<AziInfo> azi;
+    std::vector<float> ang_rsin, ang_wt;    // [A*NP]: 1/sin(theta), w*spacing*hz*sin(theta)*pi
+    std::vector<float> ang_cw, ang_fw;      // [A*NP*2]: Current weights per normal
+    int bcpg = 0;                           // boundary values per group
+    std::vector<int32_t> ang_off;           // [4*nd] boundary offset of each angle
+
+    // currents
+    std::vector<int32_t> lsurf_glob;        // [total local surfaces] global surface
+    std::vector<int32_t> ev_off;            // [2 * total rays + 1] event list offsets (fw, bw per ray)
+    std::vector<uint32_t> ev;               // packed: step (16) | local surf (15) | normal (1)
+    std::vector<int32_t> surf_slot_ptr, surf_slot_idx;
+    std::vector<double> surf_area;          // [n_surf] (0 for z-normal)
+};
+
+// Build the plan; throws std::runtime_error on inputs the kernels cannot take.
+SweepPlan build_plan(const mocgpu_problem &p, int max_rays_per_band);
+
+// 1 - exp(-j * dx) for j = 0..kExpN and the slopes between nodes: the
+// reference's Exponential_Linear<10000> interpolant of exp on [-10, 0]
+// rewritten in terms of tau (see exp_table_fill in plan.cpp).
+void exp_table_fill(std::vector<float> &h_and_slope);
+
+}  // namespace gpu
+}  // namespace mocc_amd

@@ -1,0 +1,285 @@
+// gfx950 MoC sweep kernel: one workgroup per (ray band, group).
+//
+// Restates moc::MoCSweeper::sweep1g (src/sweepers/moc/moc_sweeper_kernel.inc.hpp:36-180)
+// and moc::Current::post_ray (src/sweepers/moc/moc_current_worker.hpp:202-264)
+// for the GPU:
+//
+//   * lanes 0..63 sweep the band's rays forward, lanes 64..127 backward, each
+//     lane carrying all polar angles of its ray (independent recurrences);
+//   * the band's cross sections and transport source q = (S + Ss*phi)/(4 pi St)
+//     (source_isotropic.cpp:21-57) are staged in LDS in FP32;
+//   * every tally contribution sum_p w_p * dpsi_p is rounded to a 64-bit
+//     fixed-point integer and added into a band-local LDS tally with integer
+//     atomics. Integer addition is associative, so the band's sums do not
+//     depend on lane or wave timing; the band's scale 2^k is derived from a
+//     bound on |dpsi| (the staged q and the entry psi) so no sum can overflow;
+//   * outgoing angular fluxes are written straight into the next incoming
+//     buffer of the reflected angle (BoundaryCondition::update with Jacobi
+//     ordering, boundary_condition.cpp:155-191); vacuum faces stay zero.
+#include "kernels.hpp"
+
+namespace mocc_amd {
+namespace gpu {
+
+namespace {
+
+constexpr double FPI = 4.0 * 3.1415926535897932;
+
+// 1 - exp(-tau) as the reference's Exponential_Linear<10000> interpolant
+// (table built by exp_table_fill); beyond tau = 10 the reference falls back to
+// std::exp (exponential.hpp:71-75).
+__device__ __forceinline__ float one_minus_exp(float tau, const float2 *__restrict__ tab)
+{
+    float w = tau * 1000.0f;
+    if (w < (float)kExpN) {
+        int j = (int)w;
+        float f = w - (float)j;
+        float2 hs = tab[j];
+        return fmaf(hs.y, f, hs.x);
+    }
+    return 1.0f - __expf(-tau);
+}
+
+// Power-of-two scale 2^(62 - e) with bound < 2^e, so |sum| * scale < 2^62.
+__device__ __forceinline__ double fixed_scale(double bound)
+{
+    if (!(bound > 0.0)) return 1.0;
+    int e;
+    frexp(bound, &e);
+    return ldexp(1.0, 62 - e);
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float v, double scale)
+{
+    return (unsigned long long)__double2ll_rn((double)v * scale);
+}
+
+template <int NP, bool CUR>
+__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
+{
+    __shared__ unsigned long long tal[kMaxLocal + 1];
+    __shared__ float sig[kMaxLocal + 1];
+    __shared__ float qb[kMaxLocal + 1];
+    __shared__ unsigned long long tcur[CUR ? kMaxLSurf : 1];
+    __shared__ unsigned long long tsfl[CUR ? kMaxLSurf : 1];
+    __shared__ unsigned int s_bound;
+
+    const int item = blockIdx.x;
+    int g, band;
+    if (A.g_fixed >= 0) {
+        g = A.g_fixed;
+        band = A.order[item];
+    } else {
+        g = item % A.G;
+        band = A.order[item / A.G];
+    }
+    const BandInfo B = A.bands[band];
+    const int tid = threadIdx.x;
+    const int G = A.G;
+    if (tid == 0) s_bound = 0u;
+    __syncthreads();
+
+    // stage the band's cross sections and transport source
+    unsigned int lmax = 0u;
+    for (int l = tid; l < B.n_local; l += kWG) {
+        const int f = A.loc_fsr[B.loc_base + l];
+        const int m = A.fsr_mat[f];
+        const double tr = A.mat_tr[m * G + g];
+        const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) *
+                         (1.0 / (tr * FPI));
+        const float q32 = (float)q;
+        sig[l] = (float)tr;
+        qb[l] = q32;
+        tal[l] = 0ull;
+        lmax = max(lmax, __float_as_uint(fabsf(q32)));
+    }
+    if (tid == 0) {
+        sig[kMaxLocal] = 0.0f;
+        qb[kMaxLocal] = 0.0f;
+        tal[kMaxLocal] = 0ull;
+    }
+    if (CUR) {
+        for (int l = tid; l < B.n_lsurf; l += kWG) {
+            tcur[l] = 0ull;
+            tsfl[l] = 0ull;
+        }
+    }
+
+    const int dir = tid / kRays;
+    const int j = tid % kRays;
+    const bool active = j < B.n_rays;
+    const AziInfo az = A.azi[B.azi];
+    const size_t gbase = (size_t)g * 2 * A.bcpg;
+    const int par = (int)((A.parity >> g) & 1ull);
+    const float *__restrict__ psi_in = A.psi + gbase + (size_t)par * A.bcpg;
+    float *__restrict__ psi_out = A.psi + gbase + (size_t)(1 - par) * A.bcpg;
+
+    float psi[NP], rs[NP], wt[NP];
+    float cw[NP][2], fw[NP][2];
+    RayInfo r{0, 0, 0, 0};
+    float wsum = 0.0f, cwsum = 0.0f;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        rs[p] = A.ang_rsin[B.azi * NP + p];
+        wt[p] = A.ang_wt[B.azi * NP + p];
+        wsum += wt[p];
+        psi[p] = 0.0f;
+        if (CUR) {
+#pragma unroll
+            for (int n = 0; n < 2; n++) {
+                cw[p][n] = A.ang_cw[(B.azi * NP + p) * 2 + n];
+                fw[p][n] = A.ang_fw[(B.azi * NP + p) * 2 + n];
+            }
+            cwsum += fmaxf(fmaxf(fabsf(cw[p][0]), fabsf(cw[p][1])), fmaxf(fw[p][0], fw[p][1]));
+        }
+    }
+    if (active) {
+        r = A.rays[B.ray_base + j];
+        const int entry = dir ? r.bc1 : r.bc0;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            psi[p] = psi_in[az.in_off[dir] + p * az.stride + entry];
+            lmax = max(lmax, __float_as_uint(fabsf(psi[p])));
+        }
+    }
+    atomicMax(&s_bound, lmax);
+    __syncthreads();
+
+    // |dpsi| <= 2 * max(|psi_in|, |q|) since psi relaxes towards q; the
+    // per-contribution bound times the contribution count bounds every sum.
+    const double psi_bound = (double)__uint_as_float(s_bound);
+    const double scale = fixed_scale(2.0 * psi_bound * (double)wsum * (double)B.max_contrib * 1.0001);
+    double cscale = 1.0;
+    int ep = 0, ee = 0;
+    if (CUR) {
+        cscale = fixed_scale(psi_bound * (double)cwsum * (double)B.max_surf_contrib * 1.0001);
+        if (active) {
+            ep = A.ev_off[2 * (B.ray_base + j) + dir];
+            ee = A.ev_off[2 * (B.ray_base + j) + dir + 1];
+        }
+    }
+    const float csign = dir ? -1.0f : 1.0f;
+
+    const uint16_t *__restrict__ sl = A.seg_loc + B.seg_base;
+    const float *__restrict__ sg = A.seg_len + B.seg_base;
+    const float2 *__restrict__ tab = A.exptab;
+    const int n = r.nseg;
+    const int nr = B.n_rays;
+
+    auto tally_current = [&](uint32_t evv) {
+        const int ls = (int)((evv >> 16) & 0x7FFFu);
+        const int nn = (int)(evv >> 31);
+        float cv = 0.0f, sv = 0.0f;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            cv = fmaf(psi[p], nn ? cw[p][1] : cw[p][0], cv);
+            sv = fmaf(psi[p], nn ? fw[p][1] : fw[p][0], sv);
+        }
+        atomicAdd(&tcur[ls], to_fixed(csign * cv, cscale));
+        atomicAdd(&tsfl[ls], to_fixed(sv, cscale));
+    };
+
+    for (int t = 0; t < B.nmax; t++) {
+        if (active && t < n) {
+            if (CUR) {
+                while (ep < ee && (int)(A.ev[ep] & 0xFFFFu) == t) {
+                    tally_current(A.ev[ep]);
+                    ep++;
+                }
+            }
+            const int k = dir ? (n - 1 - t) : t;
+            const size_t e = (size_t)k * nr + j;
+            const int loc = sl[e];
+            const float len = sg[e];
+            const float st = sig[loc];
+            const float q = qb[loc];
+            const float tau0 = st * len;
+            float c = 0.0f;
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const float ex = one_minus_exp(tau0 * rs[p], tab);
+                const float d = (psi[p] - q) * ex;
+                psi[p] = psi[p] - d;
+                c = fmaf(d, wt[p], c);
+            }
+            atomicAdd(&tal[loc], to_fixed(c, scale));
+        }
+    }
+    if (CUR && active) {
+        while (ep < ee) {
+            tally_current(A.ev[ep]);
+            ep++;
+        }
+    }
+    if (active) {
+        const int exit_idx = dir ? r.bc0 : r.bc1;
+        const int ny = (r.flags >> dir) & 1;
+        const int vac = (r.flags >> (2 + dir)) & 1;
+        if (!vac) {
+#pragma unroll
+            for (int p = 0; p < NP; p++) psi_out[az.out_off[dir][ny] + p * az.stride + exit_idx] = psi[p];
+        }
+    }
+    __syncthreads();
+    const double rscale = 1.0 / scale;
+    for (int l = tid; l < B.n_local; l += kWG)
+        A.psum[(size_t)(B.loc_base + l) * G + g] = (double)(long long)tal[l] * rscale;
+    if (CUR) {
+        const double rc = 1.0 / cscale;
+        for (int l = tid; l < B.n_lsurf; l += kWG) {
+            A.csum[(size_t)(B.lsurf_base + l) * G + g] = (double)(long long)tcur[l] * rc;
+            A.ssum[(size_t)(B.lsurf_base + l) * G + g] = (double)(long long)tsfl[l] * rc;
+        }
+    }
+}
+
+// phi = T / (St V) + 4 pi q (moc_sweeper_kernel.inc.hpp:165-173), T summed
+// over the FSR's band slots in a fixed order.
+__global__ void k_reduce_flux(ReduceArgs A)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int f, g;
+    if (A.g_fixed >= 0) {
+        f = idx;
+        g = A.g_fixed;
+    } else {
+        f = idx / A.G;
+        g = idx % A.G;
+    }
+    if (f >= A.F) return;
+    const int G = A.G;
+    double T = 0.0;
+    for (int s = A.slot_ptr[f]; s < A.slot_ptr[f + 1]; s++) T += A.psum[(size_t)A.slot_idx[s] * G + g];
+    const int m = A.fsr_mat[f];
+    const double tr = A.mat_tr[m * G + g];
+    const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) * (1.0 / (tr * FPI));
+    A.phi[(size_t)f * G + g] = T / (tr * A.vol[f]) + q * FPI;
+}
+
+}  // namespace
+
+void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStream_t s)
+{
+    dim3 grid(n_items), block(kWG);
+#define MOCGPU_SWEEP_CASE(NPV)                                                   \
+    case NPV:                                                                    \
+        if (cur) hipLaunchKernelGGL((k_sweep<NPV, true>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((k_sweep<NPV, false>), grid, block, 0, s, a);    \
+        break;
+    switch (n_polar) {
+        MOCGPU_SWEEP_CASE(1)
+        MOCGPU_SWEEP_CASE(2)
+        MOCGPU_SWEEP_CASE(3)
+        MOCGPU_SWEEP_CASE(4)
+    }
+#undef MOCGPU_SWEEP_CASE
+}
+
+void launch_reduce_flux(const ReduceArgs &a, hipStream_t s)
+{
+    int n = a.g_fixed >= 0 ? a.F : a.F * a.G;
+    hipLaunchKernelGGL(k_reduce_flux, dim3((n + 255) / 256), dim3(256), 0, s, a);
+}
+
+}  // namespace gpu
+}  // namespace mocc_amd

@@ -1,0 +1,161 @@
+"""Python mirror of mocc's TransportSweeper / MoCSweeper interface over the
+gfx950 library (include/mocgpu.h).
+
+Method names and argument meaning follow src/core/transport_sweeper.hpp and
+src/sweepers/moc/moc_sweeper.hpp; errors the reference raises as exceptions
+come back as status codes and are re-raised here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .native import EigenState, HostProblem, MocgpuOptions, gpu, ptr
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class NonPositiveFluxError(ValueError):
+    pass
+
+
+class GpuMoCSweeper:
+    def __init__(self, prob: HostProblem, n_inner: int | None = None, energy_mode: int = 0,
+                 use_cmfd: bool | None = None, band_rays: int = 0, **cmfd):
+        L = gpu()
+        opt = prob.options()
+        if n_inner is not None:
+            opt.n_inner = n_inner
+        opt.energy_mode = energy_mode
+        opt.band_rays = band_rays
+        if use_cmfd is not None:
+            opt.use_cmfd = 1 if use_cmfd else 0
+        for k, v in cmfd.items():
+            setattr(opt, k, v)
+        self.prob = prob
+        self.opt = opt
+        self.h = C.c_void_p()
+        rc = L.mocgpu_create(prob.view, C.byref(opt), C.byref(self.h))
+        if rc != 0:
+            raise DeviceError(L.mocgpu_last_error(None).decode())
+        self.F, self.G = prob.n_fsr, prob.n_group
+        self.npin = prob.p.nx * prob.p.ny
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = gpu().mocgpu_last_error(self.h).decode()
+            if rc == -4:
+                raise NonPositiveFluxError(msg)
+            raise DeviceError(f"mocgpu error {rc}: {msg}")
+
+    def close(self):
+        if self.h:
+            gpu().mocgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- TransportSweeper -------------------------------------------------
+    def initialize(self):
+        self._check(gpu().mocgpu_initialize(self.h))
+
+    def flux(self) -> np.ndarray:
+        out = np.zeros((self.F, self.G))
+        self._check(gpu().mocgpu_get_flux(self.h, ptr(out)))
+        return out
+
+    def set_flux(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert phi.shape == (self.F, self.G)
+        self._check(gpu().mocgpu_set_flux(self.h, ptr(phi)))
+
+    def boundary(self) -> np.ndarray:
+        n = gpu().mocgpu_boundary_size(self.h)
+        out = np.zeros(n)
+        self._check(gpu().mocgpu_get_boundary(self.h, ptr(out), n))
+        return out
+
+    def set_boundary(self, bc):
+        bc = np.ascontiguousarray(bc, dtype=np.float64)
+        self._check(gpu().mocgpu_set_boundary(self.h, ptr(bc), bc.size))
+
+    def store_old_flux(self):
+        self._check(gpu().mocgpu_store_old_flux(self.h))
+
+    def calc_fission_source(self, k: float) -> np.ndarray:
+        out = np.zeros(self.F)
+        self._check(gpu().mocgpu_calc_fission_source(self.h, k, ptr(out)))
+        return out
+
+    def set_fission_source(self, fs):
+        fs = np.ascontiguousarray(fs, dtype=np.float64)
+        self._check(gpu().mocgpu_set_fission_source(self.h, ptr(fs)))
+
+    def build_source(self, g: int):
+        self._check(gpu().mocgpu_build_source(self.h, g))
+
+    def sweep(self, g: int, with_current: bool = False):
+        self._check(gpu().mocgpu_sweep(self.h, g, 1 if with_current else 0))
+
+    def fss_step(self, with_current: bool = False):
+        self._check(gpu().mocgpu_fss_step(self.h, 1 if with_current else 0))
+
+    def total_fission(self, old: bool = False) -> float:
+        out = np.zeros(1)
+        self._check(gpu().mocgpu_total_fission(self.h, 1 if old else 0, ptr(out)))
+        return float(out[0])
+
+    def flux_residual(self) -> float:
+        out = np.zeros(1)
+        self._check(gpu().mocgpu_flux_residual(self.h, ptr(out)))
+        return float(out[0])
+
+    def get_pin_flux_1g(self, g: int) -> np.ndarray:
+        out = np.zeros(self.npin)
+        self._check(gpu().mocgpu_get_pin_flux_1g(self.h, g, ptr(out)))
+        return out
+
+    def set_pin_flux_1g(self, g: int, pin) -> float:
+        pin = np.ascontiguousarray(pin, dtype=np.float64)
+        r = np.zeros(1)
+        self._check(gpu().mocgpu_set_pin_flux_1g(self.h, g, ptr(pin), ptr(r)))
+        return float(r[0])
+
+    def currents(self, g: int):
+        n = gpu().mocgpu_n_surf(self.h)
+        c, s = np.zeros(n), np.zeros(n)
+        self._check(gpu().mocgpu_get_currents(self.h, g, ptr(c), ptr(s)))
+        return c, s
+
+    # --- CMFD / EigenSolver -------------------------------------------------
+    def cmfd_step(self, k: float) -> float:
+        kk = np.array([k])
+        self._check(gpu().mocgpu_cmfd_step(self.h, ptr(kk)))
+        return float(kk[0])
+
+    def eigen_solve(self, k_tol: float, psi_tol: float, max_iter: int, min_iter: int = 0):
+        st = EigenState()
+        rc = gpu().mocgpu_eigen_solve(self.h, k_tol, psi_tol, max_iter, min_iter, C.byref(st))
+        if rc not in (0, -5):
+            self._check(rc)
+        return rc, st
+
+    def eigen_iterate(self, st: EigenState):
+        self._check(gpu().mocgpu_eigen_iterate(self.h, C.byref(st)))
+        return st
+
+    # --- measurement --------------------------------------------------------
+    def updates_per_sweep(self) -> int:
+        return int(gpu().mocgpu_updates_per_sweep(self.h))
+
+    def time_sweeps(self, n: int):
+        a, b = np.zeros(1), np.zeros(1)
+        self._check(gpu().mocgpu_time_sweeps(self.h, n, ptr(a), ptr(b)))
+        return float(a[0]), float(b[0])
