@@ -181,6 +181,13 @@ int mocgpu_build_source(mocgpu_handle *h, int32_t g);
  * (the reference's boundary_update="jacobi", moc_sweeper_kernel.inc.hpp:146-150). */
 int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current);
 
+/* One inner sweep of every group at once (sweep1g for all groups, Jacobi in
+ * energy: each group's source was built beforehand), enqueued on the
+ * library's stream without waiting. Used by the benchmark and batching. */
+int mocgpu_sweep_all(mocgpu_handle *h, int32_t with_current);
+/* Wait for all work enqueued on the handle's stream. */
+int mocgpu_synchronize(mocgpu_handle *h);
+
 /* FixedSourceSolver::step (src/solvers/fixed_source_solver.cpp:102-117) with
  * the handle's energy_mode: store old flux; for each group build the source
  * and sweep. */

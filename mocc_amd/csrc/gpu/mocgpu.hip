@@ -216,7 +216,7 @@ __global__ void k_fill_f(float *a, float v, size_t n)
 // Currents of the coarse surfaces: sum the band slots, normalize by area
 // (moc::Current::post_sweep, moc_current_worker.hpp:272-318). Surfaces no ray
 // crosses end up zero, as after CoarseData::zero_data_radial.
-__global__ void k_reduce_current(int n_surf_xy0, int n_surf_xy1, int G, int g_fixed, const int32_t *ptr,
+__global__ void k_reduce_current(int n_surf_xy0, int n_surf_xy1, int G, int g_fixed, int n_lslots, const int32_t *ptr,
                                  const int32_t *idx, const double *csum, const double *ssum, const double *area,
                                  double *cur, double *sfl)
 {
@@ -234,8 +234,8 @@ __global__ void k_reduce_current(int n_surf_xy0, int n_surf_xy1, int G, int g_fi
     s += n_surf_xy0;
     double c = 0.0, f = 0.0;
     for (int k = ptr[s]; k < ptr[s + 1]; k++) {
-        c += csum[(size_t)idx[k] * G + g];
-        f += ssum[(size_t)idx[k] * G + g];
+        c += csum[(size_t)g * n_lslots + idx[k]];
+        f += ssum[(size_t)g * n_lslots + idx[k]];
     }
     cur[(size_t)s * G + g] = c / area[s];
     sfl[(size_t)s * G + g] = f / area[s];
@@ -332,6 +332,10 @@ struct mocgpu_handle {
         a.G = G;
         a.F = F;
         a.bcpg = plan.bcpg;
+        a.lds_local = plan.max_local + 1;
+        a.lds_lsurf = std::max(1, plan.max_lsurf);
+        a.n_slots = (int)plan.loc_fsr.size();
+        a.n_lslots = (int)std::max<size_t>(1, plan.lsurf_glob.size());
         a.parity = parity;
         a.seg_loc = d_seg_loc.p;
         a.seg_len = d_seg_len.p;
@@ -363,6 +367,7 @@ struct mocgpu_handle {
         r.F = F;
         r.G = G;
         r.g_fixed = g_fixed;
+        r.n_slots = (int)plan.loc_fsr.size();
         r.slot_ptr = d_slot_ptr.p;
         r.slot_idx = d_slot_idx.p;
         r.psum = d_psum.p;
@@ -411,14 +416,15 @@ struct mocgpu_handle {
     void sweep_once(int g, bool with_current)
     {
         SweepArgs a = sweep_args(g);
-        int items = (int)plan.bands.size() * (g >= 0 ? 1 : G);
+        int items = sweep_grid_items((int)plan.bands.size(), G, g < 0);
         launch_sweep(a, NP, with_current, items, stream);
         launch_reduce_flux(reduce_args(g), stream);
         if (with_current) {
             int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
             size_t n = (size_t)(n1 - n0) * (g >= 0 ? 1 : G);
-            hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, d_surf_ptr.p,
-                               d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
+            int nls = (int)std::max<size_t>(1, plan.lsurf_glob.size());
+            hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, nls,
+                               d_surf_ptr.p, d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
             has_radial = true;
         }
         if (g >= 0) parity ^= (1ull << g);
@@ -756,6 +762,22 @@ int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current)
     });
 }
 
+int mocgpu_sweep_all(mocgpu_handle *h, int32_t with_current)
+{
+    return guard(h, [&] {
+        h->sweep_once(-1, with_current != 0);
+        return 0;
+    });
+}
+
+int mocgpu_synchronize(mocgpu_handle *h)
+{
+    return guard(h, [&] {
+        h->sync();
+        return 0;
+    });
+}
+
 int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current)
 {
     return guard(h, [&] {
@@ -845,7 +867,7 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double
 {
     return guard(h, [&] {
         SweepArgs a = h->sweep_args(-1);
-        int items = (int)h->plan.bands.size() * h->G;
+        int items = sweep_grid_items((int)h->plan.bands.size(), h->G, true);
         // warm-up
         launch_sweep(a, h->NP, false, items, h->stream);
         launch_reduce_flux(h->reduce_args(-1), h->stream);

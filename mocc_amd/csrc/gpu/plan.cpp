@@ -92,19 +92,21 @@ void ray_events(const mocgpu_problem &p, const Coarse &cm, int r, int nseg, std:
 // the same piecewise-linear interpolant in terms of w = 1000*tau with nodes
 // H_j = 1 - exp(x_{N-j}) gives e = H_j + (H_{j+1} - H_j) * (w - j), j = floor(w),
 // which FP32 evaluates without the cancellation of 1 - (value near 1).
+// Past tau = 10 the reference calls std::exp; the table continues there with
+// nodes of 1 - exp(-tau) itself (interpolation error < 6e-12) up to tau = 40,
+// beyond which 1 - exp(-tau) is 1 in FP32, so lookups never branch.
 void exp_table_fill(std::vector<float> &t)
 {
     const int N = kExpN;
     const double mn = -10.0, mx = 0.0;
     const double space = (mx - mn) / (double)N;
-    std::vector<double> d(N + 1);
-    for (int i = 0; i <= N; i++) d[i] = std::exp(mn + i * space);
-    t.assign(2 * N, 0.0f);
-    for (int j = 0; j < N; j++) {
-        double h0 = 1.0 - d[N - j];
-        double h1 = 1.0 - d[N - j - 1];
-        t[2 * j] = (float)h0;
-        t[2 * j + 1] = (float)(h1 - h0);
+    std::vector<double> H(kExpTable + 1);
+    for (int j = 0; j <= kExpTable; j++)
+        H[j] = (j <= N) ? 1.0 - std::exp(mn + (N - j) * space) : 1.0 - std::exp(-(double)j * space);
+    t.assign(2 * kExpTable, 0.0f);
+    for (int j = 0; j < kExpTable; j++) {
+        t[2 * j] = (float)H[j];
+        t[2 * j + 1] = (float)(H[j + 1] - H[j]);
     }
 }
 
@@ -262,11 +264,32 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
             b.n_local = n_local;
             b.loc_base = (int32_t)pl.loc_fsr.size();
             b.ray_base = (int32_t)pl.rays.size();
-            b.seg_base = (int64_t)pl.seg_loc.size();
             b.nmax = 0;
             for (int r : brays) b.nmax = std::max(b.nmax, (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]));
             if (b.nmax > 65535) throw std::runtime_error("ray longer than 65535 segments");
             for (int f : locs) pl.loc_fsr.push_back(f);
+            // forward and backward streams, chunked layout
+            const int nchunk = (b.nmax + kChunk - 1) / kChunk;
+            for (int d = 0; d < 2; d++) {
+                b.seg_base[d] = (int64_t)pl.seg_loc.size();
+                for (int c = 0; c < nchunk; c++)
+                    for (int j = 0; j < b.n_rays; j++) {
+                        int r = brays[j];
+                        int64_t s0 = p.ray_seg_off[r];
+                        int n = (int)(p.ray_seg_off[r + 1] - s0);
+                        for (int u = 0; u < kChunk; u++) {
+                            int t = c * kChunk + u;
+                            if (t < n) {
+                                int k = d == 0 ? t : n - 1 - t;
+                                pl.seg_loc.push_back((uint16_t)g2l[p.seg_fsr[s0 + k]]);
+                                pl.seg_len.push_back((float)p.seg_len[s0 + k]);
+                            } else {
+                                pl.seg_loc.push_back((uint16_t)n_local);
+                                pl.seg_len.push_back(0.0f);
+                            }
+                        }
+                    }
+            }
             // contributions per local FSR: every segment, both directions
             b.max_contrib = 1;
             for (int r : brays)
@@ -275,20 +298,6 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
                 b.max_contrib = std::max(b.max_contrib, fsr_cnt[f]);
                 fsr_cnt[f] = 0;
             }
-            // interleaved segments
-            for (int k = 0; k < b.nmax; k++)
-                for (int j = 0; j < b.n_rays; j++) {
-                    int r = brays[j];
-                    int64_t s0 = p.ray_seg_off[r];
-                    int n = (int)(p.ray_seg_off[r + 1] - s0);
-                    if (k < n) {
-                        pl.seg_loc.push_back((uint16_t)g2l[p.seg_fsr[s0 + k]]);
-                        pl.seg_len.push_back((float)p.seg_len[s0 + k]);
-                    } else {
-                        pl.seg_loc.push_back((uint16_t)kMaxLocal);
-                        pl.seg_len.push_back(0.0f);
-                    }
-                }
             // ray records and coarse events
             b.lsurf_base = (int32_t)pl.lsurf_glob.size();
             std::vector<int> lsurfs;
@@ -348,6 +357,10 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
         }
     }
     pl.n_seg_padded = (int64_t)pl.seg_loc.size();
+    for (const BandInfo &b : pl.bands) {
+        pl.max_local = std::max(pl.max_local, b.n_local);
+        pl.max_lsurf = std::max(pl.max_lsurf, b.n_lsurf);
+    }
 
     // FSR -> partial slots, surface -> current slots
     {

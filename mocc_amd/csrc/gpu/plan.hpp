@@ -4,10 +4,12 @@
 //  * A band is a run of spatially adjacent parallel rays of one azimuth whose
 //    touched FSRs fit a band-local index space of at most kMaxLocal entries,
 //    so one workgroup can tally them in LDS.
-//  * Segments of a band are stored interleaved, element (k, j) at
-//    seg_base + k * n_rays + j, so the lanes of a wave read consecutive
-//    addresses at every step; short rays are padded with the band's dummy
-//    local slot and zero length.
+//  * Segments of a band are stored twice, in forward order and in backward
+//    order, each in a chunked layout: step t of ray j lives at
+//    seg_base[d] + ((t / kChunk) * n_rays + j) * kChunk + t % kChunk, so one
+//    lane fetches kChunk steps with one 16-byte and two 16-byte loads and the
+//    lanes of a wave read one contiguous block; short rays are padded with
+//    the band's dummy local slot (n_local) and zero length.
 //  * Every band-local FSR owns one partial slot; fsr_slot_ptr/idx list the
 //    slots of each FSR in increasing order for the final integer reduction.
 //  * Coarse-surface crossings (moc::Current) are flattened into per-ray,
@@ -23,9 +25,11 @@
 namespace mocc_amd {
 namespace gpu {
 
-constexpr int kMaxLocal = 2047;  // band-local FSRs (slot kMaxLocal is the dummy)
+constexpr int kMaxLocal = 2047;  // band-local FSRs held in LDS
 constexpr int kMaxRaysPerBand = 64;
+constexpr int kChunk = 8;         // segments per lane per vector load (chunked layout)
 constexpr int kExpN = 10000;      // Exponential_Linear<10000> on [-10, 0]
+constexpr int kExpTable = 40000;  // table nodes: the reference's 10000 + 1 - exp(-tau) to tau = 40
 constexpr int kMaxLSurf = 1024;   // band-local coarse surfaces held in LDS
 
 struct BandInfo {
@@ -33,7 +37,7 @@ struct BandInfo {
     int32_t n_rays;     // rays in the band
     int32_t nmax;       // longest ray (segments)
     int32_t n_local;    // band-local FSRs
-    int64_t seg_base;   // first interleaved element
+    int64_t seg_base[2];// first element of the forward / backward stream
     int32_t loc_base;   // first local FSR / partial slot
     int32_t ray_base;   // first entry in ray_info
     int32_t n_lsurf;    // band-local coarse surfaces
@@ -61,6 +65,7 @@ struct SweepPlan {
     int F = 0, G = 0, NP = 0, A = 0, nd = 0;
     int n_surf = 0;
     int64_t n_seg_padded = 0;
+    int max_local = 0, max_lsurf = 0;       // LDS sizing
     std::vector<BandInfo> bands;
     std::vector<int32_t> band_order;        // longest-first
     std::vector<uint16_t> seg_loc;          // interleaved band-local FSR id

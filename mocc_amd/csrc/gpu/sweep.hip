@@ -8,6 +8,10 @@
 //     lane carrying all polar angles of its ray (independent recurrences);
 //   * the band's cross sections and transport source q = (S + Ss*phi)/(4 pi St)
 //     (source_isotropic.cpp:21-57) are staged in LDS in FP32;
+//   * segments stream in chunks of kChunk steps per lane (three 16-byte loads),
+//     the next chunk in flight while the current one is swept; within a chunk
+//     all exponentials (independent of psi) are gathered before the psi
+//     recurrence runs;
 //   * every tally contribution sum_p w_p * dpsi_p is rounded to a 64-bit
 //     fixed-point integer and added into a band-local LDS tally with integer
 //     atomics. Integer addition is associative, so the band's sums do not
@@ -24,20 +28,19 @@ namespace gpu {
 namespace {
 
 constexpr double FPI = 4.0 * 3.1415926535897932;
+constexpr int kNumXcd = 8;  // MI355X: workgroup i is dispatched to XCD i % 8
 
 // 1 - exp(-tau) as the reference's Exponential_Linear<10000> interpolant
-// (table built by exp_table_fill); beyond tau = 10 the reference falls back to
-// std::exp (exponential.hpp:71-75).
+// (table built by exp_table_fill). The table is extended past tau = 10 with
+// nodes of 1 - exp(-tau) itself, where the reference calls std::exp
+// (exponential.hpp:71-75), so the lookup needs no branch.
 __device__ __forceinline__ float one_minus_exp(float tau, const float2 *__restrict__ tab)
 {
-    float w = tau * 1000.0f;
-    if (w < (float)kExpN) {
-        int j = (int)w;
-        float f = w - (float)j;
-        float2 hs = tab[j];
-        return fmaf(hs.y, f, hs.x);
-    }
-    return 1.0f - __expf(-tau);
+    const float w = tau * 1000.0f;
+    const int j = min((int)w, kExpTable - 1);
+    const float f = w - (float)j;
+    const float2 hs = tab[j];
+    return fmaf(hs.y, f, hs.x);
 }
 
 // Power-of-two scale 2^(62 - e) with bound < 2^e, so |sum| * scale < 2^62.
@@ -57,21 +60,27 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale)
 template <int NP, bool CUR>
 __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
 {
-    __shared__ unsigned long long tal[kMaxLocal + 1];
-    __shared__ float sig[kMaxLocal + 1];
-    __shared__ float qb[kMaxLocal + 1];
-    __shared__ unsigned long long tcur[CUR ? kMaxLSurf : 1];
-    __shared__ unsigned long long tsfl[CUR ? kMaxLSurf : 1];
+    extern __shared__ unsigned long long smem[];
+    unsigned long long *tal = smem;
+    float *sig = (float *)(smem + A.lds_local);
+    float *qb = sig + A.lds_local;
+    unsigned long long *tcur = (unsigned long long *)(qb + A.lds_local + (A.lds_local & 1));
+    unsigned long long *tsfl = tcur + A.lds_lsurf;
     __shared__ unsigned int s_bound;
 
-    const int item = blockIdx.x;
     int g, band;
     if (A.g_fixed >= 0) {
         g = A.g_fixed;
-        band = A.order[item];
+        band = A.order[blockIdx.x];
     } else {
-        g = item % A.G;
-        band = A.order[item / A.G];
+        // XCD-aware order: the G workgroups of a band are consecutive
+        // workgroups of one XCD, so they share its L2 copy of the band.
+        const int x = blockIdx.x % kNumXcd;
+        const int s = blockIdx.x / kNumXcd;
+        const int rank = (s / A.G) * kNumXcd + x;
+        if (rank >= A.n_bands) return;
+        g = s % A.G;
+        band = A.order[rank];
     }
     const BandInfo B = A.bands[band];
     const int tid = threadIdx.x;
@@ -94,9 +103,9 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         lmax = max(lmax, __float_as_uint(fabsf(q32)));
     }
     if (tid == 0) {
-        sig[kMaxLocal] = 0.0f;
-        qb[kMaxLocal] = 0.0f;
-        tal[kMaxLocal] = 0ull;
+        sig[B.n_local] = 0.0f;
+        qb[B.n_local] = 0.0f;
+        tal[B.n_local] = 0ull;
     }
     if (CUR) {
         for (int l = tid; l < B.n_lsurf; l += kWG) {
@@ -109,6 +118,11 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     const int j = tid % kRays;
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
+    // select (not index) the per-direction offsets: indexing a local array
+    // with a runtime value would place it in scratch
+    const int in_off = dir ? az.in_off[1] : az.in_off[0];
+    const int out_x = dir ? az.out_off[1][0] : az.out_off[0][0];
+    const int out_y = dir ? az.out_off[1][1] : az.out_off[0][1];
     const size_t gbase = (size_t)g * 2 * A.bcpg;
     const int par = (int)((A.parity >> g) & 1ull);
     const float *__restrict__ psi_in = A.psi + gbase + (size_t)par * A.bcpg;
@@ -138,7 +152,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         const int entry = dir ? r.bc1 : r.bc0;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-            psi[p] = psi_in[az.in_off[dir] + p * az.stride + entry];
+            psi[p] = psi_in[in_off + p * az.stride + entry];
             lmax = max(lmax, __float_as_uint(fabsf(psi[p])));
         }
     }
@@ -159,9 +173,6 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         }
     }
     const float csign = dir ? -1.0f : 1.0f;
-
-    const uint16_t *__restrict__ sl = A.seg_loc + B.seg_base;
-    const float *__restrict__ sg = A.seg_len + B.seg_base;
     const float2 *__restrict__ tab = A.exptab;
     const int n = r.nseg;
     const int nr = B.n_rays;
@@ -179,31 +190,75 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         atomicAdd(&tsfl[ls], to_fixed(sv, cscale));
     };
 
-    for (int t = 0; t < B.nmax; t++) {
-        if (active && t < n) {
+    // chunked segment stream of this lane's ray and direction
+    const int64_t sbase = dir ? B.seg_base[1] : B.seg_base[0];
+    const uint4 *__restrict__ locp = (const uint4 *)(A.seg_loc + sbase);
+    const float4 *__restrict__ lenp = (const float4 *)(A.seg_len + sbase);
+    const int nchunk = (n + kChunk - 1) / kChunk;
+    uint4 cl = make_uint4(0, 0, 0, 0);
+    float4 ca = make_float4(0, 0, 0, 0), cb = ca;
+    if (active && nchunk > 0) {
+        cl = locp[j];
+        ca = lenp[2 * j];
+        cb = lenp[2 * j + 1];
+    }
+    for (int c = 0; c < nchunk; c++) {
+        uint4 nl = cl;
+        float4 na = ca, nb = cb;
+        if (c + 1 < nchunk) {
+            const size_t ci = (size_t)(c + 1) * nr + j;
+            nl = locp[ci];
+            na = lenp[2 * ci];
+            nb = lenp[2 * ci + 1];
+        }
+        int loc[kChunk];
+        float len[kChunk];
+        loc[0] = cl.x & 0xFFFF;
+        loc[1] = cl.x >> 16;
+        loc[2] = cl.y & 0xFFFF;
+        loc[3] = cl.y >> 16;
+        loc[4] = cl.z & 0xFFFF;
+        loc[5] = cl.z >> 16;
+        loc[6] = cl.w & 0xFFFF;
+        loc[7] = cl.w >> 16;
+        len[0] = ca.x;
+        len[1] = ca.y;
+        len[2] = ca.z;
+        len[3] = ca.w;
+        len[4] = cb.x;
+        len[5] = cb.y;
+        len[6] = cb.z;
+        len[7] = cb.w;
+        // exponentials of the whole chunk first: they do not depend on psi
+        float q[kChunk], ex[kChunk][NP];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
+            const float tau0 = sig[loc[u]] * len[u];
+            q[u] = qb[loc[u]];
+#pragma unroll
+            for (int p = 0; p < NP; p++) ex[u][p] = one_minus_exp(tau0 * rs[p], tab);
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
             if (CUR) {
+                const int t = c * kChunk + u;
                 while (ep < ee && (int)(A.ev[ep] & 0xFFFFu) == t) {
                     tally_current(A.ev[ep]);
                     ep++;
                 }
             }
-            const int k = dir ? (n - 1 - t) : t;
-            const size_t e = (size_t)k * nr + j;
-            const int loc = sl[e];
-            const float len = sg[e];
-            const float st = sig[loc];
-            const float q = qb[loc];
-            const float tau0 = st * len;
-            float c = 0.0f;
+            float cacc = 0.0f;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
-                const float ex = one_minus_exp(tau0 * rs[p], tab);
-                const float d = (psi[p] - q) * ex;
+                const float d = (psi[p] - q[u]) * ex[u][p];
                 psi[p] = psi[p] - d;
-                c = fmaf(d, wt[p], c);
+                cacc = fmaf(d, wt[p], cacc);
             }
-            atomicAdd(&tal[loc], to_fixed(c, scale));
+            atomicAdd(&tal[loc[u]], to_fixed(cacc, scale));
         }
+        cl = nl;
+        ca = na;
+        cb = nb;
     }
     if (CUR && active) {
         while (ep < ee) {
@@ -217,18 +272,20 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         const int vac = (r.flags >> (2 + dir)) & 1;
         if (!vac) {
 #pragma unroll
-            for (int p = 0; p < NP; p++) psi_out[az.out_off[dir][ny] + p * az.stride + exit_idx] = psi[p];
+            for (int p = 0; p < NP; p++) psi_out[(ny ? out_y : out_x) + p * az.stride + exit_idx] = psi[p];
         }
     }
     __syncthreads();
     const double rscale = 1.0 / scale;
-    for (int l = tid; l < B.n_local; l += kWG)
-        A.psum[(size_t)(B.loc_base + l) * G + g] = (double)(long long)tal[l] * rscale;
+    double *__restrict__ ps = A.psum + (size_t)g * A.n_slots + B.loc_base;
+    for (int l = tid; l < B.n_local; l += kWG) ps[l] = (double)(long long)tal[l] * rscale;
     if (CUR) {
         const double rc = 1.0 / cscale;
+        double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
+        double *__restrict__ ss = A.ssum + (size_t)g * A.n_lslots + B.lsurf_base;
         for (int l = tid; l < B.n_lsurf; l += kWG) {
-            A.csum[(size_t)(B.lsurf_base + l) * G + g] = (double)(long long)tcur[l] * rc;
-            A.ssum[(size_t)(B.lsurf_base + l) * G + g] = (double)(long long)tsfl[l] * rc;
+            cs[l] = (double)(long long)tcur[l] * rc;
+            ss[l] = (double)(long long)tsfl[l] * rc;
         }
     }
 }
@@ -248,8 +305,9 @@ __global__ void k_reduce_flux(ReduceArgs A)
     }
     if (f >= A.F) return;
     const int G = A.G;
+    const double *__restrict__ ps = A.psum + (size_t)g * A.n_slots;
     double T = 0.0;
-    for (int s = A.slot_ptr[f]; s < A.slot_ptr[f + 1]; s++) T += A.psum[(size_t)A.slot_idx[s] * G + g];
+    for (int s = A.slot_ptr[f]; s < A.slot_ptr[f + 1]; s++) T += ps[A.slot_idx[s]];
     const int m = A.fsr_mat[f];
     const double tr = A.mat_tr[m * G + g];
     const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) * (1.0 / (tr * FPI));
@@ -258,13 +316,21 @@ __global__ void k_reduce_flux(ReduceArgs A)
 
 }  // namespace
 
+size_t sweep_lds_bytes(const SweepArgs &a)
+{
+    return (size_t)a.lds_local * 8 + (size_t)(a.lds_local + (a.lds_local & 1)) * 8 + (size_t)a.lds_lsurf * 16;
+}
+
 void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStream_t s)
 {
+    SweepArgs b = a;
+    if (!cur) b.lds_lsurf = 0;
+    const size_t lds = sweep_lds_bytes(b);
     dim3 grid(n_items), block(kWG);
-#define MOCGPU_SWEEP_CASE(NPV)                                                   \
-    case NPV:                                                                    \
-        if (cur) hipLaunchKernelGGL((k_sweep<NPV, true>), grid, block, 0, s, a); \
-        else hipLaunchKernelGGL((k_sweep<NPV, false>), grid, block, 0, s, a);    \
+#define MOCGPU_SWEEP_CASE(NPV)                                                     \
+    case NPV:                                                                      \
+        if (cur) hipLaunchKernelGGL((k_sweep<NPV, true>), grid, block, lds, s, b); \
+        else hipLaunchKernelGGL((k_sweep<NPV, false>), grid, block, lds, s, b);    \
         break;
     switch (n_polar) {
         MOCGPU_SWEEP_CASE(1)
@@ -273,6 +339,13 @@ void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStr
         MOCGPU_SWEEP_CASE(4)
     }
 #undef MOCGPU_SWEEP_CASE
+}
+
+int sweep_grid_items(int n_bands, int G, bool all_groups)
+{
+    if (!all_groups) return n_bands;
+    const int per_xcd = (n_bands + kNumXcd - 1) / kNumXcd;
+    return per_xcd * G * kNumXcd;
 }
 
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s)

@@ -123,6 +123,8 @@ def gpu() -> C.CDLL:
         L.mocgpu_build_source.argtypes = [P, i32]
         L.mocgpu_sweep.argtypes = [P, i32, i32]
         L.mocgpu_fss_step.argtypes = [P, i32]
+        L.mocgpu_sweep_all.argtypes = [P, i32]
+        L.mocgpu_synchronize.argtypes = [P]
         L.mocgpu_total_fission.argtypes = [P, i32, P]
         L.mocgpu_flux_residual.argtypes = [P, P]
         L.mocgpu_get_pin_flux_1g.argtypes = [P, i32, P]

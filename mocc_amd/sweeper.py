@@ -104,6 +104,13 @@ class GpuMoCSweeper:
     def sweep(self, g: int, with_current: bool = False):
         self._check(gpu().mocgpu_sweep(self.h, g, 1 if with_current else 0))
 
+    def sweep_all_groups(self, with_current: bool = False):
+        """One inner sweep of all groups, enqueued without waiting."""
+        self._check(gpu().mocgpu_sweep_all(self.h, 1 if with_current else 0))
+
+    def synchronize(self):
+        self._check(gpu().mocgpu_synchronize(self.h))
+
     def fss_step(self, with_current: bool = False):
         self._check(gpu().mocgpu_fss_step(self.h, 1 if with_current else 0))
 
