@@ -51,6 +51,12 @@ int mocc_host_quadrature(const mocc_host_problem *h, double *out, int32_t cap);
  * constructor), same row layout. */
 int mocc_host_quadrature_xml(const char *ang_quad_xml, double *out, int32_t cap);
 
+/* AngularQuadrature::modify_angle of angles idx[0..n_mod) to the azimuths
+ * alpha[], then update_weights() (src/core/angular_quadrature.cpp:192-352,
+ * the step RayData's modularization performs); same row layout. */
+int mocc_host_quadrature_modify(const char *ang_quad_xml, int32_t n_mod, const int32_t *idx,
+                                const double *alpha, double *out, int32_t cap);
+
 /* Trace one ray between two boundary points of the core of h (moc::Ray).
  * seg_* receive up to cap segments; cm_head = {cell_fw, surf_fw, cell_bw,
  * surf_bw, n_cm}; cm receives 4*n_cm ints. Returns nseg or < 0. */

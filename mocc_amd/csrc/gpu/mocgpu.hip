@@ -501,7 +501,10 @@ const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str(
 
 int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
 {
-    if (!p || !out) return MOCGPU_ERR_ARG;
+    if (!p || !out) {
+        g_create_err = "mocgpu_create: null problem or output handle";
+        return MOCGPU_ERR_ARG;
+    }
     auto h = std::make_unique<mocgpu_handle>();
     int rc = guard(h.get(), [&] {
         int ndev = 0;
@@ -859,6 +862,15 @@ int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *su
         }
         return 0;
     });
+}
+
+int32_t mocgpu_exp_table(float *out, int32_t cap)
+{
+    std::vector<float> t;
+    mocc_amd::gpu::exp_table_fill(t);
+    const int32_t n = (int32_t)(t.size() / 2);
+    if (out && cap > 0) std::memcpy(out, t.data(), sizeof(float) * 2 * (size_t)std::min(cap, n));
+    return n;
 }
 
 int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return 2LL * h->NP * h->n_seg * h->G; }

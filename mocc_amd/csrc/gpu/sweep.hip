@@ -30,31 +30,32 @@ namespace {
 constexpr double FPI = 4.0 * 3.1415926535897932;
 constexpr int kNumXcd = 8;  // MI355X: workgroup i is dispatched to XCD i % 8
 
-// 1 - exp(-tau) as the reference's Exponential_Linear<10000> interpolant
-// (table built by exp_table_fill). The table is extended past tau = 10 with
-// nodes of 1 - exp(-tau) itself, where the reference calls std::exp
-// (exponential.hpp:71-75), so the lookup needs no branch.
-__device__ __forceinline__ float one_minus_exp(float tau, const float2 *__restrict__ tab)
-{
-    const float w = tau * 1000.0f;
-    const int j = min((int)w, kExpTable - 1);
-    const float f = w - (float)j;
-    const float2 hs = tab[j];
-    return fmaf(hs.y, f, hs.x);
-}
+// Largest interpolation coordinate 1000 tau: the last table node. Beyond it
+// 1 - exp(-tau) rounds to 1 in FP32.
+constexpr float kWMax = (float)(kExpTable - 1);
 
-// Power-of-two scale 2^(62 - e) with bound < 2^e, so |sum| * scale < 2^62.
-__device__ __forceinline__ double fixed_scale(double bound)
+// Power-of-two scale 2^k with bound < 2^(62 - k), so |sum| * 2^k < 2^62.
+__device__ __forceinline__ int fixed_exp(double bound)
 {
-    if (!(bound > 0.0)) return 1.0;
+    if (!(bound > 0.0)) return 0;
     int e;
     frexp(bound, &e);
-    return ldexp(1.0, 62 - e);
+    return 62 - e;
 }
 
-__device__ __forceinline__ unsigned long long to_fixed(float v, double scale)
+// v * 2^k as a two's-complement 64-bit integer, |v * 2^k| < 2^62, in FP32
+// arithmetic: x = v * 2^k is exact, hi = floor(x / 2^32) and
+// lo = x - hi * 2^32 in [0, 2^32). lo is exact except when -2^32 < x < 0,
+// where it is rounded to 24 bits and clamped below 2^32 (an error below 2^9
+// units of 2^-k).
+__device__ __forceinline__ unsigned long long to_fixed(float v, int k)
 {
-    return (unsigned long long)__double2ll_rn((double)v * scale);
+    const float x = ldexpf(v, k);
+    const float hf = floorf(x * 2.3283064365386963e-10f);
+    const float lo = fminf(fmaf(-hf, 4294967296.0f, x), 4294967040.0f);
+    const unsigned int uhi = (unsigned int)(int)hf;
+    const unsigned int ulo = (unsigned int)lo;
+    return ((unsigned long long)uhi << 32) | ulo;
 }
 
 template <int NP, bool CUR>
@@ -62,9 +63,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
 {
     extern __shared__ unsigned long long smem[];
     unsigned long long *tal = smem;
-    float *sig = (float *)(smem + A.lds_local);
-    float *qb = sig + A.lds_local;
-    unsigned long long *tcur = (unsigned long long *)(qb + A.lds_local + (A.lds_local & 1));
+    float2 *sq = (float2 *)(smem + A.lds_local);  // (Sigma_t, q) per band-local FSR
+    unsigned long long *tcur = smem + 2 * A.lds_local;
     unsigned long long *tsfl = tcur + A.lds_lsurf;
     __shared__ unsigned int s_bound;
 
@@ -97,14 +97,12 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) *
                          (1.0 / (tr * FPI));
         const float q32 = (float)q;
-        sig[l] = (float)tr;
-        qb[l] = q32;
+        sq[l] = make_float2((float)tr, q32);
         tal[l] = 0ull;
         lmax = max(lmax, __float_as_uint(fabsf(q32)));
     }
     if (tid == 0) {
-        sig[B.n_local] = 0.0f;
-        qb[B.n_local] = 0.0f;
+        sq[B.n_local] = make_float2(0.0f, 0.0f);
         tal[B.n_local] = 0ull;
     }
     if (CUR) {
@@ -128,13 +126,14 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     const float *__restrict__ psi_in = A.psi + gbase + (size_t)par * A.bcpg;
     float *__restrict__ psi_out = A.psi + gbase + (size_t)(1 - par) * A.bcpg;
 
-    float psi[NP], rs[NP], wt[NP];
+    // rk[p] = 1000 / sin(theta_p): tau * 1000 is the table coordinate
+    float psi[NP], rk[NP], wt[NP];
     float cw[NP][2], fw[NP][2];
     RayInfo r{0, 0, 0, 0};
     float wsum = 0.0f, cwsum = 0.0f;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
-        rs[p] = A.ang_rsin[B.azi * NP + p];
+        rk[p] = A.ang_rsin[B.azi * NP + p] * 1000.0f;
         wt[p] = A.ang_wt[B.azi * NP + p];
         wsum += wt[p];
         psi[p] = 0.0f;
@@ -162,18 +161,18 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     // |dpsi| <= 2 * max(|psi_in|, |q|) since psi relaxes towards q; the
     // per-contribution bound times the contribution count bounds every sum.
     const double psi_bound = (double)__uint_as_float(s_bound);
-    const double scale = fixed_scale(2.0 * psi_bound * (double)wsum * (double)B.max_contrib * 1.0001);
-    double cscale = 1.0;
-    int ep = 0, ee = 0;
+    const int kx = fixed_exp(2.0 * psi_bound * (double)wsum * (double)B.max_contrib * 1.0001);
+    int kc = 0, ep = 0, ee = 0;
     if (CUR) {
-        cscale = fixed_scale(psi_bound * (double)cwsum * (double)B.max_surf_contrib * 1.0001);
+        kc = fixed_exp(psi_bound * (double)cwsum * (double)B.max_surf_contrib * 1.0001);
         if (active) {
             ep = A.ev_off[2 * (B.ray_base + j) + dir];
             ee = A.ev_off[2 * (B.ray_base + j) + dir + 1];
         }
     }
     const float csign = dir ? -1.0f : 1.0f;
-    const float2 *__restrict__ tab = A.exptab;
+    // the table is addressed by 32-bit byte offsets from a uniform base
+    const char *__restrict__ tab = (const char *)A.exptab;
     const int n = r.nseg;
     const int nr = B.n_rays;
 
@@ -186,8 +185,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
             cv = fmaf(psi[p], nn ? cw[p][1] : cw[p][0], cv);
             sv = fmaf(psi[p], nn ? fw[p][1] : fw[p][0], sv);
         }
-        atomicAdd(&tcur[ls], to_fixed(csign * cv, cscale));
-        atomicAdd(&tsfl[ls], to_fixed(sv, cscale));
+        atomicAdd(&tcur[ls], to_fixed(csign * cv, kc));
+        atomicAdd(&tsfl[ls], to_fixed(sv, kc));
     };
 
     // chunked segment stream of this lane's ray and direction
@@ -203,23 +202,15 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         cb = lenp[2 * j + 1];
     }
     for (int c = 0; c < nchunk; c++) {
-        uint4 nl = cl;
-        float4 na = ca, nb = cb;
-        if (c + 1 < nchunk) {
-            const size_t ci = (size_t)(c + 1) * nr + j;
-            nl = locp[ci];
-            na = lenp[2 * ci];
-            nb = lenp[2 * ci + 1];
-        }
-        int loc[kChunk];
+        unsigned int loc[kChunk];
         float len[kChunk];
-        loc[0] = cl.x & 0xFFFF;
+        loc[0] = cl.x & 0xFFFFu;
         loc[1] = cl.x >> 16;
-        loc[2] = cl.y & 0xFFFF;
+        loc[2] = cl.y & 0xFFFFu;
         loc[3] = cl.y >> 16;
-        loc[4] = cl.z & 0xFFFF;
+        loc[4] = cl.z & 0xFFFFu;
         loc[5] = cl.z >> 16;
-        loc[6] = cl.w & 0xFFFF;
+        loc[6] = cl.w & 0xFFFFu;
         loc[7] = cl.w >> 16;
         len[0] = ca.x;
         len[1] = ca.y;
@@ -229,15 +220,32 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         len[5] = cb.y;
         len[6] = cb.z;
         len[7] = cb.w;
-        // exponentials of the whole chunk first: they do not depend on psi
-        float q[kChunk], ex[kChunk][NP];
+        // Phase 1: every table gather of the chunk is issued before any is
+        // used (the exponentials do not depend on psi), then the next chunk's
+        // segment data, so waiting on the gathers never waits on it.
+        float q[kChunk], fr[kChunk][NP];
+        float2 hs[kChunk][NP];
 #pragma unroll
         for (int u = 0; u < kChunk; u++) {
-            const float tau0 = sig[loc[u]] * len[u];
-            q[u] = qb[loc[u]];
+            const float2 e = sq[loc[u]];
+            q[u] = e.y;
+            const float tau0 = e.x * len[u];
 #pragma unroll
-            for (int p = 0; p < NP; p++) ex[u][p] = one_minus_exp(tau0 * rs[p], tab);
+            for (int p = 0; p < NP; p++) {
+                const float w = fminf(tau0 * rk[p], kWMax);
+                const unsigned int jj = (unsigned int)w;
+                fr[u][p] = __builtin_amdgcn_fractf(w);
+                hs[u][p] = *(const float2 *)(tab + (jj << 3));
+            }
         }
+        if (c + 1 < nchunk) {
+            const size_t ci = (size_t)(c + 1) * nr + j;
+            cl = locp[ci];
+            ca = lenp[2 * ci];
+            cb = lenp[2 * ci + 1];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // Phase 2: the psi recurrence and the tallies.
 #pragma unroll
         for (int u = 0; u < kChunk; u++) {
             if (CUR) {
@@ -250,15 +258,13 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
             float cacc = 0.0f;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
-                const float d = (psi[p] - q[u]) * ex[u][p];
+                const float ex = fmaf(hs[u][p].y, fr[u][p], hs[u][p].x);
+                const float d = (psi[p] - q[u]) * ex;
                 psi[p] = psi[p] - d;
                 cacc = fmaf(d, wt[p], cacc);
             }
-            atomicAdd(&tal[loc[u]], to_fixed(cacc, scale));
+            atomicAdd(&tal[loc[u]], to_fixed(cacc, kx));
         }
-        cl = nl;
-        ca = na;
-        cb = nb;
     }
     if (CUR && active) {
         while (ep < ee) {
@@ -276,11 +282,11 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         }
     }
     __syncthreads();
-    const double rscale = 1.0 / scale;
+    const double rscale = ldexp(1.0, -kx);
     double *__restrict__ ps = A.psum + (size_t)g * A.n_slots + B.loc_base;
     for (int l = tid; l < B.n_local; l += kWG) ps[l] = (double)(long long)tal[l] * rscale;
     if (CUR) {
-        const double rc = 1.0 / cscale;
+        const double rc = ldexp(1.0, -kc);
         double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
         double *__restrict__ ss = A.ssum + (size_t)g * A.n_lslots + B.lsurf_base;
         for (int l = tid; l < B.n_lsurf; l += kWG) {
@@ -318,7 +324,7 @@ __global__ void k_reduce_flux(ReduceArgs A)
 
 size_t sweep_lds_bytes(const SweepArgs &a)
 {
-    return (size_t)a.lds_local * 8 + (size_t)(a.lds_local + (a.lds_local & 1)) * 8 + (size_t)a.lds_lsurf * 16;
+    return (size_t)a.lds_local * 16 + (size_t)a.lds_lsurf * 16;
 }
 
 void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStream_t s)

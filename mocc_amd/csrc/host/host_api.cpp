@@ -140,6 +140,27 @@ int mocc_host_quadrature_xml(const char *ang_quad_xml, double *out, int32_t cap)
     });
 }
 
+int mocc_host_quadrature_modify(const char *ang_quad_xml, int32_t n_mod, const int32_t *idx,
+                                const double *alpha, double *out, int32_t cap)
+{
+    return guard([&] {
+        auto doc = xml_parse_string(ang_quad_xml);
+        const XmlNode *n = doc->child("ang_quad");
+        if (!n) throw Error("no <ang_quad> element");
+        AngularQuadrature q(*n);
+        for (int i = 0; i < n_mod; i++) {
+            if (idx[i] < 0 || idx[i] >= q.ndir()) throw Error("angle index out of range");
+            Angle a = q[idx[i]];
+            a.modify_alpha(alpha[i]);
+            q.modify_angle(idx[i], a);
+        }
+        q.update_weights();
+        if (q.ndir() > cap) return (int)MOCGPU_ERR_CAPACITY;
+        for (int i = 0; i < q.ndir(); i++) angle_row(q[i], out + 7 * i);
+        return q.ndir();
+    });
+}
+
 int mocc_host_trace_ray(const mocc_host_problem *h, double x1, double y1, double x2, double y2,
                         int32_t *seg_fsr, double *seg_len, int32_t cap, int32_t *cm_head,
                         int32_t *cm, int32_t cm_cap)

@@ -95,6 +95,7 @@ def host() -> C.CDLL:
         L.mocc_host_get_options.argtypes = [P, C.POINTER(MocgpuOptions)]
         L.mocc_host_quadrature.argtypes = [P, P, i32]
         L.mocc_host_quadrature_xml.argtypes = [C.c_char_p, P, i32]
+        L.mocc_host_quadrature_modify.argtypes = [C.c_char_p, i32, P, P, P, i32]
         L.mocc_host_trace_ray.argtypes = [P, f64, f64, f64, f64, P, P, i32, P, P, i32]
         L.mocc_host_fsr_areas.argtypes = [P, P, i32]
         _host = L
@@ -135,6 +136,8 @@ def gpu() -> C.CDLL:
         L.mocgpu_cmfd_step.argtypes = [P, P]
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
+        L.mocgpu_exp_table.restype = i32
+        L.mocgpu_exp_table.argtypes = [P, i32]
         L.mocgpu_updates_per_sweep.restype = i64
         L.mocgpu_updates_per_sweep.argtypes = [P]
         L.mocgpu_time_sweeps.argtypes = [P, i32, P, P]

@@ -1,0 +1,264 @@
+"""The FP64 oracle pinned against the reference's test values and against
+closed forms and invariants. CPU only.
+
+  * exp table: src/core/tests/test_Exponential.cpp:27-92;
+  * infinite homogeneous medium: src/sweepers/moc/tests/test_MoC_IHM.cpp:99-185
+    (closed form phi = M^-1 chi, k = nu Sigma_f . phi);
+  * pin flux get/set identity: src/sweepers/moc/tests/test_MoCSweeper.cpp:58-94;
+  * per-pin neutron balance: MoCSweeper::check_balance
+    (src/sweepers/moc/moc_sweeper.cpp:479-532) at inner convergence, which
+    pins the coarse-surface current tally (moc_current_worker.hpp:202-264);
+  * CMFD on a flat-flux infinite medium reduces to the k-infinity closed form
+    (no leakage, D-hat = 0), and the BiCGSTAB solve agrees with a dense solve.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import DECKS
+from mocc_amd.native import HostProblem, gpu
+from oracle.binding import Oracle, exp_table, exp_table_node, lib
+
+REAL_FUZZ = 200 * np.finfo(float).eps  # util/fp_utils.hpp:39
+
+IHM_XML = """<mesh id="1" type="rect" pitch="1.26"><sub_x>3</sub_x><sub_y>3</sub_y></mesh>
+<pin id="1" mesh="1">1 1 1 1 1 1 1 1 1</pin>
+<lattice id="1" nx="3" ny="2">1 1 1 1 1 1</lattice>
+<assembly id="1" np="1" hz="0.5"><lattices>1</lattices></assembly>
+<core nx="1" ny="1" north="reflect" south="reflect" east="reflect" west="reflect" top="reflect" bottom="reflect">1</core>
+<material_lib path="c5g7.xsl"><material id="1" name="UO2-3.3" /></material_lib>
+<source scattering="P0" />
+<sweeper type="moc" n_inner="{n_inner}"><ang_quad type="ls" order="2" /><rays spacing="0.01" /></sweeper>"""
+
+
+# --- exponential table ----------------------------------------------------------
+
+def test_exp_table_coarse():
+    n, lo, hi = 5, -5.3, 0.0
+    for i, x in enumerate([-5.3, -4.24, -3.18, -2.12, -1.06, 0.0]):
+        assert abs(exp_table_node(i, n, lo, hi) - math.exp(x)) < REAL_FUZZ
+        assert abs(exp_table(x, n, lo, hi) - exp_table_node(i, n, lo, hi)) < REAL_FUZZ
+    assert abs(exp_table(-5.088, n, lo, hi) - 6.87479349415065e-03) < REAL_FUZZ
+    assert abs(exp_table(-2.756, n, lo, hi) - 7.29640444772866e-02) < REAL_FUZZ
+
+
+def test_exp_table_10000():
+    x = -10.0
+    while x < 0.0:
+        assert abs(exp_table(x) - math.exp(x)) < 2e-8
+        x += 0.1
+    # the linear interpolant's bound h^2/8 max|f''| on [-10, 0]
+    xs = np.linspace(-10.0, -1e-9, 20011)
+    err = max(abs(exp_table(v) - math.exp(v)) for v in xs)
+    assert err <= 1.25e-7 * (1 + 1e-9)
+
+
+def test_device_exp_table():
+    """The table the sweep kernel reads (FP32 pairs H_j, H_{j+1} - H_j in
+    tau = j / 1000) reproduces the reference table read from the other end."""
+    n = gpu().mocgpu_exp_table(None, 0)
+    assert n == 40000
+    t = np.zeros(2 * n, np.float32)
+    assert gpu().mocgpu_exp_table(t.ctypes.data_as(C.c_void_p), n) == n
+    H, S = t[0::2].astype(np.float64), t[1::2].astype(np.float64)
+    # nodes: H_j = 1 - d[N - j] for j <= N
+    for j in [0, 1, 7, 500, 4999, 9999, 10000 - 1]:
+        assert abs(H[j] - (1.0 - exp_table_node(10000 - j))) <= 6e-8
+    # slopes are the node differences
+    np.testing.assert_allclose(H[:-1] + S[:-1], H[1:], rtol=0, atol=1.2e-7)
+    rng = np.random.default_rng(7)
+    tau = np.concatenate([rng.uniform(0, 10, 4000), rng.uniform(0, 0.01, 500), rng.uniform(10, 45, 500)])
+    # the kernel's FP32 evaluation: w = min(1000 tau, 39999), j = trunc(w), f = fract(w)
+    w = np.minimum(np.float32(tau) * np.float32(1000.0), np.float32(39999.0)).astype(np.float32)
+    j = w.astype(np.int64)
+    f = (w - j.astype(np.float32)).astype(np.float32)
+    e = (t[2 * j + 1] * f + t[2 * j]).astype(np.float64)
+    ref = np.array([1.0 - exp_table(-v) if v < 10.0 else 1.0 - math.exp(-v) for v in tau])
+    assert np.abs(e - ref).max() < 3e-7
+
+
+# --- sweeps against closed forms ------------------------------------------------
+
+@pytest.fixture(scope="module")
+def ihm():
+    hp = HostProblem.from_string(IHM_XML.format(n_inner=3000), DECKS)
+    G = hp.n_group
+    tr = hp.array("xs_tr", G)
+    nf = hp.array("xs_nf", G)
+    ch = hp.array("xs_ch", G)
+    sc = hp.array("xs_sc", G * G).reshape(G, G)
+    phi = np.linalg.solve(np.diag(tr) - sc, ch)
+    return hp, phi, float(nf @ phi)
+
+
+def test_ihm_closed_form(ihm):
+    hp, phi, k = ihm
+    o = Oracle(hp, n_inner=3000)
+    o.initialize()
+    o.set_flux(np.tile(phi, (hp.n_fsr, 1)))
+    fs = o.calc_fission_source(k)
+    assert np.abs(fs - 1.0).max() < 1e-13
+    for g in range(hp.n_group):
+        o.build_source(g)
+        o.sweep(g)
+    f = o.flux()
+    assert (np.abs(f - phi) / phi).max() < 1e-12
+
+
+def test_ihm_cmfd_k_infinity(ihm):
+    """Flat flux, no leakage: one CMFD solve returns the closed-form k and
+    leaves the flat flux unchanged."""
+    hp, phi, k = ihm
+    o = Oracle(hp, n_inner=1, use_cmfd=True, cmfd_k_tol=1e-12, cmfd_psi_tol=1e-12,
+               cmfd_resid_reduction=1e-10, cmfd_max_iter=200)
+    o.initialize()
+    o.set_flux(np.tile(phi, (hp.n_fsr, 1)))
+    k1, _ = o.cmfd_step(0.9)
+    assert abs(k1 - k) < 1e-9
+    ratio = o.flux() / phi[None, :]
+    np.testing.assert_allclose(ratio, ratio[0, 0], rtol=1e-8)
+
+
+def test_pin_flux_identity(c1):
+    o = Oracle(c1, n_inner=2)
+    o.initialize()
+    o.fss_step()  # a non-flat flux
+    for g in range(c1.n_group):
+        p1 = o.pin_flux(g)
+        o.set_pin_flux(g, p1)
+        p2 = o.pin_flux(g)
+        np.testing.assert_allclose(p2, p1, rtol=4 * np.finfo(float).eps, atol=0)
+
+
+def test_set_pin_flux_rejects_nonpositive(c1):
+    o = Oracle(c1)
+    o.initialize()
+    p = o.pin_flux(0)
+    p[3] = 0.0
+    with pytest.raises(ValueError):
+        o.set_pin_flux(0, p)
+
+
+def _coarse_surf(nx, ny, cell, s):
+    """Mesh::coarse_surf numbering for one plane (src/core/mesh.cpp:99-147):
+    bottom faces, x-normal faces by rows, y-normal faces by columns, top."""
+    ix, iy = cell % nx, cell // nx
+    base = nx * ny
+    return {0: base + (nx + 1) * iy + ix + 1,          # EAST
+            2: base + (nx + 1) * iy + ix,              # WEST
+            1: base + (nx + 1) * ny + (ny + 1) * ix + iy + 1,  # NORTH
+            3: base + (nx + 1) * ny + (ny + 1) * ix + iy}[s]   # SOUTH
+
+
+def test_pin_balance_with_currents(c1):
+    """At inner convergence every pin balances: sources - removal = net
+    outflow from the coarse-surface currents."""
+    p = c1.p
+    G, F = c1.n_group, c1.n_fsr
+    nx, ny, hz = p.nx, p.ny, p.hz
+    o = Oracle(c1, n_inner=400)
+    o.initialize()
+    o.calc_fission_source(1.0)
+    mat = c1.array("fsr_mat", F, np.int64)
+    tr = c1.array("xs_tr", p.n_mat * G).reshape(-1, G)
+    sc = c1.array("xs_sc", p.n_mat * G * G).reshape(-1, G, G)
+    vol = c1.array("fsr_vol", F)
+    off = c1.array("pin_fsr_off", nx * ny + 1, np.int64)
+    cell = c1.array("pin_cell", nx * ny, np.int64)
+    xv = c1.array("x_vec", nx + 1)
+    yv = c1.array("y_vec", ny + 1)
+    for g in range(G):
+        o.build_source(g)
+        o.sweep(g, with_current=True)
+        src, _ = o.source()
+        phi = o.flux()[:, g]
+        cur, _ = o.currents(g)
+        rm = tr[mat, g] - sc[mat, g, g]
+        worst = 0.0
+        for ip in range(nx * ny):
+            c = int(cell[ip])
+            r = slice(off[ip], off[ip + 1])
+            gain = float((src[r] * vol[r]).sum())
+            b = gain - float((phi[r] * vol[r] * rm[r]).sum())
+            dx, dy = xv[c % nx + 1] - xv[c % nx], yv[c // nx + 1] - yv[c // nx]
+            b -= cur[_coarse_surf(nx, ny, c, 0)] * dy * hz
+            b -= cur[_coarse_surf(nx, ny, c, 1)] * dx * hz
+            b += cur[_coarse_surf(nx, ny, c, 2)] * dy * hz
+            b += cur[_coarse_surf(nx, ny, c, 3)] * dx * hz
+            worst = max(worst, abs(b) / gain)
+        assert worst < 1e-9, (g, worst)
+
+
+# --- CMFD linear algebra --------------------------------------------------------
+
+def _five_point(n, m, rng):
+    N = n * m
+    A = np.zeros((N, N))
+    for i in range(N):
+        x, y = i % n, i // n
+        off = 0.0
+        for dx, dy in [(1, 0), (-1, 0), (0, 1), (0, -1)]:
+            if 0 <= x + dx < n and 0 <= y + dy < m:
+                v = -rng.uniform(0.1, 1.0)
+                A[i, i + dx + dy * n] = v
+                off += abs(v)
+        A[i, i] = off + rng.uniform(0.01, 0.5)
+    return A
+
+
+def test_bicgstab_vs_dense():
+    rng = np.random.default_rng(3)
+    A = _five_point(9, 7, rng)
+    # a non-symmetric perturbation of the off-diagonals (D-hat terms)
+    A += np.triu(A, 1) * rng.uniform(-0.3, 0.3, A.shape) * (A != 0)
+    n = A.shape[0]
+    rp = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        cols.extend(nz)
+        vals.extend(A[i, nz])
+        rp[i + 1] = len(cols)
+    col = np.array(cols, np.int32)
+    val = np.array(vals)
+    b = rng.uniform(0.5, 1.5, n)
+    x = np.zeros(n)
+    it = np.zeros(1, np.int32)
+    err = np.zeros(1)
+    P = C.c_void_p
+    lib().oracle_bicgstab(n, rp.ctypes.data_as(P), col.ctypes.data_as(P), val.ctypes.data_as(P),
+                          b.ctypes.data_as(P), x.ctypes.data_as(P), 1e-12, 150,
+                          it.ctypes.data_as(P), err.ctypes.data_as(P))
+    xd = np.linalg.solve(A, b)
+    assert err[0] <= 1e-12
+    assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-10
+    assert 0 < it[0] <= 150
+
+
+def test_cmfd_matrix_structure(c1):
+    """The assembled CMFD matrix (cmfd.cpp:506-544) with D-hat = 0 on the
+    first solve is symmetric with row sums V * Sigma_r plus the boundary
+    couplings (zero for reflective faces)."""
+    o = Oracle(c1, n_inner=1, use_cmfd=True)
+    o.initialize()
+    o.fss_step()
+    o.cmfd_step(1.0)
+    p = c1.p
+    nc = p.nx * p.ny
+    rp = np.zeros(nc + 1, np.int32)
+    col = np.zeros(5 * nc, np.int32)
+    val = np.zeros(5 * nc)
+    P = C.c_void_p
+    for g in range(c1.n_group):
+        nnz = lib().oracle_cmfd_matrix(o.h, g, rp.ctypes.data_as(P), col.ctypes.data_as(P),
+                                       val.ctypes.data_as(P), 5 * nc)
+        assert nnz == rp[-1] and nnz == nc + 2 * (2 * p.nx * p.ny - p.nx - p.ny)
+        A = np.zeros((nc, nc))
+        for i in range(nc):
+            A[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+        np.testing.assert_allclose(A, A.T, rtol=1e-13, atol=0)
+        assert (np.diag(A) > 0).all()
+        off = A - np.diag(np.diag(A))
+        assert (off <= 0).all()
