@@ -211,10 +211,31 @@ int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current,
                         double *surface_flux);
 int32_t mocgpu_n_surf(const mocgpu_handle *h);
 
+/* Write the CoarseData current / surface_flux of group g (CoarseData is a
+ * plain shared struct in the reference, coarse_data.hpp:35-106); marks the
+ * radial data as present for the next CMFD solve. */
+int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current,
+                        const double *surface_flux);
+
 /* CMFD::solve (src/core/cmfd.cpp:171-279) on the device in FP64, with the
  * coarse flux taken from the current fine flux (EigenSolver::do_cmfd,
- * eigen_solver.cpp:263-291: get_pin_flux, solve, set_pin_flux). */
+ * eigen_solver.cpp:263-291: get_pin_flux, solve, set_pin_flux). The coarse
+ * mesh may hold at most 3072 pin cells. A non-positive coarse flux returns
+ * MOCGPU_ERR_NONPOS_FLUX (MoCSweeper::set_pin_flux_1g throws there). */
 int mocgpu_cmfd_step(mocgpu_handle *h, double *k);
+
+/* Outer (coarse power) iterations of the last CMFD solve. */
+int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h);
+
+/* CMFD::flux() (src/core/cmfd.hpp:76-79): the coarse flux of the last solve,
+ * [n_cell][G] with n_cell = nx * ny. */
+int mocgpu_cmfd_flux(mocgpu_handle *h, double *out);
+
+/* The assembled CMFD matrix of group g from the last solve (CMFD::setup_solve,
+ * cmfd.cpp:506-544) as CSR with sorted columns; returns the number of
+ * entries (<= cap) or < 0. */
+int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *col,
+                       double *val, int32_t cap);
 
 /* EigenSolver::step (eigen_solver.cpp:225-253), device-resident: optional CMFD,
  * fission source, FSS step, k update. err_psi follows eigen_solver.cpp:171-184. */

@@ -4,19 +4,63 @@
 
 #include <hip/hip_runtime.h>
 
-#include "../../../include/mocgpu.h"
+#include <vector>
 
-struct mocgpu_handle;
+#include "../../../include/mocgpu.h"
 
 namespace mocc_amd {
 namespace gpu {
 
+// Device arrays of the sweeper the CMFD reads and writes (owned by the
+// sweeper handle). Layouts: phi [F][G], cur / sfl [n_surf][G].
+struct CmfdView {
+    int F, G, npin, n_mat;
+    const int32_t *fsr_mat, *pin_off, *pin_cell;
+    const double *area, *vol, *pin_area;
+    const double *mat_tr, *mat_nf, *mat_f, *mat_ch, *mat_sc;  // [m][g], sc [m][to][from]
+    double *phi, *cur, *sfl;
+    double *k;  // device scalar, in/out
+    int has_radial;
+};
+
 class CmfdSolver {
 public:
+    // Largest coarse mesh the single-workgroup solve holds in registers
+    // (512 lanes x 6 cells).
+    static constexpr int kMaxCells = 3072;
+
+    ~CmfdSolver();
     void setup(const mocgpu_problem &p, const mocgpu_options &opt, hipStream_t s);
     void initialize(hipStream_t s);
-    // EigenSolver::do_cmfd: get_pin_flux, CMFD::solve(k), set_pin_flux.
-    int step(mocgpu_handle &h, double &k);
+    // EigenSolver::do_cmfd (eigen_solver.cpp:263-291) enqueued on s: coarse
+    // flux from the fine flux, CMFD::solve(k) with k on the device, then the
+    // fine flux rescaled pin by pin. Nothing is read back.
+    void enqueue_step(const CmfdView &v, hipStream_t s);
+    // Outer iterations of the last solve and the non-positive-flux flag of
+    // the last rescale (read back; synchronizes s).
+    void read_info(hipStream_t s, int *iterations, int *bad_flux);
+
+    int n_cell() const { return nc_; }
+    // Assembled coefficients of group g after the last solve, rows in cell
+    // order with the columns of each row sorted (CSR, for tests).
+    void matrix_csr(int g, hipStream_t s, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
+                    std::vector<double> &val);
+    void coarse_flux(hipStream_t s, std::vector<double> &out);
+
+private:
+    void free_all();
+    int nx_ = 0, ny_ = 0, nc_ = 0, G_ = 0, n_surf_ = 0;
+    double hz_ = 0.0;
+    int bc_[6] = {0, 0, 0, 0, 0, 0};
+    mocgpu_options opt_{};
+    double *d_dx_ = nullptr, *d_dy_ = nullptr, *d_cvol_ = nullptr;
+    double *d_cflux_ = nullptr;                                             // [nc][G]
+    double *d_htr_ = nullptr, *d_hnf_ = nullptr, *d_hf_ = nullptr, *d_hch_ = nullptr, *d_hrm_ = nullptr;  // [nc][G]
+    double *d_hsc_ = nullptr;                                               // [nc][G][G]
+    double *d_dtil_ = nullptr, *d_stil_ = nullptr, *d_dhat_ = nullptr, *d_shat_ = nullptr;  // [n_surf][G]
+    double *d_coef_ = nullptr;  // [5][G][nc]: south, west, diagonal, east, north
+    int *d_info_ = nullptr;     // [0] outer iterations, [1] non-positive pin flux seen
+    int n_solve_ = 0;
 };
 
 }  // namespace gpu

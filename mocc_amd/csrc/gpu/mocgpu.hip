@@ -292,6 +292,7 @@ struct mocgpu_handle {
     uint64_t parity = 0;
     bool has_fs = false;
     bool has_radial = false;
+    int cmfd_iterations = 0;
 
     // plan
     DBuf<BandInfo> d_bands;
@@ -460,6 +461,46 @@ struct mocgpu_handle {
     }
 
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+    CmfdView cmfd_view()
+    {
+        CmfdView v{};
+        v.F = F;
+        v.G = G;
+        v.npin = npin;
+        v.n_mat = n_mat;
+        v.fsr_mat = d_fsr_mat.p;
+        v.pin_off = d_pin_off.p;
+        v.pin_cell = d_pin_cell.p;
+        v.area = d_area.p;
+        v.vol = d_vol.p;
+        v.pin_area = d_pin_area.p;
+        v.mat_tr = d_mat_tr.p;
+        v.mat_nf = d_mat_nf.p;
+        v.mat_f = d_mat_f.p;
+        v.mat_ch = d_mat_ch.p;
+        v.mat_sc = d_mat_sc.p;
+        v.phi = d_phi.p;
+        v.cur = d_cur.p;
+        v.sfl = d_sfl.p;
+        v.k = d_scal.p + S_K;
+        v.has_radial = has_radial ? 1 : 0;
+        return v;
+    }
+
+    // EigenSolver::do_cmfd with k on the device; raises on a non-positive
+    // coarse flux (set_pin_flux_1g throws there in the reference)
+    int cmfd_step_sync(int *iterations)
+    {
+        cmfd->enqueue_step(cmfd_view(), stream);
+        int bad = 0;
+        cmfd->read_info(stream, iterations, &bad);
+        if (bad) {
+            err = "Negative or zero input flux in the CMFD rescale";
+            return MOCGPU_ERR_NONPOS_FLUX;
+        }
+        return 0;
+    }
 };
 
 namespace {
@@ -864,6 +905,26 @@ int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *su
     });
 }
 
+int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current, const double *surface_flux)
+{
+    return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        std::vector<double> c((size_t)h->n_surf * h->G), s((size_t)h->n_surf * h->G);
+        HIP_CHECK(hipMemcpyAsync(c.data(), h->d_cur.p, c.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_CHECK(hipMemcpyAsync(s.data(), h->d_sfl.p, s.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        for (int i = 0; i < h->n_surf; i++) {
+            if (current) c[(size_t)i * h->G + g] = current[i];
+            if (surface_flux) s[(size_t)i * h->G + g] = surface_flux[i];
+        }
+        HIP_CHECK(hipMemcpyAsync(h->d_cur.p, c.data(), c.size() * 8, hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(hipMemcpyAsync(h->d_sfl.p, s.data(), s.size() * 8, hipMemcpyHostToDevice, h->stream));
+        h->sync();
+        h->has_radial = true;
+        return 0;
+    });
+}
+
 int32_t mocgpu_exp_table(float *out, int32_t cap)
 {
     std::vector<float> t;
@@ -910,12 +971,8 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double
 int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
 {
     return guard(h, [&] {
-        if (h->opt.use_cmfd) {
-            double k = st->k;
-            int rc = h->cmfd->step(*h, k);
-            if (rc) return rc;
-            HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, &k, 8, hipMemcpyHostToDevice, h->stream));
-        }
+        HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, &st->k, 8, hipMemcpyHostToDevice, h->stream));
+        if (h->opt.use_cmfd) h->cmfd->enqueue_step(h->cmfd_view(), h->stream);
         hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G, h->d_fsr_mat.p,
                            h->d_mat_nf.p, h->d_phi.p, h->d_scal.p + S_K, 0.0, h->d_fs.p);
         h->has_fs = true;
@@ -930,6 +987,14 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
         double sc[S_COUNT];
         HIP_CHECK(hipMemcpyAsync(sc, h->d_scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
         h->sync();
+        if (h->opt.use_cmfd) {
+            int bad = 0;
+            h->cmfd->read_info(h->stream, &h->cmfd_iterations, &bad);
+            if (bad) {
+                h->err = "Negative or zero input flux in the CMFD rescale";
+                return MOCGPU_ERR_NONPOS_FLUX;
+            }
+        }
         st->k = sc[S_K];
         st->k_prev = sc[S_KPREV];
         st->err_k = sc[S_ERRK];
@@ -976,7 +1041,44 @@ int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol, int32_t m
 
 int mocgpu_cmfd_step(mocgpu_handle *h, double *k)
 {
-    return guard(h, [&] { return h->cmfd->step(*h, *k); });
+    return guard(h, [&] {
+        if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
+        HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, k, 8, hipMemcpyHostToDevice, h->stream));
+        int rc = h->cmfd_step_sync(&h->cmfd_iterations);
+        if (rc) return rc;
+        HIP_CHECK(hipMemcpyAsync(k, h->d_scal.p + S_K, 8, hipMemcpyDeviceToHost, h->stream));
+        h->sync();
+        return 0;
+    });
+}
+
+int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h) { return h->cmfd_iterations; }
+
+int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
+{
+    return guard(h, [&] {
+        if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
+        std::vector<double> f;
+        h->cmfd->coarse_flux(h->stream, f);
+        std::copy(f.begin(), f.end(), out);
+        return 0;
+    });
+}
+
+int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *col, double *val, int32_t cap)
+{
+    return guard(h, [&] {
+        if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        std::vector<int32_t> rp, cl;
+        std::vector<double> vl;
+        h->cmfd->matrix_csr(g, h->stream, rp, cl, vl);
+        if ((int)cl.size() > cap) return (int)MOCGPU_ERR_CAPACITY;
+        std::copy(rp.begin(), rp.end(), row_ptr);
+        std::copy(cl.begin(), cl.end(), col);
+        std::copy(vl.begin(), vl.end(), val);
+        return (int)cl.size();
+    });
 }
 
 }  // extern "C"

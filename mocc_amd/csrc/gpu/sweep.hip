@@ -296,24 +296,36 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     }
 }
 
-// phi = T / (St V) + 4 pi q (moc_sweeper_kernel.inc.hpp:165-173), T summed
-// over the FSR's band slots in a fixed order.
+// phi = T / (St V) + 4 pi q (moc_sweeper_kernel.inc.hpp:165-173). T is the
+// sum of the FSR's band slots: kRedLanes lanes per (FSR, group) take the
+// slots s = first + lane, first + lane + kRedLanes, ... and an xor butterfly
+// adds the lane sums; every lane pairs the same operands, so all lanes hold
+// the same T and the order is fixed.
+constexpr int kRedLanes = 8;
+
 __global__ void k_reduce_flux(ReduceArgs A)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int item = gid / kRedLanes, lane = gid % kRedLanes;
     int f, g;
     if (A.g_fixed >= 0) {
-        f = idx;
+        f = item;
         g = A.g_fixed;
     } else {
-        f = idx / A.G;
-        g = idx % A.G;
+        f = item / A.G;
+        g = item % A.G;
     }
-    if (f >= A.F) return;
+    const bool valid = f < A.F;
     const int G = A.G;
-    const double *__restrict__ ps = A.psum + (size_t)g * A.n_slots;
     double T = 0.0;
-    for (int s = A.slot_ptr[f]; s < A.slot_ptr[f + 1]; s++) T += ps[A.slot_idx[s]];
+    if (valid) {
+        const double *__restrict__ ps = A.psum + (size_t)g * A.n_slots;
+        const int s1 = A.slot_ptr[f + 1];
+        for (int s = A.slot_ptr[f] + lane; s < s1; s += kRedLanes) T += ps[A.slot_idx[s]];
+    }
+#pragma unroll
+    for (int o = 1; o < kRedLanes; o <<= 1) T += __shfl_xor(T, o, kRedLanes);
+    if (!valid || lane != 0) return;
     const int m = A.fsr_mat[f];
     const double tr = A.mat_tr[m * G + g];
     const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) * (1.0 / (tr * FPI));
@@ -356,8 +368,8 @@ int sweep_grid_items(int n_bands, int G, bool all_groups)
 
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s)
 {
-    int n = a.g_fixed >= 0 ? a.F : a.F * a.G;
-    hipLaunchKernelGGL(k_reduce_flux, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    const size_t n = (size_t)(a.g_fixed >= 0 ? a.F : a.F * a.G) * kRedLanes;
+    hipLaunchKernelGGL(k_reduce_flux, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
 }
 
 }  // namespace gpu

@@ -645,7 +645,13 @@ int PinMesh::trace(const Point2 &p1, const Point2 &p2, int first_reg, VecF &s, V
     for (size_t ip = 1; ip < ps.size(); ip++) {
         s.push_back(ps[ip].distance(ps[ip - 1]));
         int lr = find_reg(Midpoint(ps[ip], ps[ip - 1]));
-        if (lr < 0 || lr >= n_reg) throw Error("Ray segment outside of its pin mesh");
+        // A sliver along the pin edge (length ~1e-13, from the rounding of
+        // pin-local coordinates) can have its midpoint just outside the pin:
+        // find_reg gives -1 and the reference stores -1 + first_reg, i.e. the
+        // FSR before this pin (pin_mesh_cyl.cpp:213-217, unchecked). Same here;
+        // where that index would be negative the reference reads out of
+        // bounds, so that case is an error.
+        if (lr >= n_reg || lr + first_reg < 0) throw Error("Ray segment outside of its pin mesh");
         reg.push_back(lr + first_reg);
     }
     return (int)ps.size() - 1;

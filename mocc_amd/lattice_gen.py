@@ -208,8 +208,11 @@ def c2_deck() -> Deck:
 
 def c3_deck() -> Deck:
     """3x3 assemblies of 17x17 (3 material sets), 7 groups, CY 64x3, 0.02 cm,
-    vacuum east/south, reflective west/north, CMFD on."""
-    d = Deck("c3_core3x3_7g", 7, 64, 0.02, n_sets=3, cmfd=True,
+    vacuum east/south, reflective west/north, CMFD on, 5 inner iterations
+    (as in mocc's example inputs: with a single inner iteration the D-hat of
+    the first, unconverged sweep makes the coarse matrix lose its positive
+    diagonal on this deck and mocc's CMFD iteration diverges)."""
+    d = Deck("c3_core3x3_7g", 7, 64, 0.02, n_sets=3, cmfd=True, n_inner=5,
              bc={"east": "vacuum", "south": "vacuum"})
     d.lattices = {s + 1: (17, 17, lattice_rows(17, s, True)) for s in range(3)}
     d.core = [[1, 2, 3], [2, 3, 1], [3, 1, 2]]

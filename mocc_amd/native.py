@@ -131,9 +131,14 @@ def gpu() -> C.CDLL:
         L.mocgpu_get_pin_flux_1g.argtypes = [P, i32, P]
         L.mocgpu_set_pin_flux_1g.argtypes = [P, i32, P, P]
         L.mocgpu_get_currents.argtypes = [P, i32, P, P]
+        L.mocgpu_set_currents.argtypes = [P, i32, P, P]
         L.mocgpu_n_surf.restype = i32
         L.mocgpu_n_surf.argtypes = [P]
         L.mocgpu_cmfd_step.argtypes = [P, P]
+        L.mocgpu_cmfd_iterations.restype = i32
+        L.mocgpu_cmfd_iterations.argtypes = [P]
+        L.mocgpu_cmfd_flux.argtypes = [P, P]
+        L.mocgpu_cmfd_matrix.argtypes = [P, i32, P, P, P, i32]
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_exp_table.restype = i32

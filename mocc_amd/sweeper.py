@@ -141,11 +141,37 @@ class GpuMoCSweeper:
         self._check(gpu().mocgpu_get_currents(self.h, g, ptr(c), ptr(s)))
         return c, s
 
+    def set_currents(self, g: int, current, surface_flux):
+        c = np.ascontiguousarray(current, dtype=np.float64)
+        s = np.ascontiguousarray(surface_flux, dtype=np.float64)
+        self._check(gpu().mocgpu_set_currents(self.h, g, ptr(c), ptr(s)))
+
     # --- CMFD / EigenSolver -------------------------------------------------
     def cmfd_step(self, k: float) -> float:
+        """EigenSolver::do_cmfd: returns the CMFD eigenvalue."""
         kk = np.array([k])
         self._check(gpu().mocgpu_cmfd_step(self.h, ptr(kk)))
         return float(kk[0])
+
+    def cmfd_iterations(self) -> int:
+        return int(gpu().mocgpu_cmfd_iterations(self.h))
+
+    def cmfd_flux(self) -> np.ndarray:
+        """CMFD::flux(): coarse flux [n_cell, G] of the last solve."""
+        out = np.zeros((self.npin, self.G))
+        self._check(gpu().mocgpu_cmfd_flux(self.h, ptr(out)))
+        return out
+
+    def cmfd_matrix(self, g: int):
+        """Assembled CMFD matrix of group g (CSR with sorted columns)."""
+        n = self.npin
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(5 * n, np.int32)
+        val = np.zeros(5 * n)
+        nnz = gpu().mocgpu_cmfd_matrix(self.h, g, ptr(rp), ptr(col), ptr(val), 5 * n)
+        if nnz < 0:
+            self._check(nnz)
+        return rp, col[:nnz], val[:nnz]
 
     def eigen_solve(self, k_tol: float, psi_tol: float, max_iter: int, min_iter: int = 0):
         st = EigenState()

@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
         L.oracle_set_pin_flux_1g.argtypes = [P, i32, P, P]
         L.oracle_cmfd_step.argtypes = [P, P, P]
         L.oracle_cmfd_matrix.argtypes = [P, i32, P, P, P, i32]
+        L.oracle_cmfd_flux.argtypes = [P, P]
         L.oracle_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState), P, i32]
         L.oracle_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.oracle_exp_table.restype = f64
@@ -185,6 +186,22 @@ class Oracle:
         if rc:
             raise RuntimeError(lib().oracle_last_error().decode())
         return float(kk[0]), int(it[0])
+
+    def cmfd_flux(self):
+        p = self.prob.p
+        out = np.zeros((p.nx * p.ny, self.G))
+        lib().oracle_cmfd_flux(self.h, _p(out))
+        return out
+
+    def cmfd_matrix(self, g):
+        n = self.prob.p.nx * self.prob.p.ny
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(5 * n, np.int32)
+        val = np.zeros(5 * n)
+        nnz = lib().oracle_cmfd_matrix(self.h, g, _p(rp), _p(col), _p(val), 5 * n)
+        if nnz < 0:
+            raise RuntimeError("no CMFD matrix assembled yet")
+        return rp, col[:nnz], val[:nnz]
 
     def eigen_solve(self, k_tol, psi_tol, max_iter, min_iter=0):
         st = EigenState()

@@ -1177,6 +1177,11 @@ int oracle_cmfd_step(oracle_handle *h, double *k, int32_t *iters)
         return MOCGPU_ERR_ARG;
     }
 }
+int oracle_cmfd_flux(oracle_handle *h, double *out)
+{
+    std::copy(h->o.cflux.begin(), h->o.cflux.end(), out);
+    return (int)h->o.cflux.size();
+}
 int oracle_cmfd_matrix(oracle_handle *h, int32_t g, int32_t *row_ptr, int32_t *col, double *val, int32_t cap)
 {
     Oracle &o = h->o;

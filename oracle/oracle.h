@@ -57,6 +57,7 @@ int oracle_set_pin_flux_1g(oracle_handle *h, int32_t g, const double *pin, doubl
 int oracle_cmfd_step(oracle_handle *h, double *k, int32_t *iters);
 /* CMFD linear system of group g after setup: CSR (row_ptr[n+1], col[nnz], val[nnz]). */
 int oracle_cmfd_matrix(oracle_handle *h, int32_t g, int32_t *row_ptr, int32_t *col, double *val, int32_t cap);
+int oracle_cmfd_flux(oracle_handle *h, double *out);
 int oracle_eigen_solve(oracle_handle *h, double k_tol, double psi_tol, int32_t max_iter,
                        int32_t min_iter, mocgpu_eigen_state *st, double *k_hist, int32_t hist_cap);
 int oracle_eigen_iterate(oracle_handle *h, mocgpu_eigen_state *st);

@@ -30,3 +30,19 @@ def c1_path(deck_dir):
 def c1(c1_path):
     from mocc_amd.native import HostProblem
     return HostProblem.load(c1_path)
+
+
+@pytest.fixture(scope="session")
+def cmfd_decks(deck_dir):
+    """Decks for the CMFD and eigenvalue parity tests: C1 itself, one 17x17
+    assembly (289 cells) and the 3x3-assembly core geometry of C3 (51x51
+    cells, vacuum east/south) with coarser rays so the FP64 oracle finishes
+    in seconds."""
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    paths = {
+        "c1": lg.write_config("c1", deck_dir),
+        "asm17": lg.write_config("c2", deck_dir, name="asm17_cmfd", n_azimuthal=4, spacing=0.1, groups=2),
+        "core51": lg.write_config("c3", deck_dir, name="core51_cmfd", n_azimuthal=4, spacing=0.1),
+    }
+    return {k: HostProblem.load(v) for k, v in paths.items()}

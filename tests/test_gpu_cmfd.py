@@ -1,0 +1,83 @@
+"""Device CMFD (FP64) against the oracle's CMFD::solve restatement
+(src/core/cmfd.cpp:171-638, eigen_solver.cpp:263-291) on identical inputs:
+the fine flux and the coarse-surface currents of three oracle outer
+iterations are loaded into the device handle, then one do_cmfd runs on each.
+
+With tight CMFD tolerances both converge to the same coarse solution, so the
+matrix, k, the coarse flux and the rescaled fine flux agree to ~1e-12
+relative (bar 1e-8); with the default tolerances k agrees to the tolerance
+(bar 1e-6).
+"""
+import numpy as np
+import pytest
+
+from oracle.binding import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(cmfd_k_tol=1e-12, cmfd_psi_tol=1e-12, cmfd_resid_reduction=1e-10, cmfd_max_iter=500)
+
+
+def _oracle_state(hp, outer=3, **cmfd):
+    ora = Oracle(hp, n_inner=hp.params().n_inner, use_cmfd=True, **cmfd)
+    ora.initialize()
+    ora.calc_fission_source(1.0)
+    for _ in range(outer):
+        ora.fss_step(with_current=True)
+    return ora
+
+
+def _device_from(hp, ora, **cmfd):
+    from mocc_amd.sweeper import GpuMoCSweeper
+    dev = GpuMoCSweeper(hp, use_cmfd=True, **cmfd)
+    dev.initialize()
+    dev.set_flux(ora.flux())
+    for g in range(hp.n_group):
+        dev.set_currents(g, *ora.currents(g))
+    return dev
+
+
+def _check_pair(hp, tight=True):
+    cm = TIGHT if tight else {}
+    print(f"case F={hp.n_fsr} G={hp.n_group} cells={hp.p.nx * hp.p.ny} tight={tight}", flush=True)
+    ora = _oracle_state(hp, **cm)
+    dev = _device_from(hp, ora, **cm)
+    kd = dev.cmfd_step(1.0)
+    ko, _ = ora.cmfd_step(1.0)
+    return dev, ora, kd, ko
+
+
+@pytest.mark.parametrize("deck", ["c1", "asm17", "core51"])
+def test_cmfd_step_parity(deck, cmfd_decks):
+    hp = cmfd_decks[deck]
+    dev, ora, kd, ko = _check_pair(hp, tight=True)
+    for g in range(hp.n_group):
+        rd, cd, vd = dev.cmfd_matrix(g)
+        ro, co, vo = ora.cmfd_matrix(g)
+        assert np.array_equal(rd, ro) and np.array_equal(cd, co)
+        np.testing.assert_allclose(vd, vo, rtol=1e-12, atol=0)
+    assert abs(kd - ko) < 1e-10, (kd, ko)
+    np.testing.assert_allclose(dev.cmfd_flux(), ora.cmfd_flux(), rtol=1e-8, atol=0)
+    np.testing.assert_allclose(dev.flux(), ora.flux(), rtol=1e-8, atol=0)
+    for g in range(hp.n_group):
+        c_d, s_d = dev.currents(g)
+        c_o, s_o = ora.currents(g)
+        scale = np.abs(c_o).max()
+        assert np.abs(c_d - c_o).max() <= 1e-8 * scale
+        np.testing.assert_allclose(s_d, s_o, rtol=1e-8, atol=1e-14 * np.abs(s_o).max())
+
+
+@pytest.mark.parametrize("deck", ["c1", "core51"])
+def test_cmfd_step_default_tolerances(deck, cmfd_decks):
+    hp = cmfd_decks[deck]
+    dev, ora, kd, ko = _check_pair(hp, tight=False)
+    assert abs(kd - ko) < 1e-6, (kd, ko)
+    assert dev.cmfd_iterations() >= 1
+
+
+def test_cmfd_rejects_without_option(c1):
+    from mocc_amd.sweeper import DeviceError, GpuMoCSweeper
+    dev = GpuMoCSweeper(c1, n_inner=1, use_cmfd=False)
+    dev.initialize()
+    with pytest.raises(DeviceError):
+        dev.cmfd_step(1.0)
