@@ -67,6 +67,12 @@ int mocc_host_trace_ray(const mocc_host_problem *h, double x1, double y1, double
 /* Geometry-only decks (no <sweeper>): build only the core mesh, for ray tests. */
 int mocc_host_load_mesh_string(const char *xml, const char *base_dir, mocc_host_problem **out);
 
+/* Replace the cross sections of a loaded deck by those of another XSL file
+ * that defines the same material names (cross-section branch cases); the
+ * geometry and the rays are kept and mocc_host_view() reflects the new data.
+ * Same library reader as the deck's <material_lib> (material_lib.cpp). */
+int mocc_host_reload_xs(mocc_host_problem *h, const char *xsl_path);
+
 /* Total FSR area per region of the plane, [n_fsr] (the true volumes the
  * FLAT correction preserves). */
 int mocc_host_fsr_areas(const mocc_host_problem *h, double *out, int32_t cap);

@@ -146,6 +146,20 @@ void mocgpu_default_options(mocgpu_options *opt);
 int mocgpu_create(const mocgpu_problem *prob, const mocgpu_options *opt,
                   mocgpu_handle **out);
 int mocgpu_destroy(mocgpu_handle *h);
+
+/* A handle for another problem on the SAME geometry as `like` (same deck
+ * geometry and ray parameters; cross sections and group count may differ, as
+ * for cross-section branch cases): the ray plan, segment streams and FSR/pin
+ * arrays are shared with `like` on the device, only cross sections and state
+ * are new. The shared memory is released with the last handle using it. No
+ * reference counterpart: mocc builds one sweeper per run. */
+int mocgpu_create_like(const mocgpu_handle *like, const mocgpu_problem *p,
+                       const mocgpu_options *opt, mocgpu_handle **out);
+
+/* Device bytes held by a handle: its own state and cross sections, and the
+ * geometry it may share with other handles. */
+int64_t mocgpu_state_bytes(const mocgpu_handle *h);
+int64_t mocgpu_geometry_bytes(const mocgpu_handle *h);
 const char *mocgpu_last_error(const mocgpu_handle *h);
 
 /* MoCSweeper::initialize (moc_sweeper.cpp:234-253): phi = 1, psi_bc = 1/(4 pi)
@@ -251,6 +265,47 @@ int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol,
                        mocgpu_eigen_state *state);
 /* One outer iteration (EigenSolver::step + the convergence bookkeeping of solve). */
 int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
+
+/* ---------------------------------------------------------------------------
+ * Batch engine: many independent problems advanced together. No reference
+ * counterpart (mocc solves one problem per run); it serves cross-section
+ * branch cases and many-lattice workloads. A batch does not own its handles:
+ * each handle must stay alive until it is removed from the batch.
+ *
+ * Every unfinished problem advances by whole outer iterations of
+ * EigenSolver::step (eigen_solver.cpp:225-253) with Jacobi energy sweeps and
+ * the handle's n_inner; the convergence test is EigenSolver::solve's
+ * (k_tol, psi_tol, min_iter, max_iter). A problem's result is bitwise the same
+ * whichever other problems share the batch. CMFD is not used in a batch.
+ * ------------------------------------------------------------------------- */
+typedef struct mocgpu_batch mocgpu_batch;
+
+/* budget_bytes: admission limit on the summed mocgpu_state_bytes of the
+ * problems in the batch (<= 0: unlimited). use_graph: replay the outer
+ * iteration as a captured HIP graph (1) or launch it eagerly (0). */
+int mocgpu_batch_create(int64_t budget_bytes, int32_t use_graph, mocgpu_batch **out);
+int mocgpu_batch_destroy(mocgpu_batch *b);
+const char *mocgpu_batch_last_error(const mocgpu_batch *b);
+
+/* Admit a problem: resets it to the start of EigenSolver::solve (flat flux,
+ * k = 1) and returns its id. MOCGPU_ERR_CAPACITY when the budget would be
+ * exceeded (nothing changes). */
+int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi_tol,
+                     int32_t max_iter, int32_t min_iter, int32_t *id);
+
+/* Enqueue n_outer outer iterations of every unfinished problem (returns
+ * before they complete). */
+int mocgpu_batch_run(mocgpu_batch *b, int32_t n_outer);
+
+/* Wait for the enqueued work; report problems that finished since the last
+ * poll (converged, max_iter or NaN), up to cap. Returns the count. The flux of
+ * a finished problem stays readable with mocgpu_get_flux on its handle. */
+int mocgpu_batch_poll(mocgpu_batch *b, int32_t *ids, mocgpu_eigen_state *states, int32_t cap);
+
+/* Remove a problem (finished or not); its handle is the caller's again. */
+int mocgpu_batch_remove(mocgpu_batch *b, int32_t id);
+int32_t mocgpu_batch_size(const mocgpu_batch *b);
+int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b);
 
 /* Number of (2-D segment, polar, direction, group) updates per inner sweep of
  * all groups: 2 * P * S * G. */

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "mocc_amd")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
-GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip"]
+GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # No packed FP32 instructions (v_pk_*_f32) anywhere in device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -1,0 +1,548 @@
+// Batch engine: many independent k-eigenvalue problems (cross-section branch
+// cases, lattices of mixed size) advanced together on one device.
+//
+// One outer iteration of EigenSolver::step (eigen_solver.cpp:225-253) for
+// every unfinished problem is a fixed sequence of launches, each covering all
+// problems at once:
+//
+//   fission source -> phi_old -> group sources (Jacobi in energy) ->
+//   n_inner x [ work-queue sweep of every (problem, band, group) item ->
+//               flux reduction -> boundary parity flip ] ->
+//   per-problem reductions (total fission + k, normalization, err_psi) ->
+//   per-problem convergence test.
+//
+// The sequence is captured once per batch composition into a HIP graph and
+// replayed; problems that have converged stay in the graph and are skipped by
+// every kernel from their status word. Each problem's arithmetic (fixed-point
+// band tallies, fixed-order flux sums, one workgroup per problem for every
+// reduction) does not depend on the other problems, so a problem's result is
+// bitwise the same alone or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "handle.hpp"
+
+namespace {
+
+struct CaseDesc {
+    int32_t F, G, max_iter, min_iter;
+    double k_tol, psi_tol;
+    const int32_t *fsr_mat;
+    const double *mat_nf, *mat_ch, *mat_sc, *vol;
+    double *phi, *phi_old, *src, *fs, *fs_prev;
+    double *scal;         // [S_COUNT]
+    uint64_t *parity;     // boundary buffer parity word
+    uint64_t gmask;       // bits of the groups
+};
+
+constexpr int kBT = 256;  // threads per block of the element-wise launches
+
+__device__ __forceinline__ bool finished(const int32_t *status, int c) { return status[2 * c + 1] != 0; }
+
+// TransportSweeper::calc_fission_source (transport_sweeper.cpp:119-134), fs
+// of every unfinished problem; copy_prev also stores fs_prev = fs.
+__global__ void k_b_fission(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
+                            const int32_t *__restrict__ status, int copy_prev)
+{
+    const BlockMap b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    if (finished(status, c)) return;
+    const CaseDesc &d = cs[c];
+    const int f = b.first + threadIdx.x;
+    if (f >= d.F) return;
+    const double rk = 1.0 / d.scal[S_K];
+    const int m = d.fsr_mat[f];
+    double s = 0.0;
+    for (int g = 0; g < d.G; g++) s += rk * d.mat_nf[m * d.G + g] * d.phi[(size_t)f * d.G + g];
+    d.fs[f] = s;
+    if (copy_prev) d.fs_prev[f] = s;
+}
+
+__global__ void k_b_copy_phi(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
+                             const int32_t *__restrict__ status)
+{
+    const BlockMap b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    if (finished(status, c)) return;
+    const CaseDesc &d = cs[c];
+    const int i = b.first + threadIdx.x;
+    if (i < d.F * d.G) d.phi_old[i] = d.phi[i];
+}
+
+// Source::initialize_group + fission + in_scatter (source.cpp:41-112), all
+// groups from the previous outer iteration's flux.
+__global__ void k_b_source(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
+                           const int32_t *__restrict__ status)
+{
+    const BlockMap b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    if (finished(status, c)) return;
+    const CaseDesc &d = cs[c];
+    const int i = b.first + threadIdx.x;
+    const int G = d.G;
+    if (i >= d.F * G) return;
+    const int f = i / G, g = i % G;
+    const int m = d.fsr_mat[f];
+    double s = d.mat_ch[m * G + g] * d.fs[f];
+    const double *row = d.mat_sc + ((size_t)m * G + g) * G;
+    for (int gg = 0; gg < G; gg++) {
+        if (gg == g) continue;
+        s += row[gg] * d.phi_old[(size_t)f * G + gg];
+    }
+    d.src[i] = s;
+}
+
+__global__ void k_b_flip(const CaseDesc *__restrict__ cs, const int32_t *__restrict__ status,
+                         const int32_t *__restrict__ n_inner, int n_cases, int inner)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cases || finished(status, c) || n_inner[c] <= inner) return;
+    *cs[c].parity ^= cs[c].gmask;
+}
+
+__global__ void k_b_reset(unsigned int *counter)
+{
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+// The per-problem reductions of EigenSolver::step, one workgroup per problem
+// with a fixed tree (modes as the single-problem path: total fission and the
+// k update, Normalize, err_psi).
+__global__ __launch_bounds__(kBT) void k_b_reduce(const CaseDesc *__restrict__ cs,
+                                                  const int32_t *__restrict__ status, int mode)
+{
+    const int c = blockIdx.x;
+    if (finished(status, c)) return;
+    const CaseDesc &d = cs[c];
+    __shared__ double sh[kBT][4];
+    double acc[4] = {0, 0, 0, 0};
+    const int G = d.G;
+    double *s = d.scal;
+    for (int f = threadIdx.x; f < d.F; f += kBT) {
+        if (mode == RED_FISSION) {
+            const int m = d.fsr_mat[f];
+            for (int g = 0; g < G; g++) {
+                acc[0] += d.phi[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
+                acc[1] += d.phi_old[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
+            }
+        } else if (mode == RED_NORM) {
+            acc[0] += d.fs[f] > 0.0 ? 1.0 : 0.0;
+            acc[1] += d.fs[f];
+            acc[2] += d.fs_prev[f] > 0.0 ? 1.0 : 0.0;
+            acc[3] += d.fs_prev[f];
+        } else {
+            const double e = d.fs[f] * s[S_FA] - d.fs_prev[f] * s[S_FB];
+            acc[0] += e * e;
+        }
+    }
+    const int t = threadIdx.x;
+    for (int q = 0; q < 4; q++) sh[t][q] = acc[q];
+    __syncthreads();
+    for (int w = kBT / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 4; q++) sh[t][q] += sh[t + w][q];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    if (mode == RED_FISSION) {
+        s[S_TF_NEW] = sh[0][0];
+        s[S_TF_OLD] = sh[0][1];
+        s[S_KPREV] = s[S_K];
+        s[S_K] = s[S_K] * sh[0][0] / sh[0][1];
+        s[S_ERRK] = fabs(s[S_K] - s[S_KPREV]);
+    } else if (mode == RED_NORM) {
+        s[S_FA] = sh[0][0] / sh[0][1];
+        s[S_FB] = sh[0][2] / sh[0][3];
+    } else {
+        s[S_ERRPSI] = sqrt(sh[0][0] / s[S_NFISS]);
+    }
+}
+
+// EigenSolver::solve's test after each step (eigen_solver.cpp:162-217):
+// status[2c] counts steps, status[2c+1] becomes 1 (converged), 2 (max_iter)
+// or 3 (k not a number).
+__global__ void k_b_check(const CaseDesc *__restrict__ cs, int32_t *__restrict__ status, int n_cases)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cases || finished(status, c)) return;
+    const CaseDesc &d = cs[c];
+    const int it = ++status[2 * c];
+    const double *s = d.scal;
+    int fin = 0;
+    if (!(s[S_K] == s[S_K])) fin = 3;
+    else if (s[S_ERRK] < d.k_tol && s[S_ERRPSI] < d.psi_tol && it - 1 >= d.min_iter) fin = 1;
+    else if (it >= d.max_iter) fin = 2;
+    status[2 * c + 1] = fin;
+}
+
+std::vector<BlockMap> block_map(const std::vector<size_t> &counts)
+{
+    std::vector<BlockMap> m;
+    for (size_t c = 0; c < counts.size(); c++)
+        for (size_t first = 0; first < counts[c]; first += kBT) m.push_back(BlockMap{(int32_t)c, (int32_t)first});
+    return m;
+}
+
+}  // namespace
+
+struct mocgpu_batch {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int64_t budget = 0, used = 0;
+    bool use_graph = true;
+    struct Case {
+        mocgpu_handle *h;
+        int32_t id;
+        double k_tol, psi_tol;
+        int32_t max_iter, min_iter;
+        int32_t status[2];  // host mirror, valid after a poll
+        bool reported;
+        int64_t bytes;
+    };
+    std::vector<Case> cases;
+    int32_t next_id = 1;
+    bool dirty = true;
+    int NP = 0;
+    // device tables of the current composition
+    DBuf<CaseDesc> d_desc;
+    DBuf<SweepArgs> d_sweep;
+    DBuf<ReduceArgs> d_red;
+    DBuf<int4> d_items;
+    DBuf<BlockMap> d_bmF, d_bmFG, d_bmR;
+    DBuf<int32_t> d_status, d_ninner;
+    DBuf<uint32_t> d_counter;
+    int n_bmF = 0, n_bmFG = 0, n_bmR = 0, n_items = 0, max_inner = 1, q_blocks = 1;
+    size_t lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~mocgpu_batch()
+    {
+        drop_graph();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+    // status of every case from the device into the host mirrors
+    void pull_status()
+    {
+        if (!d_status.p || cases.empty()) return;
+        std::vector<int32_t> st(2 * cases.size());
+        HIP_CHECK(hipMemcpyAsync(st.data(), d_status.p, st.size() * 4, hipMemcpyDeviceToHost, stream));
+        sync();
+        for (size_t i = 0; i < cases.size(); i++) {
+            cases[i].status[0] = st[2 * i];
+            cases[i].status[1] = st[2 * i + 1];
+        }
+    }
+
+    void rebuild()
+    {
+        drop_graph();
+        const int n = (int)cases.size();
+        std::vector<CaseDesc> desc(n);
+        std::vector<SweepArgs> sw(n);
+        std::vector<ReduceArgs> rd(n);
+        std::vector<int32_t> st(2 * n), ni(n);
+        std::vector<size_t> cF(n), cFG(n), cR(n);
+        struct It {
+            int64_t cost;
+            int4 w;
+        };
+        std::vector<It> items;
+        lds = 0;
+        max_inner = 1;
+        for (int i = 0; i < n; i++) {
+            mocgpu_handle *h = cases[i].h;
+            CaseDesc &d = desc[i];
+            d.F = h->F;
+            d.G = h->G;
+            d.max_iter = cases[i].max_iter;
+            d.min_iter = cases[i].min_iter;
+            d.k_tol = cases[i].k_tol;
+            d.psi_tol = cases[i].psi_tol;
+            d.fsr_mat = h->d_fsr_mat.p;
+            d.mat_nf = h->d_mat_nf.p;
+            d.mat_ch = h->d_mat_ch.p;
+            d.mat_sc = h->d_mat_sc.p;
+            d.vol = h->d_vol.p;
+            d.phi = h->d_phi.p;
+            d.phi_old = h->d_phi_old.p;
+            d.src = h->d_src.p;
+            d.fs = h->d_fs.p;
+            d.fs_prev = h->d_fs_prev.p;
+            d.scal = h->d_scal.p;
+            d.parity = h->d_parity.p;
+            d.gmask = h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull);
+            SweepArgs a = h->sweep_args(-1);
+            a.parity_dev = h->d_parity.p;
+            a.lds_lsurf = 0;
+            sw[i] = a;
+            lds = std::max(lds, sweep_lds_bytes(a));
+            rd[i] = h->reduce_args(-1);
+            st[2 * i] = cases[i].status[0];
+            st[2 * i + 1] = cases[i].status[1];
+            ni[i] = std::max(1, h->opt.n_inner);
+            max_inner = std::max(max_inner, ni[i]);
+            cF[i] = (size_t)h->F;
+            cFG[i] = (size_t)h->F * h->G;
+            cR[i] = (size_t)h->F * h->G * reduce_flux_lanes();
+            const SweepPlan &pl = *h->plan;
+            for (size_t r = 0; r < pl.band_order.size(); r++) {
+                const int band = pl.band_order[r];
+                const int64_t cost = (int64_t)pl.bands[band].nmax;
+                for (int g = 0; g < h->G; g++) items.push_back(It{cost, make_int4(i, band, g, 0)});
+            }
+        }
+        // longest items first over all problems (the G items of a band stay together)
+        std::stable_sort(items.begin(), items.end(), [](const It &x, const It &y) { return x.cost > y.cost; });
+        std::vector<int4> iv(items.size());
+        for (size_t k = 0; k < items.size(); k++) iv[k] = items[k].w;
+        n_items = (int)iv.size();
+        auto bF = block_map(cF), bFG = block_map(cFG), bR = block_map(cR);
+        n_bmF = (int)bF.size();
+        n_bmFG = (int)bFG.size();
+        n_bmR = (int)bR.size();
+        d_desc.upload(desc, stream);
+        d_sweep.upload(sw, stream);
+        d_red.upload(rd, stream);
+        d_items.upload(iv, stream);
+        d_bmF.upload(bF, stream);
+        d_bmFG.upload(bFG, stream);
+        d_bmR.upload(bR, stream);
+        d_status.upload(st, stream);
+        d_ninner.upload(ni, stream);
+        if (!d_counter.p) d_counter.alloc(1);
+        int dev = 0, ncu = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        q_blocks = std::max(1, std::min(n_items, ncu * queue_blocks_per_cu(NP, lds)));
+        sync();
+        dirty = false;
+    }
+
+    // one outer iteration of every unfinished problem
+    void enqueue_outer()
+    {
+        const int n = (int)cases.size();
+        hipStream_t s = stream;
+        hipLaunchKernelGGL(k_b_fission, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, 1);
+        hipLaunchKernelGGL(k_b_copy_phi, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
+        hipLaunchKernelGGL(k_b_source, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
+        for (int inner = 0; inner < max_inner; inner++) {
+            hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, d_counter.p);
+            QueueArgs q{};
+            q.cases = d_sweep.p;
+            q.items = d_items.p;
+            q.n_items = n_items;
+            q.inner = inner;
+            q.counter = d_counter.p;
+            q.status = d_status.p;
+            q.n_inner = d_ninner.p;
+            launch_sweep_queue(q, NP, lds, q_blocks, s);
+            launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
+            hipLaunchKernelGGL(k_b_flip, dim3(nblk(n, 64)), dim3(64), 0, s, d_desc.p, d_status.p, d_ninner.p, n,
+                               inner);
+        }
+        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_FISSION);
+        hipLaunchKernelGGL(k_b_fission, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, 0);
+        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_NORM);
+        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_PSIERR);
+        hipLaunchKernelGGL(k_b_check, dim3(nblk(n, 64)), dim3(64), 0, s, d_desc.p, d_status.p, n);
+    }
+
+    void run(int n_outer)
+    {
+        if (cases.empty() || n_outer <= 0) return;
+        if (dirty) rebuild();
+        if (use_graph) {
+            if (!exec) {
+                HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                enqueue_outer();
+                HIP_CHECK(hipStreamEndCapture(stream, &graph));
+                HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
+            for (int i = 0; i < n_outer; i++) HIP_CHECK(hipGraphLaunch(exec, stream));
+        } else {
+            for (int i = 0; i < n_outer; i++) enqueue_outer();
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+
+    int find(int32_t id) const
+    {
+        for (size_t i = 0; i < cases.size(); i++)
+            if (cases[i].id == id) return (int)i;
+        return -1;
+    }
+};
+
+namespace {
+template <typename F> int bguard(mocgpu_batch *b, F &&f)
+{
+    try {
+        return f();
+    } catch (const HipError &e) {
+        if (b) b->err = e.what();
+        return MOCGPU_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        if (b) b->err = e.what();
+        return MOCGPU_ERR_ARG;
+    }
+}
+thread_local std::string g_batch_err;
+}  // namespace
+
+extern "C" {
+
+int mocgpu_batch_create(int64_t budget_bytes, int32_t use_graph, mocgpu_batch **out)
+{
+    if (!out) return MOCGPU_ERR_ARG;
+    auto b = std::make_unique<mocgpu_batch>();
+    int rc = bguard(b.get(), [&] {
+        HIP_CHECK(hipStreamCreate(&b->stream));
+        b->budget = budget_bytes;
+        b->use_graph = use_graph != 0;
+        return 0;
+    });
+    if (rc) {
+        g_batch_err = b->err;
+        return rc;
+    }
+    *out = b.release();
+    return 0;
+}
+
+int mocgpu_batch_destroy(mocgpu_batch *b)
+{
+    delete b;
+    return 0;
+}
+
+const char *mocgpu_batch_last_error(const mocgpu_batch *b) { return b ? b->err.c_str() : g_batch_err.c_str(); }
+
+int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter,
+                     int32_t min_iter, int32_t *id)
+{
+    if (!b || !h || !id) return MOCGPU_ERR_ARG;
+    return bguard(b, [&] {
+        if (h->opt.use_cmfd) {
+            b->err = "the batch engine runs problems without CMFD";
+            return (int)MOCGPU_ERR_UNSUPPORTED;
+        }
+        if (!b->cases.empty() && h->NP != b->NP) {
+            b->err = "all problems of a batch must have the same number of polar angles";
+            return (int)MOCGPU_ERR_UNSUPPORTED;
+        }
+        for (const auto &c : b->cases)
+            if (c.h == h) throw std::runtime_error("handle already in the batch");
+        // the host status mirrors must be current before the composition changes
+        b->sync();
+        if (!b->dirty) b->pull_status();
+        const int64_t bytes = mocgpu_state_bytes(h);
+        if (b->budget > 0 && b->used + bytes > b->budget) {
+            b->err = "device memory budget exhausted";
+            return (int)MOCGPU_ERR_CAPACITY;
+        }
+        // EigenSolver::solve's start (mocgpu_eigen_solve): flat flux, the
+        // initial incoming flux, k = 1, step counter 0
+        int rc = mocgpu_initialize(h);
+        if (rc) {
+            b->err = h->err;
+            return rc;
+        }
+        std::vector<double> sc(S_COUNT, 0.0);
+        sc[S_K] = 1.0;
+        sc[S_KPREV] = 1.0;
+        sc[S_NFISS] = (double)h->n_fissile;
+        HIP_CHECK(hipMemcpyAsync(h->d_scal.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, b->stream));
+        const uint64_t zero = 0;
+        HIP_CHECK(hipMemcpyAsync(h->d_parity.p, &zero, 8, hipMemcpyHostToDevice, b->stream));
+        b->sync();
+        mocgpu_batch::Case c{};
+        c.h = h;
+        c.id = b->next_id++;
+        c.k_tol = k_tol;
+        c.psi_tol = psi_tol;
+        c.max_iter = max_iter;
+        c.min_iter = min_iter;
+        c.bytes = bytes;
+        b->cases.push_back(c);
+        b->used += bytes;
+        b->NP = h->NP;
+        b->dirty = true;
+        *id = c.id;
+        return 0;
+    });
+}
+
+int mocgpu_batch_run(mocgpu_batch *b, int32_t n_outer)
+{
+    if (!b) return MOCGPU_ERR_ARG;
+    return bguard(b, [&] {
+        b->run(n_outer);
+        return 0;
+    });
+}
+
+int mocgpu_batch_poll(mocgpu_batch *b, int32_t *ids, mocgpu_eigen_state *states, int32_t cap)
+{
+    if (!b) return MOCGPU_ERR_ARG;
+    return bguard(b, [&] {
+        b->sync();
+        if (!b->dirty) b->pull_status();
+        int n = 0;
+        for (auto &c : b->cases) {
+            if (c.reported || c.status[1] == 0 || n >= cap) continue;
+            double sc[S_COUNT];
+            HIP_CHECK(hipMemcpyAsync(sc, c.h->d_scal.p, sizeof(sc), hipMemcpyDeviceToHost, b->stream));
+            b->sync();
+            if (ids) ids[n] = c.id;
+            if (states) {
+                mocgpu_eigen_state s{};
+                s.k = sc[S_K];
+                s.k_prev = sc[S_KPREV];
+                s.err_k = sc[S_ERRK];
+                s.err_psi = sc[S_ERRPSI];
+                s.iterations = c.status[0];
+                s.converged = c.status[1] == 1 ? 1 : 0;
+                states[n] = s;
+            }
+            c.reported = true;
+            n++;
+        }
+        return n;
+    });
+}
+
+int mocgpu_batch_remove(mocgpu_batch *b, int32_t id)
+{
+    if (!b) return MOCGPU_ERR_ARG;
+    return bguard(b, [&] {
+        const int i = b->find(id);
+        if (i < 0) throw std::runtime_error("no such problem in the batch");
+        b->sync();
+        if (!b->dirty) b->pull_status();
+        b->used -= b->cases[i].bytes;
+        b->cases.erase(b->cases.begin() + i);
+        b->dirty = true;
+        return 0;
+    });
+}
+
+int32_t mocgpu_batch_size(const mocgpu_batch *b) { return b ? (int32_t)b->cases.size() : 0; }
+
+int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b) { return b ? b->used : 0; }
+
+}  // extern "C"

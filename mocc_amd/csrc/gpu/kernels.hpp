@@ -25,6 +25,7 @@ struct SweepArgs {
     int32_t n_slots;        // partial slots per group (psum is [G][n_slots])
     int32_t n_lslots;       // current slots per group (csum/ssum are [G][n_lslots])
     uint64_t parity;        // bit g: which boundary buffer of group g is incoming
+    const uint64_t *parity_dev;  // if set, the parity word is read from device memory instead
     const uint16_t *seg_loc;
     const float *seg_len;
     const int32_t *loc_fsr;
@@ -52,12 +53,39 @@ struct ReduceArgs {
     double *phi;
 };
 
+// One persistent launch over a work queue of (case, band, group) items of many
+// problems: workgroups take items from *counter until n_items is reached.
+// Items of cases whose status[2 * case + 1] is non-zero (finished) or whose
+// n_inner[case] <= inner are skipped.
+struct QueueArgs {
+    const SweepArgs *cases;
+    const int4 *items;        // (case, band, group, 0)
+    int32_t n_items;
+    int32_t inner;
+    unsigned int *counter;
+    const int32_t *status;    // [2 * case + 1]: non-zero once the case has finished
+    const int32_t *n_inner;   // [case]
+};
+
 void launch_sweep(const SweepArgs &a, int n_polar, bool with_current, int n_items, hipStream_t s);
+// Workgroups resident per CU for the queue kernel with lds bytes of LDS.
+int queue_blocks_per_cu(int n_polar, size_t lds);
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s);
 size_t sweep_lds_bytes(const SweepArgs &a);
 // Workgroups of a sweep launch: one per band (one group), or the XCD-aware
 // (band, group) grid of all groups.
 int sweep_grid_items(int n_bands, int G, bool all_groups);
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s);
+
+// Block -> (problem, first element) map of a launch over many problems; each
+// problem's elements are covered by whole 256-thread blocks.
+struct BlockMap {
+    int32_t c, first;
+};
+// Lanes per (FSR, group) item of the flux reduction.
+int reduce_flux_lanes();
+void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
+                              const int32_t *n_inner, int inner, hipStream_t s);
 
 }  // namespace gpu
 }  // namespace mocc_amd

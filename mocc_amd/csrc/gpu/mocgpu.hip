@@ -12,52 +12,11 @@
 #include <vector>
 
 #include "../../../include/mocgpu.h"
-#include "cmfd.hpp"
-#include "kernels.hpp"
-#include "plan.hpp"
+#include "handle.hpp"
 
 using namespace mocc_amd::gpu;
 
 namespace {
-
-constexpr double PI = 3.1415926535897932;
-constexpr double FPI = 4.0 * PI;
-constexpr int kRedBlocks = 256;  // fixed stage-1 grid: reductions are bitwise reproducible
-constexpr int kRedThreads = 256;
-
-struct HipError : std::runtime_error {
-    explicit HipError(const std::string &m) : std::runtime_error(m) {}
-};
-
-#define HIP_CHECK(expr)                                                                        \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
-
-template <typename T> struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    void alloc(size_t count)
-    {
-        free();
-        n = count;
-        if (count) HIP_CHECK(hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)));
-    }
-    void upload(const std::vector<T> &v, hipStream_t s)
-    {
-        alloc(v.size());
-        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void free()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DBuf() { free(); }
-};
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -103,17 +62,7 @@ __global__ void k_fission_source(int F, int G, const int32_t *fsr_mat, const dou
     fs[f] = s;
 }
 
-enum RedMode { RED_FISSION = 0, RED_NORM = 1, RED_PSIERR = 2, RED_RESID = 3 };
 
-struct RedArgs {
-    int F, G;
-    const int32_t *fsr_mat;
-    const double *mat_nf, *vol, *phi, *phi_old, *fs, *fs_prev;
-    double *scal;  // device scalars (see Scal)
-};
-
-// device scalar slots
-enum Scal { S_K = 0, S_KPREV, S_TF_NEW, S_TF_OLD, S_ERRK, S_ERRPSI, S_FA, S_FB, S_NFISS, S_RESID, S_RESERVED, S_COUNT };
 
 __device__ __forceinline__ void red_values(const RedArgs &a, int mode, int f, double v[4])
 {
@@ -273,235 +222,51 @@ __global__ void k_set_pin_flux(int npin, int G, int g, const int32_t *pin_off, c
     e2[ip] = e * e;
 }
 
-inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// handle
+// handle methods that launch kernels
 // ---------------------------------------------------------------------------
-struct mocgpu_handle {
-    std::string err;
-    mocgpu_options opt{};
-    SweepPlan plan;
-    int F = 0, G = 0, NP = 0, A = 0, nd = 0, nx = 0, ny = 0, npin = 0, n_surf = 0, n_mat = 0;
-    int n_fissile = 0;
-    int64_t n_seg = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t parity = 0;
-    bool has_fs = false;
-    bool has_radial = false;
-    int cmfd_iterations = 0;
+void mocgpu_handle::reduce(int mode, int update_k)
+{
+    RedArgs a = red_args();
+    hipLaunchKernelGGL(k_red_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, mode, d_partial.p);
+    hipLaunchKernelGGL(k_red_stage2, dim3(1), dim3(kRedThreads), 0, stream, a, mode, d_partial.p, kRedBlocks,
+                       update_k, d_red_out.p);
+}
 
-    // plan
-    DBuf<BandInfo> d_bands;
-    DBuf<int32_t> d_order, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
-    DBuf<uint16_t> d_seg_loc;
-    DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw, d_exptab;
-    DBuf<RayInfo> d_rays;
-    DBuf<AziInfo> d_azi;
-    DBuf<uint32_t> d_ev;
-    DBuf<double> d_surf_area;
-    // cross sections and mesh
-    DBuf<int32_t> d_fsr_mat, d_pin_off, d_pin_cell;
-    DBuf<double> d_mat_tr, d_mat_ss, d_mat_nf, d_mat_ch, d_mat_sc, d_mat_f, d_vol, d_area, d_pin_area;
-    // state
-    DBuf<double> d_phi, d_phi_old, d_src, d_fs, d_fs_prev, d_psum, d_csum, d_ssum, d_cur, d_sfl;
-    DBuf<float> d_psi;
-    DBuf<double> d_partial, d_scal, d_red_out, d_pin, d_e2;
-    std::unique_ptr<CmfdSolver> cmfd;
+void mocgpu_handle::build_source(int g_fixed, bool jacobi)
+{
+    size_t n = g_fixed >= 0 ? (size_t)F : (size_t)F * G;
+    hipLaunchKernelGGL(k_build_source, dim3(nblk(n)), dim3(256), 0, stream, F, G, g_fixed, has_fs ? 1 : 0,
+                       d_fsr_mat.p, d_mat_ch.p, d_mat_sc.p, d_fs.p, jacobi ? d_phi_old.p : d_phi.p,
+                       d_src.p);
+}
 
-    // host copies
-    std::vector<float> bc_template;  // one group of incoming values after initialize
-    std::vector<int32_t> h_fsr_mat;
-
-    ~mocgpu_handle()
-    {
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
+void mocgpu_handle::sweep_once(int g, bool with_current)
+{
+    SweepArgs a = sweep_args(g);
+    int items = sweep_grid_items((int)plan->bands.size(), G, g < 0);
+    launch_sweep(a, NP, with_current, items, stream);
+    launch_reduce_flux(reduce_args(g), stream);
+    if (with_current) {
+        int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
+        size_t n = (size_t)(n1 - n0) * (g >= 0 ? 1 : G);
+        int nls = (int)std::max<size_t>(1, plan->lsurf_glob.size());
+        hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, nls,
+                           d_surf_ptr.p, d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
+        has_radial = true;
     }
+    if (g >= 0) parity ^= (1ull << g);
+    else parity ^= (G >= 64 ? ~0ull : ((1ull << G) - 1ull));
+}
 
-    SweepArgs sweep_args(int g_fixed) const
-    {
-        SweepArgs a{};
-        a.bands = d_bands.p;
-        a.order = d_order.p;
-        a.n_bands = (int)plan.bands.size();
-        a.g_fixed = g_fixed;
-        a.G = G;
-        a.F = F;
-        a.bcpg = plan.bcpg;
-        a.lds_local = plan.max_local + 1;
-        a.lds_lsurf = std::max(1, plan.max_lsurf);
-        a.n_slots = (int)plan.loc_fsr.size();
-        a.n_lslots = (int)std::max<size_t>(1, plan.lsurf_glob.size());
-        a.parity = parity;
-        a.seg_loc = d_seg_loc.p;
-        a.seg_len = d_seg_len.p;
-        a.loc_fsr = d_loc_fsr.p;
-        a.rays = d_rays.p;
-        a.azi = d_azi.p;
-        a.ang_rsin = d_ang_rsin.p;
-        a.ang_wt = d_ang_wt.p;
-        a.ang_cw = d_ang_cw.p;
-        a.ang_fw = d_ang_fw.p;
-        a.exptab = (const float2 *)d_exptab.p;
-        a.fsr_mat = d_fsr_mat.p;
-        a.mat_tr = d_mat_tr.p;
-        a.mat_ss = d_mat_ss.p;
-        a.src = d_src.p;
-        a.phi = d_phi.p;
-        a.psi = d_psi.p;
-        a.psum = d_psum.p;
-        a.ev_off = d_ev_off.p;
-        a.ev = d_ev.p;
-        a.csum = d_csum.p;
-        a.ssum = d_ssum.p;
-        return a;
-    }
-
-    ReduceArgs reduce_args(int g_fixed) const
-    {
-        ReduceArgs r{};
-        r.F = F;
-        r.G = G;
-        r.g_fixed = g_fixed;
-        r.n_slots = (int)plan.loc_fsr.size();
-        r.slot_ptr = d_slot_ptr.p;
-        r.slot_idx = d_slot_idx.p;
-        r.psum = d_psum.p;
-        r.fsr_mat = d_fsr_mat.p;
-        r.mat_tr = d_mat_tr.p;
-        r.mat_ss = d_mat_ss.p;
-        r.vol = d_vol.p;
-        r.src = d_src.p;
-        r.phi = d_phi.p;
-        return r;
-    }
-
-    RedArgs red_args() const
-    {
-        RedArgs a{};
-        a.F = F;
-        a.G = G;
-        a.fsr_mat = d_fsr_mat.p;
-        a.mat_nf = d_mat_nf.p;
-        a.vol = d_vol.p;
-        a.phi = d_phi.p;
-        a.phi_old = d_phi_old.p;
-        a.fs = d_fs.p;
-        a.fs_prev = d_fs_prev.p;
-        a.scal = d_scal.p;
-        return a;
-    }
-
-    void reduce(int mode, int update_k)
-    {
-        RedArgs a = red_args();
-        hipLaunchKernelGGL(k_red_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, stream, a, mode, d_partial.p);
-        hipLaunchKernelGGL(k_red_stage2, dim3(1), dim3(kRedThreads), 0, stream, a, mode, d_partial.p, kRedBlocks,
-                           update_k, d_red_out.p);
-    }
-
-    void build_source(int g_fixed, bool jacobi)
-    {
-        size_t n = g_fixed >= 0 ? (size_t)F : (size_t)F * G;
-        hipLaunchKernelGGL(k_build_source, dim3(nblk(n)), dim3(256), 0, stream, F, G, g_fixed, has_fs ? 1 : 0,
-                           d_fsr_mat.p, d_mat_ch.p, d_mat_sc.p, d_fs.p, jacobi ? d_phi_old.p : d_phi.p,
-                           d_src.p);
-    }
-
-    // one inner iteration of sweep1g for group g (g < 0: all groups)
-    void sweep_once(int g, bool with_current)
-    {
-        SweepArgs a = sweep_args(g);
-        int items = sweep_grid_items((int)plan.bands.size(), G, g < 0);
-        launch_sweep(a, NP, with_current, items, stream);
-        launch_reduce_flux(reduce_args(g), stream);
-        if (with_current) {
-            int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
-            size_t n = (size_t)(n1 - n0) * (g >= 0 ? 1 : G);
-            int nls = (int)std::max<size_t>(1, plan.lsurf_glob.size());
-            hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, nls,
-                               d_surf_ptr.p, d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
-            has_radial = true;
-        }
-        if (g >= 0) parity ^= (1ull << g);
-        else parity ^= (G >= 64 ? ~0ull : ((1ull << G) - 1ull));
-    }
-
-    // MoCSweeper::sweep (moc_sweeper.cpp:189-225)
-    void mocsweep(int g, bool with_current)
-    {
-        for (int inner = 0; inner < opt.n_inner; inner++)
-            sweep_once(g, with_current && inner == opt.n_inner - 1);
-    }
-
-    void copy_flux_to_old()
-    {
-        size_t n = (size_t)F * G;
-        hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, stream, d_phi.p, d_phi_old.p, n);
-    }
-
-    // FixedSourceSolver::step (fixed_source_solver.cpp:102-117)
-    void fss_step(bool with_current)
-    {
-        copy_flux_to_old();
-        if (opt.energy_mode == 1) {
-            build_source(-1, true);
-            mocsweep(-1, with_current);
-        } else {
-            for (int g = 0; g < G; g++) {
-                build_source(g, false);
-                mocsweep(g, with_current);
-            }
-        }
-    }
-
-    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
-
-    CmfdView cmfd_view()
-    {
-        CmfdView v{};
-        v.F = F;
-        v.G = G;
-        v.npin = npin;
-        v.n_mat = n_mat;
-        v.fsr_mat = d_fsr_mat.p;
-        v.pin_off = d_pin_off.p;
-        v.pin_cell = d_pin_cell.p;
-        v.area = d_area.p;
-        v.vol = d_vol.p;
-        v.pin_area = d_pin_area.p;
-        v.mat_tr = d_mat_tr.p;
-        v.mat_nf = d_mat_nf.p;
-        v.mat_f = d_mat_f.p;
-        v.mat_ch = d_mat_ch.p;
-        v.mat_sc = d_mat_sc.p;
-        v.phi = d_phi.p;
-        v.cur = d_cur.p;
-        v.sfl = d_sfl.p;
-        v.k = d_scal.p + S_K;
-        v.has_radial = has_radial ? 1 : 0;
-        return v;
-    }
-
-    // EigenSolver::do_cmfd with k on the device; raises on a non-positive
-    // coarse flux (set_pin_flux_1g throws there in the reference)
-    int cmfd_step_sync(int *iterations)
-    {
-        cmfd->enqueue_step(cmfd_view(), stream);
-        int bad = 0;
-        cmfd->read_info(stream, iterations, &bad);
-        if (bad) {
-            err = "Negative or zero input flux in the CMFD rescale";
-            return MOCGPU_ERR_NONPOS_FLUX;
-        }
-        return 0;
-    }
-};
+void mocgpu_handle::copy_flux_to_old()
+{
+    size_t n = (size_t)F * G;
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, stream, d_phi.p, d_phi_old.p, n);
+}
 
 namespace {
 template <typename F> int guard(mocgpu_handle *h, F &&f)
@@ -540,7 +305,14 @@ void mocgpu_default_options(mocgpu_options *o)
 
 const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
 
-int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
+}  // extern "C"
+
+namespace {
+// mocgpu_create / mocgpu_create_like: with a template handle, the geometry
+// (plan, segment streams, exp table, FSR and pin arrays) is shared with it
+// and only the cross sections and the state are new.
+int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu_handle *like,
+                mocgpu_handle **out)
 {
     if (!p || !out) {
         g_create_err = "mocgpu_create: null problem or output handle";
@@ -554,8 +326,19 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
         if (opt) h->opt = *opt;
         else mocgpu_default_options(&h->opt);
         if (p->n_group > 64) throw std::runtime_error("at most 64 energy groups are supported");
-        h->plan = build_plan(*p, h->opt.band_rays);
-        const SweepPlan &pl = h->plan;
+        if (like) {
+            bool same = p->n_fsr == like->F && p->n_seg == like->n_seg && p->n_polar == like->NP &&
+                        p->n_azi == like->A && p->nx == like->nx && p->ny == like->ny &&
+                        p->n_ray == (int32_t)like->plan->rays.size();
+            for (int i = 0; same && i < 6; i++) same = p->bc[i] == like->bc[i];
+            for (int f = 0; same && f < p->n_fsr; f++) same = p->fsr_mat[f] == like->h_fsr_mat[f];
+            if (!same) throw std::runtime_error("mocgpu_create_like: the problem's geometry differs from the template's");
+            h->plan = like->plan;
+        } else {
+            h->plan = std::make_shared<SweepPlan>(build_plan(*p, h->opt.band_rays));
+        }
+        const SweepPlan &pl = *h->plan;
+        for (int i = 0; i < 6; i++) h->bc[i] = p->bc[i];
         h->F = p->n_fsr;
         h->G = p->n_group;
         h->NP = p->n_polar;
@@ -571,6 +354,36 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
         HIP_CHECK(hipEventCreate(&h->ev0));
         HIP_CHECK(hipEventCreate(&h->ev1));
         hipStream_t s = h->stream;
+        const int F = h->F, G = h->G;
+        if (like) {
+            h->d_bands = like->d_bands;
+            h->d_order = like->d_order;
+            h->d_seg_loc = like->d_seg_loc;
+            h->d_seg_len = like->d_seg_len;
+            h->d_loc_fsr = like->d_loc_fsr;
+            h->d_rays = like->d_rays;
+            h->d_azi = like->d_azi;
+            h->d_ang_rsin = like->d_ang_rsin;
+            h->d_ang_wt = like->d_ang_wt;
+            h->d_ang_cw = like->d_ang_cw;
+            h->d_ang_fw = like->d_ang_fw;
+            h->d_slot_ptr = like->d_slot_ptr;
+            h->d_slot_idx = like->d_slot_idx;
+            h->d_ev_off = like->d_ev_off;
+            h->d_ev = like->d_ev;
+            h->d_surf_ptr = like->d_surf_ptr;
+            h->d_surf_idx = like->d_surf_idx;
+            h->d_surf_area = like->d_surf_area;
+            h->d_exptab = like->d_exptab;
+            h->h_fsr_mat = like->h_fsr_mat;
+            h->d_fsr_mat = like->d_fsr_mat;
+            h->d_vol = like->d_vol;
+            h->d_area = like->d_area;
+            h->d_pin_off = like->d_pin_off;
+            h->d_pin_cell = like->d_pin_cell;
+            h->d_pin_area = like->d_pin_area;
+            h->bc_template = like->bc_template;
+        } else {
         h->d_bands.upload(pl.bands, s);
         h->d_order.upload(pl.band_order, s);
         h->d_seg_loc.upload(pl.seg_loc, s);
@@ -592,10 +405,19 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
         std::vector<float> tab;
         exp_table_fill(tab);
         h->d_exptab.upload(tab, s);
-        const int F = h->F, G = h->G;
-        std::vector<int32_t> fm(p->fsr_mat, p->fsr_mat + F);
-        h->h_fsr_mat = fm;
-        h->d_fsr_mat.upload(fm, s);
+        h->h_fsr_mat.assign(p->fsr_mat, p->fsr_mat + F);
+        h->d_fsr_mat.upload(h->h_fsr_mat, s);
+        h->d_vol.upload(std::vector<double>(p->fsr_vol, p->fsr_vol + F), s);
+        h->d_area.upload(std::vector<double>(p->fsr_area, p->fsr_area + F), s);
+        h->d_pin_off.upload(std::vector<int32_t>(p->pin_fsr_off, p->pin_fsr_off + h->npin + 1), s);
+        h->d_pin_cell.upload(std::vector<int32_t>(p->pin_cell, p->pin_cell + h->npin), s);
+        h->d_pin_area.upload(std::vector<double>(p->pin_area, p->pin_area + h->npin), s);
+        h->sync();
+        // the segment streams live on the device from here on
+        std::vector<uint16_t>().swap(h->plan->seg_loc);
+        std::vector<float>().swap(h->plan->seg_len);
+        }
+        const std::vector<int32_t> &fm = h->h_fsr_mat;
         auto vecd = [](const double *x, size_t n) { return std::vector<double>(x, x + n); };
         h->d_mat_tr.upload(vecd(p->xs_tr, (size_t)p->n_mat * G), s);
         h->d_mat_nf.upload(vecd(p->xs_nf, (size_t)p->n_mat * G), s);
@@ -606,11 +428,6 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
         for (int m = 0; m < p->n_mat; m++)
             for (int g = 0; g < G; g++) ss[m * G + g] = p->xs_sc[((size_t)m * G + g) * G + g];
         h->d_mat_ss.upload(ss, s);
-        h->d_vol.upload(vecd(p->fsr_vol, F), s);
-        h->d_area.upload(vecd(p->fsr_area, F), s);
-        h->d_pin_off.upload(std::vector<int32_t>(p->pin_fsr_off, p->pin_fsr_off + h->npin + 1), s);
-        h->d_pin_cell.upload(std::vector<int32_t>(p->pin_cell, p->pin_cell + h->npin), s);
-        h->d_pin_area.upload(vecd(p->pin_area, h->npin), s);
         // state
         h->d_phi.alloc((size_t)F * G);
         h->d_phi_old.alloc((size_t)F * G);
@@ -651,6 +468,7 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
         h->d_scal.upload(sc, s);
         // template of the initial incoming flux: 1/(4 pi) on non-vacuum faces
         // (MoCSweeper::initialize + BoundaryCondition::initialize_scalar)
+        if (!like) {
         h->bc_template.assign(pl.bcpg, 0.0f);
         const float val = (float)(1.0 / FPI);
         for (int iang = 0; iang < 4 * pl.nd; iang++) {
@@ -666,6 +484,9 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
             if (p->bc[sy] == MOCGPU_BC_REFLECT)
                 for (int i = 0; i < nx_; i++) h->bc_template[pl.ang_off[iang] + ny_ + i] = val;
         }
+        }
+        h->d_parity.alloc(1);
+        h->d_status.alloc(2);
         h->cmfd = std::make_unique<CmfdSolver>();
         h->cmfd->setup(*p, h->opt, s);
         h->sync();
@@ -677,6 +498,49 @@ int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_han
     }
     *out = h.release();
     return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
+{
+    return create_impl(p, opt, nullptr, out);
+}
+
+int mocgpu_create_like(const mocgpu_handle *like, const mocgpu_problem *p, const mocgpu_options *opt,
+                       mocgpu_handle **out)
+{
+    if (!like) {
+        g_create_err = "mocgpu_create_like: null template handle";
+        return MOCGPU_ERR_ARG;
+    }
+    return create_impl(p, opt, like, out);
+}
+
+int64_t mocgpu_state_bytes(const mocgpu_handle *h)
+{
+    if (!h) return 0;
+    const size_t b = h->d_mat_tr.bytes() + h->d_mat_ss.bytes() + h->d_mat_nf.bytes() + h->d_mat_ch.bytes() +
+                     h->d_mat_sc.bytes() + h->d_mat_f.bytes() + h->d_phi.bytes() + h->d_phi_old.bytes() +
+                     h->d_src.bytes() + h->d_fs.bytes() + h->d_fs_prev.bytes() + h->d_psum.bytes() +
+                     h->d_csum.bytes() + h->d_ssum.bytes() + h->d_cur.bytes() + h->d_sfl.bytes() +
+                     h->d_psi.bytes() + h->d_partial.bytes() + h->d_scal.bytes() + h->d_red_out.bytes() +
+                     h->d_pin.bytes() + h->d_e2.bytes() + h->d_parity.bytes() + h->d_status.bytes();
+    return (int64_t)b;
+}
+
+int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
+{
+    if (!h) return 0;
+    const size_t b = h->d_bands.bytes() + h->d_order.bytes() + h->d_seg_loc.bytes() + h->d_seg_len.bytes() +
+                     h->d_loc_fsr.bytes() + h->d_rays.bytes() + h->d_azi.bytes() + h->d_ang_rsin.bytes() +
+                     h->d_ang_wt.bytes() + h->d_ang_cw.bytes() + h->d_ang_fw.bytes() + h->d_slot_ptr.bytes() +
+                     h->d_slot_idx.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +
+                     h->d_surf_idx.bytes() + h->d_surf_area.bytes() + h->d_exptab.bytes() + h->d_fsr_mat.bytes() +
+                     h->d_vol.bytes() + h->d_area.bytes() + h->d_pin_off.bytes() + h->d_pin_cell.bytes() +
+                     h->d_pin_area.bytes();
+    return (int64_t)b;
 }
 
 int mocgpu_destroy(mocgpu_handle *h)
@@ -693,7 +557,7 @@ int mocgpu_initialize(mocgpu_handle *h)
         HIP_CHECK(hipMemcpyAsync(h->d_phi_old.p, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, h->stream));
         for (int g = 0; g < h->G; g++)
             for (int b = 0; b < 2; b++)
-                HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + b) * h->plan.bcpg, h->bc_template.data(),
+                HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + b) * h->plan->bcpg, h->bc_template.data(),
                                          h->bc_template.size() * 4, hipMemcpyHostToDevice, h->stream));
         h->parity = 0;
         h->cmfd->initialize(h->stream);
@@ -720,19 +584,19 @@ int mocgpu_get_flux(mocgpu_handle *h, double *phi)
     });
 }
 
-int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return (int64_t)h->G * h->plan.bcpg; }
+int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return (int64_t)h->G * h->plan->bcpg; }
 
 int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
 {
     return guard(h, [&] {
-        if (n != (int64_t)h->G * h->plan.bcpg) throw std::runtime_error("boundary size mismatch");
-        std::vector<float> tmp(h->plan.bcpg);
+        if (n != (int64_t)h->G * h->plan->bcpg) throw std::runtime_error("boundary size mismatch");
+        std::vector<float> tmp(h->plan->bcpg);
         for (int g = 0; g < h->G; g++) {
             int par = (int)((h->parity >> g) & 1ull);
-            HIP_CHECK(hipMemcpyAsync(tmp.data(), h->d_psi.p + ((size_t)g * 2 + par) * h->plan.bcpg, tmp.size() * 4,
+            HIP_CHECK(hipMemcpyAsync(tmp.data(), h->d_psi.p + ((size_t)g * 2 + par) * h->plan->bcpg, tmp.size() * 4,
                                      hipMemcpyDeviceToHost, h->stream));
             h->sync();
-            for (int i = 0; i < h->plan.bcpg; i++) bc[(size_t)g * h->plan.bcpg + i] = tmp[i];
+            for (int i = 0; i < h->plan->bcpg; i++) bc[(size_t)g * h->plan->bcpg + i] = tmp[i];
         }
         return 0;
     });
@@ -741,12 +605,12 @@ int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
 int mocgpu_set_boundary(mocgpu_handle *h, const double *bc, int64_t n)
 {
     return guard(h, [&] {
-        if (n != (int64_t)h->G * h->plan.bcpg) throw std::runtime_error("boundary size mismatch");
-        std::vector<float> tmp(h->plan.bcpg);
+        if (n != (int64_t)h->G * h->plan->bcpg) throw std::runtime_error("boundary size mismatch");
+        std::vector<float> tmp(h->plan->bcpg);
         for (int g = 0; g < h->G; g++) {
-            for (int i = 0; i < h->plan.bcpg; i++) tmp[i] = (float)bc[(size_t)g * h->plan.bcpg + i];
+            for (int i = 0; i < h->plan->bcpg; i++) tmp[i] = (float)bc[(size_t)g * h->plan->bcpg + i];
             int par = (int)((h->parity >> g) & 1ull);
-            HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + par) * h->plan.bcpg, tmp.data(), tmp.size() * 4,
+            HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + par) * h->plan->bcpg, tmp.data(), tmp.size() * 4,
                                      hipMemcpyHostToDevice, h->stream));
             h->sync();
         }
@@ -940,7 +804,7 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double
 {
     return guard(h, [&] {
         SweepArgs a = h->sweep_args(-1);
-        int items = sweep_grid_items((int)h->plan.bands.size(), h->G, true);
+        int items = sweep_grid_items((int)h->plan->bands.size(), h->G, true);
         // warm-up
         launch_sweep(a, h->NP, false, items, h->stream);
         launch_reduce_flux(h->reduce_args(-1), h->stream);

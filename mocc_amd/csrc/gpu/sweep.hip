@@ -58,30 +58,18 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, int k)
     return ((unsigned long long)uhi << 32) | ulo;
 }
 
+// One (band, group) item: the whole workgroup sweeps the band's rays in both
+// directions for group g. Ends after the band's partial tallies are stored;
+// the caller synchronizes before reusing smem.
 template <int NP, bool CUR>
-__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
+__device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, unsigned long long *smem,
+                                           unsigned int &s_bound)
 {
-    extern __shared__ unsigned long long smem[];
     unsigned long long *tal = smem;
     float2 *sq = (float2 *)(smem + A.lds_local);  // (Sigma_t, q) per band-local FSR
     unsigned long long *tcur = smem + 2 * A.lds_local;
     unsigned long long *tsfl = tcur + A.lds_lsurf;
-    __shared__ unsigned int s_bound;
 
-    int g, band;
-    if (A.g_fixed >= 0) {
-        g = A.g_fixed;
-        band = A.order[blockIdx.x];
-    } else {
-        // XCD-aware order: the G workgroups of a band are consecutive
-        // workgroups of one XCD, so they share its L2 copy of the band.
-        const int x = blockIdx.x % kNumXcd;
-        const int s = blockIdx.x / kNumXcd;
-        const int rank = (s / A.G) * kNumXcd + x;
-        if (rank >= A.n_bands) return;
-        g = s % A.G;
-        band = A.order[rank];
-    }
     const BandInfo B = A.bands[band];
     const int tid = threadIdx.x;
     const int G = A.G;
@@ -122,7 +110,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     const int out_x = dir ? az.out_off[1][0] : az.out_off[0][0];
     const int out_y = dir ? az.out_off[1][1] : az.out_off[0][1];
     const size_t gbase = (size_t)g * 2 * A.bcpg;
-    const int par = (int)((A.parity >> g) & 1ull);
+    const uint64_t parity = A.parity_dev ? *A.parity_dev : A.parity;
+    const int par = (int)((parity >> g) & 1ull);
     const float *__restrict__ psi_in = A.psi + gbase + (size_t)par * A.bcpg;
     float *__restrict__ psi_out = A.psi + gbase + (size_t)(1 - par) * A.bcpg;
 
@@ -296,6 +285,53 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
     }
 }
 
+template <int NP, bool CUR>
+__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
+{
+    extern __shared__ unsigned long long smem[];
+    __shared__ unsigned int s_bound;
+    int g, band;
+    if (A.g_fixed >= 0) {
+        g = A.g_fixed;
+        band = A.order[blockIdx.x];
+    } else {
+        // XCD-aware order: the G workgroups of a band are consecutive
+        // workgroups of one XCD, so they share its L2 copy of the band.
+        const int x = blockIdx.x % kNumXcd;
+        const int s = blockIdx.x / kNumXcd;
+        const int rank = (s / A.G) * kNumXcd + x;
+        if (rank >= A.n_bands) return;
+        g = s % A.G;
+        band = A.order[rank];
+    }
+    sweep_band<NP, CUR>(A, band, g, smem, s_bound);
+}
+
+// Persistent work-queue sweep over many problems. Thread 0 takes the next
+// item index with a (vector) global atomic and shares it through LDS; every
+// workgroup leaves once the counter passes n_items, so the grid always drains.
+template <int NP>
+__global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArgs *__restrict__ cases)
+{
+    extern __shared__ unsigned long long smem[];
+    __shared__ unsigned int s_bound;
+    __shared__ int s_item;
+    while (true) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(Q.counter, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= Q.n_items) break;
+        const int4 w = Q.items[it];
+        const int c = __builtin_amdgcn_readfirstlane(w.x);
+        if (Q.status[2 * c + 1] != 0 || Q.n_inner[c] <= Q.inner) continue;
+        const SweepArgs A = cases[c];
+        sweep_band<NP, false>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z), smem,
+                              s_bound);
+        __syncthreads();
+    }
+}
+
 // phi = T / (St V) + 4 pi q (moc_sweeper_kernel.inc.hpp:165-173). T is the
 // sum of the FSR's band slots: kRedLanes lanes per (FSR, group) take the
 // slots s = first + lane, first + lane + kRedLanes, ... and an xor butterfly
@@ -303,10 +339,8 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
 // the same T and the order is fixed.
 constexpr int kRedLanes = 8;
 
-__global__ void k_reduce_flux(ReduceArgs A)
+__device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, int lane)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int item = gid / kRedLanes, lane = gid % kRedLanes;
     int f, g;
     if (A.g_fixed >= 0) {
         f = item;
@@ -330,6 +364,26 @@ __global__ void k_reduce_flux(ReduceArgs A)
     const double tr = A.mat_tr[m * G + g];
     const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) * (1.0 / (tr * FPI));
     A.phi[(size_t)f * G + g] = T / (tr * A.vol[f]) + q * FPI;
+}
+
+__global__ void k_reduce_flux(ReduceArgs A)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    reduce_flux_item(A, gid / kRedLanes, gid % kRedLanes);
+}
+
+// The same over many problems: block b covers elements [first, first + 256)
+// of problem bm[b].c (kRedLanes elements per (FSR, group) item).
+__global__ void k_reduce_flux_batch(const ReduceArgs *__restrict__ cases, const BlockMap *__restrict__ bm,
+                                    const int32_t *__restrict__ status, const int32_t *__restrict__ n_inner,
+                                    int inner)
+{
+    const BlockMap b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    if (status[2 * c + 1] != 0 || n_inner[c] <= inner) return;
+    const ReduceArgs A = cases[c];
+    const int gid = __builtin_amdgcn_readfirstlane(b.first) + threadIdx.x;
+    reduce_flux_item(A, gid / kRedLanes, gid % kRedLanes);
 }
 
 }  // namespace
@@ -359,12 +413,45 @@ void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStr
 #undef MOCGPU_SWEEP_CASE
 }
 
+int queue_blocks_per_cu(int n_polar, size_t lds)
+{
+    int nb = 0;
+    hipError_t e = hipSuccess;
+    switch (n_polar) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<1>, kWG, lds); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<2>, kWG, lds); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<3>, kWG, lds); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<4>, kWG, lds); break;
+    }
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s)
+{
+    dim3 grid(n_blocks), block(kWG);
+    switch (n_polar) {
+    case 1: hipLaunchKernelGGL(k_sweep_queue<1>, grid, block, lds, s, q, q.cases); break;
+    case 2: hipLaunchKernelGGL(k_sweep_queue<2>, grid, block, lds, s, q, q.cases); break;
+    case 3: hipLaunchKernelGGL(k_sweep_queue<3>, grid, block, lds, s, q, q.cases); break;
+    default: hipLaunchKernelGGL(k_sweep_queue<4>, grid, block, lds, s, q, q.cases); break;
+    }
+}
+
 int sweep_grid_items(int n_bands, int G, bool all_groups)
 {
     if (!all_groups) return n_bands;
     const int per_xcd = (n_bands + kNumXcd - 1) / kNumXcd;
     return per_xcd * G * kNumXcd;
 }
+
+void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
+                              const int32_t *n_inner, int inner, hipStream_t s)
+{
+    if (n_blocks > 0)
+        hipLaunchKernelGGL(k_reduce_flux_batch, dim3(n_blocks), dim3(256), 0, s, cases, bm, status, n_inner, inner);
+}
+
+int reduce_flux_lanes() { return kRedLanes; }
 
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s)
 {

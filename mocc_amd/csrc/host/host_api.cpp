@@ -188,6 +188,16 @@ int mocc_host_trace_ray(const mocc_host_problem *h, double x1, double y1, double
     });
 }
 
+int mocc_host_reload_xs(mocc_host_problem *h, const char *xsl_path)
+{
+    if (!h || !xsl_path) return MOCGPU_ERR_ARG;
+    return guard([&] {
+        if (!h->prob) throw Error("a geometry-only deck has no cross sections to replace");
+        h->prob->reload_xs(xsl_path);
+        return 0;
+    });
+}
+
 int mocc_host_fsr_areas(const mocc_host_problem *h, double *out, int32_t cap)
 {
     if (!h) return MOCGPU_ERR_ARG;

@@ -421,11 +421,46 @@ void AngularQuadrature::update_weights()
 // ---------------------------------------------------------------------------
 // materials: core/material_lib.cpp:45-149, core/material.cpp:22-62
 // ---------------------------------------------------------------------------
+namespace {
+void parse_xsl(MaterialLib &L, const std::string &path);
+}
+
 void MaterialLib::load(const XmlNode &in, const std::string &base_dir)
 {
     std::string path = in.attr_str("path");
     if (path.empty()) throw Error("No material library specified.");
     if (path[0] != '/' && !base_dir.empty()) path = base_dir + "/" + path;
+    parse_xsl(*this, path);
+    for (const XmlNode *mn : in.children_named("material")) {
+        int id = mn->attr_int("id", 0);
+        std::string nm = mn->attr_str("name");
+        auto it = by_name.find(nm);
+        if (it == by_name.end()) throw Error("Material not found in library: " + nm);
+        id_to_lib[id] = it->second;
+        id_to_name[id] = nm;
+    }
+}
+
+void MaterialLib::reload(const std::string &path)
+{
+    MaterialLib fresh;
+    parse_xsl(fresh, path);
+    for (const auto &kv : id_to_name) {
+        auto it = fresh.by_name.find(kv.second);
+        if (it == fresh.by_name.end()) throw Error("Material not found in library: " + kv.second);
+        fresh.id_to_lib[kv.first] = it->second;
+    }
+    fresh.id_to_name = id_to_name;
+    *this = std::move(fresh);
+}
+
+namespace {
+void parse_xsl(MaterialLib &L, const std::string &path)
+{
+    int &n_group = L.n_group;
+    VecF &g_bounds = L.g_bounds;
+    auto &by_name = L.by_name;
+    auto &lib = L.lib;
     std::ifstream f(path);
     if (!f) throw Error("Failed to open the cross-section library at: " + path);
     // FileScrubber: strip '!' comments and blank lines
@@ -501,14 +536,8 @@ void MaterialLib::load(const XmlNode &in, const std::string &base_dir)
         by_name[mat.name] = im;
         lib.push_back(mat);
     }
-    for (const XmlNode *mn : in.children_named("material")) {
-        int id = mn->attr_int("id", 0);
-        std::string nm = mn->attr_str("name");
-        auto it = by_name.find(nm);
-        if (it == by_name.end()) throw Error("Material not found in library: " + nm);
-        id_to_lib[id] = it->second;
-    }
 }
+}  // namespace
 
 const Material &MaterialLib::by_id(int id) const
 {

@@ -137,7 +137,11 @@ struct MaterialLib {
     std::vector<Material> lib;
     std::map<std::string, int> by_name;
     std::map<int, int> id_to_lib;  // user material id -> lib index
+    std::map<int, std::string> id_to_name;
     void load(const XmlNode &material_lib, const std::string &base_dir);
+    // Re-read the library from another XSL file with the same material names
+    // (cross-section branch cases); the deck's id -> name binding is kept.
+    void reload(const std::string &path);
     const Material &by_id(int id) const;
     bool has(int id) const { return id_to_lib.count(id) > 0; }
 };

@@ -107,8 +107,6 @@ void Problem::fill_view()
     int nd = q.ndir_oct();
     int A = (int)rd.rays.size();
     int P = 2 * nd / A;
-    const MaterialLib &lib = m.mat_lib;
-    int G = lib.n_group;
 
     for (int i = 0; i < 2 * nd; i++) {
         const Angle &a = q[i];
@@ -150,33 +148,7 @@ void Problem::fill_view()
         azi_ray_off_.push_back((int32_t)(ray_seg_off_.size() - 1));
     }
 
-    // XS mesh (xs_mesh.cpp:27-118): regions ordered by material ID.
-    std::map<int, std::vector<int>> fsr_map;
-    {
-        int ireg = 0;
-        for (const Pin *pin : m.core_pins) {
-            int ixs = 0;
-            for (int mid : pin->mat_ids) {
-                for (int r = 0; r < pin->mesh->n_fsrs(ixs); r++) fsr_map[mid].push_back(ireg++);
-                ixs++;
-            }
-        }
-    }
-    fsr_mat_.assign(m.n_reg, -1);
-    int imat = 0;
-    for (const auto &kv : fsr_map) {
-        const Material &mat = lib.by_id(kv.first);
-        for (int g = 0; g < G; g++) {
-            xs_tr_.push_back(mat.xstr[g]);
-            xs_nf_.push_back(mat.xsnf[g]);
-            xs_f_.push_back(mat.xsf[g]);
-            xs_ch_.push_back(mat.xsch[g]);
-        }
-        for (int to = 0; to < G; to++)
-            for (int from = 0; from < G; from++) xs_sc_.push_back(mat.scat[to][from]);
-        for (int f : kv.second) fsr_mat_[f] = imat;
-        imat++;
-    }
+    fill_xs();
     VecF areas = m.areas();
     fsr_area_ = areas;
     for (double a : areas) fsr_vol_.push_back(a * m.hz);
@@ -193,11 +165,9 @@ void Problem::fill_view()
 
     mocgpu_problem &v = view_;
     v.n_fsr = m.n_reg;
-    v.n_group = G;
     v.n_polar = P;
     v.n_azi = A;
     v.n_ray = (int32_t)(ray_seg_off_.size() - 1);
-    v.n_mat = imat;
     v.n_seg = (int64_t)seg_fsr_.size();
     v.ang_weight = ang_weight_.data();
     v.ang_sin_theta = ang_sin_.data();
@@ -219,12 +189,6 @@ void Problem::fill_view()
     v.cm_data = cm_data_.data();
     v.fsr_vol = fsr_vol_.data();
     v.fsr_area = fsr_area_.data();
-    v.fsr_mat = fsr_mat_.data();
-    v.xs_tr = xs_tr_.data();
-    v.xs_nf = xs_nf_.data();
-    v.xs_f = xs_f_.data();
-    v.xs_ch = xs_ch_.data();
-    v.xs_sc = xs_sc_.data();
     v.nx = m.nx;
     v.ny = m.ny;
     v.x_vec = x_vec_.data();
@@ -235,6 +199,61 @@ void Problem::fill_view()
     v.pin_area = pin_area_.data();
     for (int i = 0; i < 6; i++) v.bc[i] = (int32_t)m.bc[i];
     v.n_fuel_pins = m.n_fuel;
+}
+
+// XS mesh (xs_mesh.cpp:27-118): one XS region per material, ordered by
+// material ID; fsr_mat maps every FSR to its region.
+void Problem::fill_xs()
+{
+    const CoreMesh &m = *mesh_;
+    const MaterialLib &lib = m.mat_lib;
+    const int G = lib.n_group;
+    std::map<int, std::vector<int>> fsr_map;
+    {
+        int ireg = 0;
+        for (const Pin *pin : m.core_pins) {
+            int ixs = 0;
+            for (int mid : pin->mat_ids) {
+                for (int r = 0; r < pin->mesh->n_fsrs(ixs); r++) fsr_map[mid].push_back(ireg++);
+                ixs++;
+            }
+        }
+    }
+    fsr_mat_.assign(m.n_reg, -1);
+    xs_tr_.clear();
+    xs_nf_.clear();
+    xs_f_.clear();
+    xs_ch_.clear();
+    xs_sc_.clear();
+    int imat = 0;
+    for (const auto &kv : fsr_map) {
+        const Material &mat = lib.by_id(kv.first);
+        for (int g = 0; g < G; g++) {
+            xs_tr_.push_back(mat.xstr[g]);
+            xs_nf_.push_back(mat.xsnf[g]);
+            xs_f_.push_back(mat.xsf[g]);
+            xs_ch_.push_back(mat.xsch[g]);
+        }
+        for (int to = 0; to < G; to++)
+            for (int from = 0; from < G; from++) xs_sc_.push_back(mat.scat[to][from]);
+        for (int f : kv.second) fsr_mat_[f] = imat;
+        imat++;
+    }
+    mocgpu_problem &v = view_;
+    v.n_group = G;
+    v.n_mat = imat;
+    v.fsr_mat = fsr_mat_.data();
+    v.xs_tr = xs_tr_.data();
+    v.xs_nf = xs_nf_.data();
+    v.xs_f = xs_f_.data();
+    v.xs_ch = xs_ch_.data();
+    v.xs_sc = xs_sc_.data();
+}
+
+void Problem::reload_xs(const std::string &xsl_path)
+{
+    mesh_->mat_lib.reload(xsl_path);
+    fill_xs();
 }
 
 }  // namespace mocc_amd

@@ -34,10 +34,14 @@ public:
     const CoreMesh &mesh() const { return *mesh_; }
     const RayData &rays() const { return *rays_; }
     int ndir_oct() const { return ndir_oct_; }
+    // Replace the cross sections by those of another XSL file with the same
+    // material names; geometry and rays are kept. The view is updated.
+    void reload_xs(const std::string &xsl_path);
 
 private:
     void build(const XmlNode &doc, const std::string &base_dir);
     void fill_view();
+    void fill_xs();
 
     std::unique_ptr<CoreMesh> mesh_;
     std::unique_ptr<RayData> rays_;

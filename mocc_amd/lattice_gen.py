@@ -132,6 +132,26 @@ def lattice_rows(n: int, set_id: int, guide_tubes: bool) -> list[list[int]]:
     return rows
 
 
+def deck_materials(d: Deck, seed: int) -> list[Material]:
+    """The deck's base materials (FUEL<s>, MOD<s> per material set)."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for s in range(d.n_sets):
+        mats.append(make_material(rng, f"FUEL{s}", d.groups, True))
+        mats.append(make_material(rng, f"MOD{s}", d.groups, False))
+    return mats
+
+
+def write_branch_xsl(d: Deck, seed: int, case_seed: int, rel: float, path: str) -> list[Material]:
+    """A cross-section branch case of the deck: every base cross section times
+    U[1-rel, 1+rel] drawn from case_seed; same material names, so it can
+    replace the deck's library (mocc_host_reload_xs)."""
+    rng = np.random.default_rng(case_seed)
+    mats = [perturb(m, rng, rel, m.name) for m in deck_materials(d, seed)]
+    write_xsl(path, mats)
+    return mats
+
+
 def write_deck(d: Deck, out_dir: str, seed: int, rel_perturb: float = 0.0) -> str:
     """Write <out_dir>/<name>.xml and .xsl; return the XML path."""
     os.makedirs(out_dir, exist_ok=True)

@@ -98,6 +98,7 @@ def host() -> C.CDLL:
         L.mocc_host_quadrature_modify.argtypes = [C.c_char_p, i32, P, P, P, i32]
         L.mocc_host_trace_ray.argtypes = [P, f64, f64, f64, f64, P, P, i32, P, P, i32]
         L.mocc_host_fsr_areas.argtypes = [P, P, i32]
+        L.mocc_host_reload_xs.argtypes = [P, C.c_char_p]
         _host = L
     return _host
 
@@ -143,6 +144,23 @@ def gpu() -> C.CDLL:
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_exp_table.restype = i32
         L.mocgpu_exp_table.argtypes = [P, i32]
+        L.mocgpu_create_like.argtypes = [P, C.POINTER(MocgpuProblem), C.POINTER(MocgpuOptions), C.POINTER(P)]
+        L.mocgpu_state_bytes.restype = i64
+        L.mocgpu_state_bytes.argtypes = [P]
+        L.mocgpu_geometry_bytes.restype = i64
+        L.mocgpu_geometry_bytes.argtypes = [P]
+        L.mocgpu_batch_create.argtypes = [i64, i32, C.POINTER(P)]
+        L.mocgpu_batch_destroy.argtypes = [P]
+        L.mocgpu_batch_last_error.restype = C.c_char_p
+        L.mocgpu_batch_last_error.argtypes = [P]
+        L.mocgpu_batch_add.argtypes = [P, P, f64, f64, i32, i32, C.POINTER(i32)]
+        L.mocgpu_batch_run.argtypes = [P, i32]
+        L.mocgpu_batch_poll.argtypes = [P, P, P, i32]
+        L.mocgpu_batch_remove.argtypes = [P, i32]
+        L.mocgpu_batch_size.restype = i32
+        L.mocgpu_batch_size.argtypes = [P]
+        L.mocgpu_batch_bytes_used.restype = i64
+        L.mocgpu_batch_bytes_used.argtypes = [P]
         L.mocgpu_updates_per_sweep.restype = i64
         L.mocgpu_updates_per_sweep.argtypes = [P]
         L.mocgpu_time_sweeps.argtypes = [P, i32, P, P]
@@ -173,6 +191,12 @@ class HostProblem:
         if host().mocc_host_load_string(xml.encode(), base_dir.encode(), C.byref(h)) != 0:
             raise RuntimeError(host().mocc_host_last_error().decode())
         return cls(h)
+
+    def reload_xs(self, xsl_path: str):
+        """Swap in the cross sections of another XSL with the same material
+        names (geometry and rays are kept)."""
+        if host().mocc_host_reload_xs(self.h, xsl_path.encode()) != 0:
+            raise RuntimeError(host().mocc_host_last_error().decode())
 
     def params(self) -> HostParams:
         out = HostParams()

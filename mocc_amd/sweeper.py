@@ -24,7 +24,10 @@ class NonPositiveFluxError(ValueError):
 
 class GpuMoCSweeper:
     def __init__(self, prob: HostProblem, n_inner: int | None = None, energy_mode: int = 0,
-                 use_cmfd: bool | None = None, band_rays: int = 0, **cmfd):
+                 use_cmfd: bool | None = None, band_rays: int = 0, like: "GpuMoCSweeper | None" = None,
+                 **cmfd):
+        """like: another sweeper on the same geometry whose device ray plan is
+        shared (mocgpu_create_like)."""
         L = gpu()
         opt = prob.options()
         if n_inner is not None:
@@ -38,7 +41,10 @@ class GpuMoCSweeper:
         self.prob = prob
         self.opt = opt
         self.h = C.c_void_p()
-        rc = L.mocgpu_create(prob.view, C.byref(opt), C.byref(self.h))
+        if like is not None:
+            rc = L.mocgpu_create_like(like.h, prob.view, C.byref(opt), C.byref(self.h))
+        else:
+            rc = L.mocgpu_create(prob.view, C.byref(opt), C.byref(self.h))
         if rc != 0:
             raise DeviceError(L.mocgpu_last_error(None).decode())
         self.F, self.G = prob.n_fsr, prob.n_group
@@ -184,6 +190,12 @@ class GpuMoCSweeper:
         self._check(gpu().mocgpu_eigen_iterate(self.h, C.byref(st)))
         return st
 
+    def state_bytes(self) -> int:
+        return int(gpu().mocgpu_state_bytes(self.h))
+
+    def geometry_bytes(self) -> int:
+        return int(gpu().mocgpu_geometry_bytes(self.h))
+
     # --- measurement --------------------------------------------------------
     def updates_per_sweep(self) -> int:
         return int(gpu().mocgpu_updates_per_sweep(self.h))
@@ -192,3 +204,74 @@ class GpuMoCSweeper:
         a, b = np.zeros(1), np.zeros(1)
         self._check(gpu().mocgpu_time_sweeps(self.h, n, ptr(a), ptr(b)))
         return float(a[0]), float(b[0])
+
+
+class CapacityError(RuntimeError):
+    pass
+
+
+class GpuBatch:
+    """Many problems advanced together (mocgpu_batch_*). Handles stay owned by
+    the caller and must outlive their membership."""
+
+    def __init__(self, budget_bytes: int = 0, use_graph: bool = True):
+        self.b = C.c_void_p()
+        rc = gpu().mocgpu_batch_create(int(budget_bytes), 1 if use_graph else 0, C.byref(self.b))
+        if rc != 0:
+            raise DeviceError(gpu().mocgpu_batch_last_error(None).decode())
+        self.members = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = gpu().mocgpu_batch_last_error(self.b).decode()
+            if rc == -7:
+                raise CapacityError(msg)
+            raise DeviceError(f"mocgpu batch error {rc}: {msg}")
+        return rc
+
+    def add(self, s: GpuMoCSweeper, k_tol: float, psi_tol: float, max_iter: int, min_iter: int = 0) -> int:
+        cid = C.c_int32()
+        self._check(gpu().mocgpu_batch_add(self.b, s.h, k_tol, psi_tol, max_iter, min_iter, C.byref(cid)))
+        self.members[cid.value] = s
+        return cid.value
+
+    def run(self, n_outer: int = 1):
+        self._check(gpu().mocgpu_batch_run(self.b, n_outer))
+
+    def poll(self, cap: int = 4096):
+        ids = (C.c_int32 * cap)()
+        sts = (EigenState * cap)()
+        n = self._check(gpu().mocgpu_batch_poll(self.b, ids, sts, cap))
+        return [(ids[i], sts[i]) for i in range(n)]
+
+    def remove(self, cid: int):
+        self._check(gpu().mocgpu_batch_remove(self.b, cid))
+        self.members.pop(cid, None)
+
+    def size(self) -> int:
+        return int(gpu().mocgpu_batch_size(self.b))
+
+    def bytes_used(self) -> int:
+        return int(gpu().mocgpu_batch_bytes_used(self.b))
+
+    def solve_all(self, chunk: int = 4, max_outer: int = 100000):
+        """Run until every member has finished; returns {id: EigenState}."""
+        done = {}
+        outer = 0
+        while len(done) < len(self.members) and outer < max_outer:
+            self.run(chunk)
+            outer += chunk
+            for cid, st in self.poll():
+                done[cid] = st
+        return done
+
+    def close(self):
+        if self.b:
+            gpu().mocgpu_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

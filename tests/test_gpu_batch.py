@@ -1,0 +1,128 @@
+"""Batch engine (mocgpu_batch_*): many problems advanced together by one
+captured outer-iteration graph.
+
+* each problem converges to the same eigenvalue as its one-at-a-time device
+  solve (same Jacobi energy sweeps; only the reduction trees differ);
+* a problem's result is bitwise the same alone and in a mix (the serving
+  contract);
+* graph replays, with eager device work and synchronizations between them,
+  give bitwise the result of eager launches after every replay;
+* admission respects the device-memory budget;
+* a handle created on a shared geometry solves bitwise like an independent one.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+K_TOL, PSI_TOL, MAX_IT = 1e-7, 1e-6, 3000
+
+
+@pytest.fixture(scope="module")
+def batch_decks(deck_dir):
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    decks = {
+        "pin1": lg.write_deck(lg.pin_lattice(1, 2, 16, 0.05, "b_pin1_2g"), deck_dir, 21),
+        "c1": lg.write_config("c1", deck_dir),
+        "c1pert": lg.write_deck(lg.c1_deck().__class__(**{**lg.c1_deck().__dict__, "name": "b_c1_pert"}),
+                                deck_dir, 1, rel_perturb=0.1),
+        "asm17": lg.write_deck(lg.pin_lattice(17, 2, 4, 0.1, "b_asm17_2g", guide_tubes=True), deck_dir, 2),
+    }
+    return {k: HostProblem.load(v) for k, v in decks.items()}
+
+
+def _sweepers(batch_decks, names):
+    from mocc_amd.sweeper import GpuMoCSweeper
+    return {n: GpuMoCSweeper(batch_decks[n], energy_mode=1) for n in names}
+
+
+def _solve_batch(sweepers, order, use_graph=True):
+    from mocc_amd.sweeper import GpuBatch
+    b = GpuBatch(use_graph=use_graph)
+    ids = {}
+    for n in order:
+        ids[b.add(sweepers[n], K_TOL, PSI_TOL, MAX_IT)] = n
+    res = b.solve_all(chunk=4)
+    out = {ids[cid]: st for cid, st in res.items()}
+    b.close()
+    return out
+
+
+def test_batch_matches_single(batch_decks):
+    names = ["pin1", "c1", "c1pert", "asm17"]
+    sw = _sweepers(batch_decks, names)
+    res = _solve_batch(sw, names)
+    assert set(res) == set(names)
+    for n in names:
+        print(f"case {n}", flush=True)
+        single = _sweepers(batch_decks, [n])[n]
+        rc, st1 = single.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+        assert rc == 0 and st1.converged
+        st = res[n]
+        assert st.converged, (n, st.iterations)
+        assert abs(st.k - st1.k) < 1e-9, (n, st.k, st1.k)
+        assert abs(st.iterations - st1.iterations) <= 1
+        f_b, f_s = sw[n].flux(), single.flux()
+        assert np.abs(f_b / f_s - 1.0).max() < 1e-7
+
+
+def test_alone_vs_mixed_bitwise(batch_decks):
+    target = "c1pert"
+    alone = _sweepers(batch_decks, [target])
+    r_alone = _solve_batch(alone, [target])[target]
+    f_alone = alone[target].flux()
+    for order in (["asm17", "pin1", target, "c1"], [target, "c1", "asm17"]):
+        sw = _sweepers(batch_decks, order)
+        r = _solve_batch(sw, order)[target]
+        assert r.k == r_alone.k and r.iterations == r_alone.iterations
+        assert r.err_psi == r_alone.err_psi
+        assert np.array_equal(sw[target].flux(), f_alone)
+
+
+def test_graph_replays_match_eager(batch_decks):
+    from mocc_amd.sweeper import GpuBatch
+    names = ["pin1", "c1", "asm17"]
+    sg, se = _sweepers(batch_decks, names), _sweepers(batch_decks, names)
+    bg, be = GpuBatch(use_graph=True), GpuBatch(use_graph=False)
+    for n in names:
+        bg.add(sg[n], K_TOL, PSI_TOL, MAX_IT)
+        be.add(se[n], K_TOL, PSI_TOL, MAX_IT)
+    for rep in range(12):
+        bg.run(1)
+        be.run(1)
+        bg.poll()
+        be.poll()
+        for n in names:  # eager reads and synchronizations between replays
+            assert np.array_equal(sg[n].flux(), se[n].flux()), (rep, n)
+            assert sg[n].total_fission() == se[n].total_fission()
+
+
+def test_admission_budget(batch_decks):
+    from mocc_amd.sweeper import CapacityError, GpuBatch
+    sw = _sweepers(batch_decks, ["c1", "c1pert"])
+    need = sw["c1"].state_bytes()
+    assert need > 0 and sw["c1pert"].state_bytes() == need
+    b = GpuBatch(budget_bytes=need + need // 2)
+    i1 = b.add(sw["c1"], K_TOL, PSI_TOL, MAX_IT)
+    with pytest.raises(CapacityError):
+        b.add(sw["c1pert"], K_TOL, PSI_TOL, MAX_IT)
+    assert b.size() == 1 and b.bytes_used() == need
+    b.remove(i1)
+    b.add(sw["c1pert"], K_TOL, PSI_TOL, MAX_IT)
+    res = b.solve_all()
+    assert len(res) == 1 and list(res.values())[0].converged
+
+
+def test_create_like_shares_geometry(batch_decks):
+    from mocc_amd.sweeper import DeviceError, GpuMoCSweeper
+    a = GpuMoCSweeper(batch_decks["c1"], energy_mode=1)
+    b = GpuMoCSweeper(batch_decks["c1pert"], energy_mode=1, like=a)
+    c = GpuMoCSweeper(batch_decks["c1pert"], energy_mode=1)
+    assert b.geometry_bytes() == c.geometry_bytes() > 0
+    rb, sb = b.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    rc, sc = c.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    assert rb == rc == 0 and sb.k == sc.k
+    assert np.array_equal(b.flux(), c.flux())
+    with pytest.raises(DeviceError):
+        GpuMoCSweeper(batch_decks["pin1"], energy_mode=1, like=a)
