@@ -8,8 +8,10 @@ Yamamoto 32 azimuths x 3 polar per octant, ray spacing 0.03 cm, reflective
 A step is one inner sweep of all 7 groups (sweep kernel + flux reduction),
 i.e. 2 * P * S * G segment-group updates. Inputs are resident in HBM before
 the timed region. The JSON line also carries the roofline of the sweep
-kernel (HIP-event kernel time on the library's stream) and the FP64 CPU
-oracle timed on this host.
+kernel (HIP-event kernel time on the library's stream), the FP64 CPU oracle
+timed on this host, the batched branch-case solves/s against one-at-a-time
+device solves ("batched") and the serving loop's p50/p99 latency under
+Poisson arrivals ("serving").
 """
 from __future__ import annotations
 
@@ -70,6 +72,132 @@ def cpu_baseline(hp, n_sweeps: int):
                       f"deck (after 1 warm-up), FP64, sweep1g only, OMP threads={threads}"}
 
 
+K_TOL, PSI_TOL, MAX_IT = 1e-7, 1e-6, 3000
+
+
+def oracle_k(factory, case):
+    """FP64 eigenvalue of a branch case from the oracle (reference defaults)."""
+    from oracle.binding import Oracle
+    hp = factory.host_problem(case)
+    o = Oracle(hp, n_inner=factory.n_inner)
+    rc, st, _ = o.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    return st.k
+
+
+def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
+    """Batched branch cases (one graph-replayed batch) against the same
+    problems solved one at a time with the single-problem device path; keff
+    of a sample checked against the FP64 oracle."""
+    from mocc_amd.branch import BranchFactory, c4_cases
+    from mocc_amd.sweeper import GpuBatch
+    factory = BranchFactory(os.path.join(work_dir, "c4"))
+    cases = c4_cases(n_cases)
+    t0 = time.perf_counter()
+    handles = [factory.make(c) for c in cases]
+    t_setup = time.perf_counter() - t0
+    batch = GpuBatch(use_graph=True)
+    ids = [batch.add(h, K_TOL, PSI_TOL, MAX_IT) for h in handles]
+    t0 = time.perf_counter()
+    res = batch.solve_all(chunk=8)
+    t_batch = time.perf_counter() - t0
+    k_batch = {i: res[cid].k for i, cid in enumerate(ids)}
+    iters = [res[cid].iterations for cid in ids]
+    n_conv = sum(res[cid].converged for cid in ids)
+    batch.close()
+    for h in handles:
+        h.close()
+    # serial: every (n_cases // n_serial)-th case, same mix
+    step = max(1, n_cases // n_serial)
+    sub = list(range(0, n_cases, step))[:n_serial]
+    sh = [factory.make(cases[i]) for i in sub]
+    t0 = time.perf_counter()
+    k_serial = {}
+    for i, h in zip(sub, sh):
+        rc, st = h.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+        k_serial[i] = st.k
+    t_serial = time.perf_counter() - t0
+    for h in sh:
+        h.close()
+    # keff against the FP64 oracle on a sample of the smaller problems
+    small = [i for i, c in enumerate(cases) if c.kind != "17x17"][:: max(1, n_cases // (2 * max(n_oracle, 1)))]
+    dk = [abs(k_batch[i] - oracle_k(factory, cases[i])) for i in small[:n_oracle]]
+    factory.close()
+    bat = n_cases / t_batch
+    ser = len(sub) / t_serial
+    return {
+        "workload": f"{n_cases} branch cases: 1/4 single pin, 1/2 3x3, 1/4 17x17 (25 guide tubes); "
+                    "2/7 groups alternating; CY 16x3; spacing from {0.02, 0.035, 0.05} cm; XS x U[0.9,1.1]",
+        "batched_solves_per_s": bat, "serial_solves_per_s": ser, "speedup": bat / ser,
+        "serial_sample": len(sub), "converged": int(n_conv), "outer_iterations_max": int(max(iters)),
+        "batch_s": t_batch, "setup_s": t_setup,
+        "keff_max_abs_err_vs_fp64": max(dk) if dk else None, "keff_checked": len(dk),
+        "keff_max_abs_diff_batch_vs_serial": max(abs(k_batch[i] - k_serial[i]) for i in sub),
+        "energy": "Jacobi in energy, Jacobi boundary update, k_tol 1e-7, psi_tol 1e-6",
+    }
+
+
+def bench_serving(work_dir: str, n_req: int, seed: int = 5):
+    """Poisson arrivals of branch-case requests served by the continuous-
+    batching loop at 0.5x and 1.0x of its burst throughput, beside the same
+    requests solved one at a time."""
+    import numpy as np
+    from mocc_amd.branch import BranchFactory, c5_request
+    from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
+    factory = BranchFactory(os.path.join(work_dir, "c5"))
+    rng = np.random.default_rng(seed)
+    cases = [c5_request(rng, i) for i in range(n_req)]
+    for c in cases:  # trace every geometry before timing
+        factory.host_problem(c)
+    srv = Server(factory, chunk=4)
+    burst = summarize(srv.serve(np.zeros(n_req), cases))
+    out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
+                       "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]",
+           "burst": burst}
+    for frac in (0.5, 1.0):
+        rate = frac * burst["solves_per_s"]
+        out[f"poisson_{frac:g}x"] = {"rate_per_s": rate,
+                                     **summarize(srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases))}
+    srv.close()
+    out["serial_burst"] = summarize(serve_serial(factory, np.zeros(n_req), cases))
+    factory.close()
+    return out
+
+
+def bench_c3(work_dir: str, with_oracle: bool):
+    """The 3x3-assembly core (51x51 pins, vacuum east/south, CMFD, 5 inner
+    iterations, CY 64x3 at 0.02 cm): full eigenvalue solve on the device with
+    the FP64 CMFD; optionally the FP64 oracle's solve of the same deck."""
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    from mocc_amd.sweeper import GpuMoCSweeper
+    t0 = time.perf_counter()
+    hp = HostProblem.load(lg.write_config("c3", work_dir))
+    t_trace = time.perf_counter() - t0
+    par = hp.params()
+    t0 = time.perf_counter()
+    dev = GpuMoCSweeper(hp)
+    t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc, st = dev.eigen_solve(par.k_tol, par.psi_tol, par.max_iter)
+    t_solve = time.perf_counter() - t0
+    out = {"workload": "3x3 assemblies of 17x17 (51x51 pins, 3 material sets), 7 groups, CY 64x3, "
+                       "spacing 0.02 cm, vacuum east/south, CMFD, 5 inner iterations",
+           "n_fsr": hp.n_fsr, "n_seg": hp.p.n_seg, "k": st.k, "outer_iterations": st.iterations,
+           "converged": bool(st.converged), "solve_s": t_solve, "trace_s": t_trace, "plan_upload_s": t_plan,
+           "updates_per_s": dev.updates_per_sweep() * par.n_inner * st.iterations / t_solve,
+           "k_tol": par.k_tol, "psi_tol": par.psi_tol}
+    dev.close()
+    if with_oracle:
+        from oracle.binding import Oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        o = Oracle(hp, n_inner=par.n_inner, use_cmfd=True, n_threads=threads)
+        t0 = time.perf_counter()
+        rc_o, st_o, _ = o.eigen_solve(par.k_tol, par.psi_tol, par.max_iter)
+        out["oracle"] = {"k": st_o.k, "outer_iterations": st_o.iterations, "solve_s": time.perf_counter() - t0,
+                         "cores": threads, "keff_abs_err": abs(st.k - st_o.k)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +205,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-sweeps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched branch-case section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--batch-cases", type=int, default=256)
+    ap.add_argument("--serial-cases", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--c3", action="store_true", help="add the full C3 core solve with CMFD")
+    ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on one GPU only "
@@ -134,6 +269,13 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hp, args.cpu_sweeps)
+    dev.close()
+    if not args.no_batch:
+        line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 6)
+    if not args.no_serve:
+        line["serving"] = bench_serving(deck_dir, args.serve_requests)
+    if args.c3:
+        line["c3"] = bench_c3(deck_dir, args.c3_oracle)
     print(json.dumps(line), flush=True)
 
 

@@ -1,0 +1,123 @@
+"""Cross-section branch cases of a few lattice geometries.
+
+A branch case is one lattice (a single pin, 3x3 pins, a 17x17 assembly or
+2x2 assemblies) with 2 or 7 groups and every cross section of the seeded base
+materials scaled by U[0.9, 1.1]. Geometries are traced once per (lattice,
+ray spacing) and their device ray plan is uploaded once; each case then only
+adds its own cross sections and state (mocgpu_create_like), so admitting a
+case costs a library reload and a few small device copies.
+"""
+from __future__ import annotations
+
+import os
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lattice_gen as lg
+from .native import HostProblem
+from .sweeper import GpuMoCSweeper
+
+N_AZIMUTHAL = 16  # per quadrant, CY with 3 polar angles
+LATTICES = ("pin", "3x3", "17x17", "2x2x17x17")
+
+
+def lattice_deck(kind: str, groups: int, spacing: float) -> lg.Deck:
+    name = f"br_{kind}_{groups}g_{int(round(spacing * 1000))}"
+    if kind == "pin":
+        return lg.pin_lattice(1, groups, N_AZIMUTHAL, spacing, name)
+    if kind == "3x3":
+        return lg.pin_lattice(3, groups, N_AZIMUTHAL, spacing, name)
+    if kind == "17x17":
+        return lg.pin_lattice(17, groups, N_AZIMUTHAL, spacing, name, guide_tubes=True)
+    if kind == "2x2x17x17":
+        d = lg.Deck(name, groups, N_AZIMUTHAL, spacing)
+        d.lattices = {1: (17, 17, lg.lattice_rows(17, 0, True))}
+        d.core = [[1, 1], [1, 1]]
+        return d
+    raise ValueError(kind)
+
+
+@dataclass
+class Case:
+    kind: str
+    groups: int
+    spacing: float
+    seed: int  # branch seed (cross-section draws)
+
+
+class BranchFactory:
+    """Makes device handles for branch cases. Rays are traced once per
+    (lattice, spacing); a prototype handle per geometry holds the device ray
+    plan that every case of that geometry shares."""
+
+    BASE_SEED = 4
+
+    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1):
+        self.work_dir = work_dir
+        self.energy_mode = energy_mode
+        self.n_inner = n_inner
+        self._hp: dict = {}
+        self._proto: dict = {}
+        self._lock = threading.Lock()
+        os.makedirs(work_dir, exist_ok=True)
+
+    def _host(self, kind: str, spacing: float) -> HostProblem:
+        key = (kind, spacing)
+        hp = self._hp.get(key)
+        if hp is None:
+            path = lg.write_deck(lattice_deck(kind, 2, spacing), self.work_dir, self.BASE_SEED)
+            hp = HostProblem.load(path)
+            self._hp[key] = hp
+        return hp
+
+    def deck(self, c: Case) -> lg.Deck:
+        return lattice_deck(c.kind, c.groups, c.spacing)
+
+    def host_problem(self, c: Case) -> HostProblem:
+        """The traced geometry of the case, loaded with the case's cross
+        sections (valid until the next call for the same geometry)."""
+        hp = self._host(c.kind, c.spacing)
+        d = self.deck(c)
+        xsl = os.path.join(self.work_dir, f"{d.name}_b{c.seed}.xsl")
+        lg.write_branch_xsl(d, self.BASE_SEED, c.seed, 0.1, xsl)
+        hp.reload_xs(xsl)
+        os.remove(xsl)
+        return hp
+
+    def make(self, c: Case) -> GpuMoCSweeper:
+        with self._lock:
+            hp = self.host_problem(c)
+            gkey = (c.kind, c.spacing)
+            proto = self._proto.get(gkey)
+            if proto is None:
+                proto = GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=False)
+                self._proto[gkey] = proto
+            return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=False,
+                                 like=proto)
+
+    def close(self):
+        for p in self._proto.values():
+            p.close()
+        self._proto.clear()
+
+
+def c4_cases(n: int = 256, seed: int = 4) -> list[Case]:
+    """The batched branch workload: a quarter single pins, half 3x3 lattices,
+    a quarter 17x17 assemblies; 2 and 7 groups alternating; ray spacing drawn
+    from {0.02, 0.035, 0.05} cm; per-case cross sections +-10 %."""
+    rng = np.random.default_rng(seed)
+    kinds = ["pin"] * (n // 4) + ["3x3"] * (n // 2) + ["17x17"] * (n - n // 4 - n // 2)
+    out = []
+    for i, k in enumerate(kinds):
+        sp = float(rng.choice([0.02, 0.035, 0.05]))
+        out.append(Case(k, 2 if i % 2 == 0 else 7, sp, 1000 + i))
+    return out
+
+
+def c5_request(rng: np.random.Generator, i: int) -> Case:
+    """One serving request: a lattice of {pin, 3x3, 17x17, 2x2 of 17x17} with
+    2 or 7 groups at 0.05 cm spacing and its own cross-section draws."""
+    kind = LATTICES[int(rng.integers(0, len(LATTICES)))]
+    return Case(kind, int(rng.choice([2, 7])), 0.05, 50000 + i)

@@ -144,7 +144,7 @@ struct mocgpu_handle {
         a.G = G;
         a.F = F;
         a.bcpg = plan->bcpg;
-        a.lds_local = plan->max_local + 1;
+        a.lds_local = plan->max_local + kDummySlots;
         a.lds_lsurf = std::max(1, plan->max_lsurf);
         a.n_slots = (int)plan->loc_fsr.size();
         a.n_lslots = (int)std::max<size_t>(1, plan->lsurf_glob.size());

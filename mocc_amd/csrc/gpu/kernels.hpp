@@ -20,7 +20,7 @@ struct SweepArgs {
     int32_t n_bands;
     int32_t g_fixed;        // >= 0: one group; < 0: all groups (item = band_rank*G + g)
     int32_t G, F, bcpg;
-    int32_t lds_local;      // LDS FSR slots per workgroup (max band-local FSRs + 1)
+    int32_t lds_local;      // LDS FSR slots per workgroup (max band-local FSRs + kDummySlots)
     int32_t lds_lsurf;      // LDS coarse-surface slots (0 without currents)
     int32_t n_slots;        // partial slots per group (psum is [G][n_slots])
     int32_t n_lslots;       // current slots per group (csum/ssum are [G][n_lslots])

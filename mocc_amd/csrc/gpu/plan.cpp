@@ -284,7 +284,8 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
                                 pl.seg_loc.push_back((uint16_t)g2l[p.seg_fsr[s0 + k]]);
                                 pl.seg_len.push_back((float)p.seg_len[s0 + k]);
                             } else {
-                                pl.seg_loc.push_back((uint16_t)n_local);
+                                // padding: a zero-length step into the lane's own dummy slot
+                                pl.seg_loc.push_back((uint16_t)(n_local + j));
                                 pl.seg_len.push_back(0.0f);
                             }
                         }

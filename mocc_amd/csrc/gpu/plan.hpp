@@ -9,7 +9,8 @@
 //    seg_base[d] + ((t / kChunk) * n_rays + j) * kChunk + t % kChunk, so one
 //    lane fetches kChunk steps with one 16-byte and two 16-byte loads and the
 //    lanes of a wave read one contiguous block; short rays are padded with
-//    the band's dummy local slot (n_local) and zero length.
+//    zero-length steps into the dummy slot n_local + j of ray j (one per
+//    lane, so padding lanes never share an LDS address).
 //  * Every band-local FSR owns one partial slot; fsr_slot_ptr/idx list the
 //    slots of each FSR in increasing order for the final integer reduction.
 //  * Coarse-surface crossings (moc::Current) are flattened into per-ray,
@@ -27,6 +28,7 @@ namespace gpu {
 
 constexpr int kMaxLocal = 2047;  // band-local FSRs held in LDS
 constexpr int kMaxRaysPerBand = 64;
+constexpr int kDummySlots = kMaxRaysPerBand;  // padding slots after a band's n_local FSRs
 constexpr int kChunk = 8;         // segments per lane per vector load (chunked layout)
 constexpr int kExpN = 10000;      // Exponential_Linear<10000> on [-10, 0]
 constexpr int kExpTable = 40000;  // table nodes: the reference's 10000 + 1 - exp(-tau) to tau = 40

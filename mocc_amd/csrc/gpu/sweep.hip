@@ -89,9 +89,9 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         tal[l] = 0ull;
         lmax = max(lmax, __float_as_uint(fabsf(q32)));
     }
-    if (tid == 0) {
-        sq[B.n_local] = make_float2(0.0f, 0.0f);
-        tal[B.n_local] = 0ull;
+    if (tid < kDummySlots) {
+        sq[B.n_local + tid] = make_float2(0.0f, 0.0f);
+        tal[B.n_local + tid] = 0ull;
     }
     if (CUR) {
         for (int l = tid; l < B.n_lsurf; l += kWG) {
@@ -100,8 +100,12 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         }
     }
 
-    const int dir = tid / kRays;
+    // Each wave takes every ray of the band, even rays in one direction and
+    // odd rays in the other (the other wave the reverse): neighbouring rays
+    // then enter a given FSR at different steps, so fewer lanes of a wave hit
+    // the same LDS tally address in the same step.
     const int j = tid % kRays;
+    const int dir = (j & 1) ^ (tid / kRays);
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
     // select (not index) the per-direction offsets: indexing a local array

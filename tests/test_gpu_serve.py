@@ -1,0 +1,67 @@
+"""Serving loop (continuous batching over the batch engine): every request
+is answered, and each answer is bitwise the answer of the same request solved
+alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def factory(tmp_path_factory):
+    from mocc_amd.branch import BranchFactory
+    f = BranchFactory(str(tmp_path_factory.mktemp("branch")))
+    yield f
+    f.close()
+
+
+def _requests(n, seed):
+    from mocc_amd.branch import Case
+    rng = np.random.default_rng(seed)
+    kinds = ["pin", "3x3", "17x17"]
+    cases = [Case(kinds[int(rng.integers(0, 3))], int(rng.choice([2, 7])), 0.1, 900 + i) for i in range(n)]
+    return cases
+
+
+def _solve_alone(factory, case, k_tol=1e-7, psi_tol=1e-6):
+    from mocc_amd.sweeper import GpuBatch
+    s = factory.make(case)
+    b = GpuBatch()
+    b.add(s, k_tol, psi_tol, 3000)
+    (cid, st), = list(b.solve_all().items())
+    flux = s.flux()
+    b.close()
+    s.close()
+    return st, flux
+
+
+def test_serve_matches_alone(factory):
+    from mocc_amd.serve import Server, poisson_arrivals, summarize
+    cases = _requests(16, 3)
+    arrivals = poisson_arrivals(len(cases), rate=200.0, seed=3)
+    srv = Server(factory, chunk=2, keep_flux=True)
+    recs = srv.serve(arrivals, cases)
+    srv.close()
+    summ = summarize(recs)
+    print(summ, flush=True)
+    assert summ["converged"] == len(cases)
+    for r in recs[::5]:
+        print(f"request {r.index} {r.case}", flush=True)
+        st, flux = _solve_alone(factory, r.case)
+        assert st.k == r.k and st.iterations == r.iterations
+        assert np.array_equal(flux, r.flux)
+
+
+def test_serve_under_memory_budget(factory):
+    """A budget that holds about two problems at a time still serves all."""
+    from mocc_amd.serve import Server, poisson_arrivals
+    cases = _requests(6, 5)
+    probe = factory.make(cases[0])
+    budget = 2 * max(probe.state_bytes(), 1)
+    probe.close()
+    # every request fits alone: use cases no larger than the probe's
+    cases = [c for c in cases if c.kind == cases[0].kind and c.groups <= cases[0].groups] or cases[:1]
+    srv = Server(factory, budget_bytes=budget, chunk=2)
+    recs = srv.serve(poisson_arrivals(len(cases), rate=1000.0, seed=5), cases)
+    srv.close()
+    assert all(r.converged for r in recs)
