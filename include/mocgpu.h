@@ -132,7 +132,8 @@ typedef struct mocgpu_options {
     double cmfd_psi_tol;    /* <cmfd psi_tol>, default 1e-5                */
     double cmfd_resid_reduction; /* <cmfd residual_reduction>, default 1e-3 */
     int32_t cmfd_solver;    /* 0 BiCGSTAB (Eigen default), 1 GMRES         */
-    int32_t reserved;
+    int32_t gs_boundary;    /* <sweeper boundary_update>: 1 Gauss-Seidel in angle
+                               (mocc's default, moc_sweeper.cpp:105-115), 0 Jacobi */
 } mocgpu_options;
 
 typedef struct mocgpu_handle mocgpu_handle;

@@ -6,8 +6,9 @@
 // problems at once:
 //
 //   fission source -> phi_old -> group sources (Jacobi in energy) ->
-//   n_inner x [ work-queue sweep of every (problem, band, group) item ->
-//               flux reduction -> boundary parity flip ] ->
+//   n_inner x [ work-queue sweep of every (problem, band, group) item (two
+//               queues for Gauss-Seidel problems: octant-1 bands, then
+//               octant-2 bands) -> flux reduction -> boundary parity flip ] ->
 //   per-problem reductions (total fission + k, normalization, err_psi) ->
 //   per-problem convergence test.
 //
@@ -216,6 +217,7 @@ struct mocgpu_batch {
     DBuf<int32_t> d_status, d_ninner;
     DBuf<uint32_t> d_counter;
     int n_bmF = 0, n_bmFG = 0, n_bmR = 0, n_items = 0, max_inner = 1, q_blocks = 1;
+    int phase_first[2] = {0, 0}, phase_items[2] = {0, 0};
     size_t lds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -260,7 +262,7 @@ struct mocgpu_batch {
             int64_t cost;
             int4 w;
         };
-        std::vector<It> items;
+        std::vector<It> items[2];
         lds = 0;
         max_inner = 1;
         for (int i = 0; i < n; i++) {
@@ -284,7 +286,8 @@ struct mocgpu_batch {
             d.fs_prev = h->d_fs_prev.p;
             d.scal = h->d_scal.p;
             d.parity = h->d_parity.p;
-            d.gmask = h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull);
+            // Gauss-Seidel problems keep one boundary buffer: nothing to flip
+            d.gmask = h->opt.gs_boundary ? 0ull : (h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull));
             SweepArgs a = h->sweep_args(-1);
             a.parity_dev = h->d_parity.p;
             a.lds_lsurf = 0;
@@ -298,17 +301,25 @@ struct mocgpu_batch {
             cF[i] = (size_t)h->F;
             cFG[i] = (size_t)h->F * h->G;
             cR[i] = (size_t)h->F * h->G * reduce_flux_lanes();
+            // phase 0: every band (Jacobi) or the octant-1 bands (Gauss-Seidel);
+            // phase 1: the octant-2 bands of Gauss-Seidel problems
             const SweepPlan &pl = *h->plan;
             for (size_t r = 0; r < pl.band_order.size(); r++) {
                 const int band = pl.band_order[r];
                 const int64_t cost = (int64_t)pl.bands[band].nmax;
-                for (int g = 0; g < h->G; g++) items.push_back(It{cost, make_int4(i, band, g, 0)});
+                const int ph = (h->opt.gs_boundary && pl.bands[band].azi >= pl.A / 2) ? 1 : 0;
+                for (int g = 0; g < h->G; g++) items[ph].push_back(It{cost, make_int4(i, band, g, 0)});
             }
         }
-        // longest items first over all problems (the G items of a band stay together)
-        std::stable_sort(items.begin(), items.end(), [](const It &x, const It &y) { return x.cost > y.cost; });
-        std::vector<int4> iv(items.size());
-        for (size_t k = 0; k < items.size(); k++) iv[k] = items[k].w;
+        // per phase, longest items first over all problems (the G items of a
+        // band stay together)
+        std::vector<int4> iv;
+        for (int ph = 0; ph < 2; ph++) {
+            std::stable_sort(items[ph].begin(), items[ph].end(), [](const It &x, const It &y) { return x.cost > y.cost; });
+            phase_first[ph] = (int)iv.size();
+            for (const It &it : items[ph]) iv.push_back(it.w);
+            phase_items[ph] = (int)items[ph].size();
+        }
         n_items = (int)iv.size();
         auto bF = block_map(cF), bFG = block_map(cFG), bR = block_map(cR);
         n_bmF = (int)bF.size();
@@ -327,7 +338,7 @@ struct mocgpu_batch {
         int dev = 0, ncu = 0;
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        q_blocks = std::max(1, std::min(n_items, ncu * queue_blocks_per_cu(NP, lds)));
+        q_blocks = std::max(1, ncu * queue_blocks_per_cu(NP, lds));
         sync();
         dirty = false;
     }
@@ -341,16 +352,19 @@ struct mocgpu_batch {
         hipLaunchKernelGGL(k_b_copy_phi, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
         hipLaunchKernelGGL(k_b_source, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
         for (int inner = 0; inner < max_inner; inner++) {
-            hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, d_counter.p);
-            QueueArgs q{};
-            q.cases = d_sweep.p;
-            q.items = d_items.p;
-            q.n_items = n_items;
-            q.inner = inner;
-            q.counter = d_counter.p;
-            q.status = d_status.p;
-            q.n_inner = d_ninner.p;
-            launch_sweep_queue(q, NP, lds, q_blocks, s);
+            for (int ph = 0; ph < 2; ph++) {
+                if (phase_items[ph] == 0) continue;
+                hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, d_counter.p);
+                QueueArgs q{};
+                q.cases = d_sweep.p;
+                q.items = d_items.p + phase_first[ph];
+                q.n_items = phase_items[ph];
+                q.inner = inner;
+                q.counter = d_counter.p;
+                q.status = d_status.p;
+                q.n_inner = d_ninner.p;
+                launch_sweep_queue(q, NP, lds, std::min(q_blocks, phase_items[ph]), s);
+            }
             launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
             hipLaunchKernelGGL(k_b_flip, dim3(nblk(n, 64)), dim3(64), 0, s, d_desc.p, d_status.p, d_ninner.p, n,
                                inner);

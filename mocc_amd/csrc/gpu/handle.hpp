@@ -104,7 +104,7 @@ struct mocgpu_handle {
 
     // plan
     DBuf<BandInfo> d_bands;
-    DBuf<int32_t> d_order, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
+    DBuf<int32_t> d_order, d_order_gs, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
     DBuf<uint16_t> d_seg_loc;
     DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw, d_exptab;
     DBuf<RayInfo> d_rays;
@@ -149,6 +149,7 @@ struct mocgpu_handle {
         a.n_slots = (int)plan->loc_fsr.size();
         a.n_lslots = (int)std::max<size_t>(1, plan->lsurf_glob.size());
         a.parity = parity;
+        a.gs = opt.gs_boundary ? 1 : 0;
         a.seg_loc = d_seg_loc.p;
         a.seg_len = d_seg_len.p;
         a.loc_fsr = d_loc_fsr.p;

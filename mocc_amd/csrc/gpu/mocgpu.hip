@@ -247,8 +247,18 @@ void mocgpu_handle::build_source(int g_fixed, bool jacobi)
 void mocgpu_handle::sweep_once(int g, bool with_current)
 {
     SweepArgs a = sweep_args(g);
-    int items = sweep_grid_items((int)plan->bands.size(), G, g < 0);
-    launch_sweep(a, NP, with_current, items, stream);
+    if (opt.gs_boundary) {
+        // Gauss-Seidel in angle: octant-1 bands, then octant-2 bands
+        const int n1 = plan->n_bands_oct1, n2 = (int)plan->bands.size() - n1;
+        a.order = d_order_gs.p;
+        a.n_bands = n1;
+        if (n1 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n1, G, g < 0), stream);
+        a.order = d_order_gs.p + n1;
+        a.n_bands = n2;
+        if (n2 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n2, G, g < 0), stream);
+    } else {
+        launch_sweep(a, NP, with_current, sweep_grid_items((int)plan->bands.size(), G, g < 0), stream);
+    }
     launch_reduce_flux(reduce_args(g), stream);
     if (with_current) {
         int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
@@ -258,8 +268,10 @@ void mocgpu_handle::sweep_once(int g, bool with_current)
                            d_surf_ptr.p, d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
         has_radial = true;
     }
-    if (g >= 0) parity ^= (1ull << g);
-    else parity ^= (G >= 64 ? ~0ull : ((1ull << G) - 1ull));
+    if (!opt.gs_boundary) {
+        if (g >= 0) parity ^= (1ull << g);
+        else parity ^= (G >= 64 ? ~0ull : ((1ull << G) - 1ull));
+    }
 }
 
 void mocgpu_handle::copy_flux_to_old()
@@ -301,6 +313,7 @@ void mocgpu_default_options(mocgpu_options *o)
     o->cmfd_psi_tol = 1.0e-5;
     o->cmfd_resid_reduction = 0.001;
     o->cmfd_solver = 0;
+    o->gs_boundary = 1;
 }
 
 const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
@@ -358,6 +371,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         if (like) {
             h->d_bands = like->d_bands;
             h->d_order = like->d_order;
+            h->d_order_gs = like->d_order_gs;
             h->d_seg_loc = like->d_seg_loc;
             h->d_seg_len = like->d_seg_len;
             h->d_loc_fsr = like->d_loc_fsr;
@@ -386,6 +400,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         } else {
         h->d_bands.upload(pl.bands, s);
         h->d_order.upload(pl.band_order, s);
+        h->d_order_gs.upload(pl.band_order_gs, s);
         h->d_seg_loc.upload(pl.seg_loc, s);
         h->d_seg_len.upload(pl.seg_len, s);
         h->d_loc_fsr.upload(pl.loc_fsr, s);
@@ -533,7 +548,7 @@ int64_t mocgpu_state_bytes(const mocgpu_handle *h)
 int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
 {
     if (!h) return 0;
-    const size_t b = h->d_bands.bytes() + h->d_order.bytes() + h->d_seg_loc.bytes() + h->d_seg_len.bytes() +
+    const size_t b = h->d_bands.bytes() + h->d_order.bytes() + h->d_order_gs.bytes() + h->d_seg_loc.bytes() + h->d_seg_len.bytes() +
                      h->d_loc_fsr.bytes() + h->d_rays.bytes() + h->d_azi.bytes() + h->d_ang_rsin.bytes() +
                      h->d_ang_wt.bytes() + h->d_ang_cw.bytes() + h->d_ang_fw.bytes() + h->d_slot_ptr.bytes() +
                      h->d_slot_idx.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +

@@ -392,6 +392,14 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
     std::stable_sort(pl.band_order.begin(), pl.band_order.end(), [&](int x, int y) {
         return (int64_t)pl.bands[x].nmax > (int64_t)pl.bands[y].nmax;
     });
+    // Gauss-Seidel order in angle: every band of an octant-1 azimuth, then
+    // every band of an octant-2 azimuth (each longest-first)
+    pl.band_order_gs.clear();
+    for (int oct = 0; oct < 2; oct++) {
+        for (int b : pl.band_order)
+            if ((pl.bands[b].azi >= pl.A / 2) == (oct == 1)) pl.band_order_gs.push_back(b);
+        if (oct == 0) pl.n_bands_oct1 = (int32_t)pl.band_order_gs.size();
+    }
     return pl;
 }
 

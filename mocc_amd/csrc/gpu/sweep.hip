@@ -116,8 +116,11 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const size_t gbase = (size_t)g * 2 * A.bcpg;
     const uint64_t parity = A.parity_dev ? *A.parity_dev : A.parity;
     const int par = (int)((parity >> g) & 1ull);
-    const float *__restrict__ psi_in = A.psi + gbase + (size_t)par * A.bcpg;
-    float *__restrict__ psi_out = A.psi + gbase + (size_t)(1 - par) * A.bcpg;
+    // Jacobi: exit fluxes go to the other buffer, read by the next sweep.
+    // Gauss-Seidel: they go straight to the incoming buffer; a phase only
+    // writes the incoming slots of the other phase's angles.
+    const float *psi_in = A.psi + gbase + (size_t)par * A.bcpg;
+    float *psi_out = A.psi + gbase + (size_t)(A.gs ? par : 1 - par) * A.bcpg;
 
     // rk[p] = 1000 / sin(theta_p): tau * 1000 is the table coordinate
     float psi[NP], rk[NP], wt[NP];

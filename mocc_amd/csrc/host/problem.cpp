@@ -18,6 +18,7 @@ void host_default_options(mocgpu_options *o)
     o->cmfd_psi_tol = 1.0e-5;
     o->cmfd_resid_reduction = 0.001;
     o->cmfd_solver = 0;
+    o->gs_boundary = 1;
 }
 
 namespace {
@@ -91,6 +92,7 @@ void Problem::build(const XmlNode &doc, const std::string &base_dir)
         else if (b == "gs") params_.gauss_seidel_boundary = true;
         else throw Error("Unrecognized boundary update option.");
     }
+    opts_.gs_boundary = params_.gauss_seidel_boundary ? 1 : 0;
     const XmlNode *rays = sweeper->child("rays");
     if (!rays) throw Error("No input privided for ray spacing.");
     AngularQuadrature quad(*find_angquad(sweeper));

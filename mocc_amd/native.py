@@ -51,7 +51,7 @@ class MocgpuOptions(C.Structure):
         ("n_inner", i32), ("energy_mode", i32), ("lane_map", i32), ("band_rays", i32),
         ("use_cmfd", i32), ("cmfd_max_iter", i32), ("cmfd_k_tol", f64),
         ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("cmfd_solver", i32),
-        ("reserved", i32),
+        ("gs_boundary", i32),
     ]
 
 

@@ -25,13 +25,16 @@ class NonPositiveFluxError(ValueError):
 class GpuMoCSweeper:
     def __init__(self, prob: HostProblem, n_inner: int | None = None, energy_mode: int = 0,
                  use_cmfd: bool | None = None, band_rays: int = 0, like: "GpuMoCSweeper | None" = None,
-                 **cmfd):
-        """like: another sweeper on the same geometry whose device ray plan is
-        shared (mocgpu_create_like)."""
+                 gs_boundary: bool | None = None, **cmfd):
+        """Options default to the deck's. like: another sweeper on the same
+        geometry whose device ray plan is shared (mocgpu_create_like);
+        gs_boundary: Gauss-Seidel (mocc's default) or Jacobi boundary update."""
         L = gpu()
         opt = prob.options()
         if n_inner is not None:
             opt.n_inner = n_inner
+        if gs_boundary is not None:
+            opt.gs_boundary = 1 if gs_boundary else 0
         opt.energy_mode = energy_mode
         opt.band_rays = band_rays
         if use_cmfd is not None:
