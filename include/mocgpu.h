@@ -124,8 +124,11 @@ typedef struct mocgpu_options {
     int32_t n_inner;        /* <sweeper n_inner>                           */
     int32_t energy_mode;    /* 0: Gauss-Seidel over groups (FixedSourceSolver::step),
                                1: Jacobi (all groups from the previous outer) */
-    int32_t lane_map;       /* -1 autotune; else a ray/polar/group -> lane mapping id */
-    int32_t band_rays;      /* rays per band (0 = library default)         */
+    int32_t lane_map;       /* ray/direction -> lane mapping of the sweep: 1 (default)
+                               ray-parity interleave of directions in each wave, 0 one
+                               direction per wave, -1 autotune lane_map x band_rays at
+                               create (times candidate plans on the device)          */
+    int32_t band_rays;      /* rays per band, 1..64 (0 = 64)                */
     int32_t use_cmfd;       /* <solver cmfd="t">                           */
     int32_t cmfd_max_iter;  /* <cmfd max_iter>, default 100                */
     double cmfd_k_tol;      /* <cmfd k_tol>, default 1e-6                  */
@@ -140,6 +143,10 @@ typedef struct mocgpu_handle mocgpu_handle;
 
 /* Fill *opt with the reference defaults. */
 void mocgpu_default_options(mocgpu_options *opt);
+
+/* The options a handle runs with (band_rays and lane_map as chosen by the
+ * autotuner when created with lane_map = -1). */
+int mocgpu_get_options(const mocgpu_handle *h, mocgpu_options *out);
 
 /* Build the device-resident sweeper for one problem.
  * Replaces moc::MoCSweeper::MoCSweeper (src/sweepers/moc/moc_sweeper.cpp:62-187)

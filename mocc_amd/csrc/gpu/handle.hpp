@@ -101,6 +101,7 @@ struct mocgpu_handle {
     bool has_fs = false;
     bool has_radial = false;
     int cmfd_iterations = 0;
+    double tuned_ms = 0.0;  // sweep kernel time of the autotuned lane map
 
     // plan
     DBuf<BandInfo> d_bands;
@@ -150,6 +151,7 @@ struct mocgpu_handle {
         a.n_lslots = (int)std::max<size_t>(1, plan->lsurf_glob.size());
         a.parity = parity;
         a.gs = opt.gs_boundary ? 1 : 0;
+        a.interleave = opt.lane_map == 0 ? 0 : 1;
         a.seg_loc = d_seg_loc.p;
         a.seg_len = d_seg_len.p;
         a.loc_fsr = d_loc_fsr.p;

@@ -305,7 +305,7 @@ void mocgpu_default_options(mocgpu_options *o)
     *o = mocgpu_options{};
     o->n_inner = 1;
     o->energy_mode = 0;
-    o->lane_map = -1;
+    o->lane_map = 1;
     o->band_rays = 0;
     o->use_cmfd = 0;
     o->cmfd_max_iter = 100;
@@ -347,6 +347,9 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
             for (int f = 0; same && f < p->n_fsr; f++) same = p->fsr_mat[f] == like->h_fsr_mat[f];
             if (!same) throw std::runtime_error("mocgpu_create_like: the problem's geometry differs from the template's");
             h->plan = like->plan;
+            // the shared plan fixes the band width and the lane map
+            h->opt.band_rays = like->opt.band_rays;
+            h->opt.lane_map = like->opt.lane_map;
         } else {
             h->plan = std::make_shared<SweepPlan>(build_plan(*p, h->opt.band_rays));
         }
@@ -514,13 +517,75 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
     *out = h.release();
     return 0;
 }
+// Autotuner of the ray -> lane mapping: build the plan for each band width,
+// time the sweep kernel (HIP events, a few all-group sweeps) with each lane
+// map, keep the fastest (band width, lane map) and return its handle reset to
+// the initial state.
+int autotune_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
+{
+    mocgpu_options o = *opt;
+    const int widths[3] = {64, 48, 32};
+    const int maps[2] = {1, 0};
+    mocgpu_handle *best = nullptr;
+    double best_ms = 0.0;
+    int best_map = 1;
+    for (int w : widths) {
+        o.band_rays = w;
+        o.lane_map = 1;
+        mocgpu_handle *h = nullptr;
+        int rc = create_impl(p, &o, nullptr, &h);
+        if (rc == 0) rc = mocgpu_initialize(h);
+        if (rc != 0) {
+            if (h) mocgpu_destroy(h);
+            if (best) mocgpu_destroy(best);
+            return rc;
+        }
+        bool keep = false;
+        for (int m : maps) {
+            h->opt.lane_map = m;
+            double ms = 0.0, kms = 0.0;
+            rc = mocgpu_time_sweeps(h, 3, &ms, &kms);
+            if (rc != 0) {
+                mocgpu_destroy(h);
+                if (best) mocgpu_destroy(best);
+                return rc;
+            }
+            if (!best || kms < best_ms) {
+                best_ms = kms;
+                best_map = m;
+                keep = true;
+                if (best && best != h) mocgpu_destroy(best);
+                best = h;
+            }
+        }
+        if (!keep) mocgpu_destroy(h);
+    }
+    best->opt.lane_map = best_map;
+    best->tuned_ms = best_ms;
+    int rc = mocgpu_initialize(best);
+    if (rc != 0) {
+        mocgpu_destroy(best);
+        return rc;
+    }
+    *out = best;
+    return 0;
+}
 }  // namespace
 
 extern "C" {
 
 int mocgpu_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_handle **out)
 {
+    if (opt && opt->lane_map == -1) return autotune_create(p, opt, out);
     return create_impl(p, opt, nullptr, out);
+}
+
+int mocgpu_get_options(const mocgpu_handle *h, mocgpu_options *out)
+{
+    if (!h || !out) return MOCGPU_ERR_ARG;
+    *out = h->opt;
+    if (out->band_rays <= 0) out->band_rays = kMaxRaysPerBand;
+    return 0;
 }
 
 int mocgpu_create_like(const mocgpu_handle *like, const mocgpu_problem *p, const mocgpu_options *opt,
@@ -819,6 +884,7 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double
 {
     return guard(h, [&] {
         SweepArgs a = h->sweep_args(-1);
+        a.gs = 0;  // one launch over every band, reading and writing separate buffers
         int items = sweep_grid_items((int)h->plan->bands.size(), h->G, true);
         // warm-up
         launch_sweep(a, h->NP, false, items, h->stream);

@@ -100,12 +100,13 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         }
     }
 
-    // Each wave takes every ray of the band, even rays in one direction and
-    // odd rays in the other (the other wave the reverse): neighbouring rays
-    // then enter a given FSR at different steps, so fewer lanes of a wave hit
+    // Lane map. Plain: one wave sweeps the band forward, the other backward.
+    // Interleaved: each wave takes every ray, even rays in one direction and
+    // odd rays in the other (the other wave the reverse), so neighbouring
+    // rays enter a given FSR at different steps and fewer lanes of a wave hit
     // the same LDS tally address in the same step.
     const int j = tid % kRays;
-    const int dir = (j & 1) ^ (tid / kRays);
+    const int dir = A.interleave ? ((j & 1) ^ (tid / kRays)) : (tid / kRays);
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
     // select (not index) the per-direction offsets: indexing a local array

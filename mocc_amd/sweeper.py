@@ -25,7 +25,7 @@ class NonPositiveFluxError(ValueError):
 class GpuMoCSweeper:
     def __init__(self, prob: HostProblem, n_inner: int | None = None, energy_mode: int = 0,
                  use_cmfd: bool | None = None, band_rays: int = 0, like: "GpuMoCSweeper | None" = None,
-                 gs_boundary: bool | None = None, **cmfd):
+                 gs_boundary: bool | None = None, lane_map: int | None = None, **cmfd):
         """Options default to the deck's. like: another sweeper on the same
         geometry whose device ray plan is shared (mocgpu_create_like);
         gs_boundary: Gauss-Seidel (mocc's default) or Jacobi boundary update."""
@@ -35,6 +35,8 @@ class GpuMoCSweeper:
             opt.n_inner = n_inner
         if gs_boundary is not None:
             opt.gs_boundary = 1 if gs_boundary else 0
+        if lane_map is not None:
+            opt.lane_map = lane_map
         opt.energy_mode = energy_mode
         opt.band_rays = band_rays
         if use_cmfd is not None:
@@ -192,6 +194,13 @@ class GpuMoCSweeper:
     def eigen_iterate(self, st: EigenState):
         self._check(gpu().mocgpu_eigen_iterate(self.h, C.byref(st)))
         return st
+
+    def options(self):
+        """Effective options (band_rays, lane_map after autotuning)."""
+        from .native import MocgpuOptions
+        o = MocgpuOptions()
+        self._check(gpu().mocgpu_get_options(self.h, C.byref(o)))
+        return o
 
     def state_bytes(self) -> int:
         return int(gpu().mocgpu_state_bytes(self.h))

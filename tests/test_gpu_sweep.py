@@ -103,3 +103,21 @@ def test_c2_fss_step_parity(c2, gs):
         s.fss_step(with_current=True)
     assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
     assert _rel(dev.boundary(), ora.boundary()) < 1e-4
+
+
+def test_autotuned_lane_map(c1):
+    """lane_map = -1: the autotuner picks a band width and lane map by timing
+    the candidates; the tuned handle sweeps as accurately as the default."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    dev = GpuMoCSweeper(c1, n_inner=1, energy_mode=0, gs_boundary=True, lane_map=-1)
+    o = dev.options()
+    assert o.band_rays in (32, 48, 64) and o.lane_map in (0, 1)
+    ora = Oracle(c1, n_inner=1, gs_boundary=True, energy_mode=0)
+    rng = np.random.default_rng(2)
+    phi0 = rng.uniform(0.5, 2.0, (c1.n_fsr, c1.n_group))
+    for s in (dev, ora):
+        s.initialize()
+        s.set_flux(phi0)
+        s.calc_fission_source(1.0)
+        s.fss_step(with_current=False)
+    assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
