@@ -315,6 +315,13 @@ int mocgpu_batch_remove(mocgpu_batch *b, int32_t id);
 int32_t mocgpu_batch_size(const mocgpu_batch *b);
 int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b);
 
+/* Build the band plan of a problem on the host (no device) and verify its
+ * invariants: every segment once per direction in the streams (backward
+ * reversed), padding, band-local FSR slots, slot lists, band orders, current
+ * events. Writes a one-line report. No reference counterpart (a test of this
+ * build's data layout). */
+int mocgpu_plan_check(const mocgpu_problem *p, int32_t band_rays, char *msg, int32_t cap);
+
 /* Number of (2-D segment, polar, direction, group) updates per inner sweep of
  * all groups: 2 * P * S * G. */
 int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h);

@@ -1,6 +1,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
@@ -318,6 +319,7 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
                 ri.flags = fw_y | (bw_y << 1) | ((p.bc[fw_face] == MOCGPU_BC_VACUUM) << 2) |
                            ((p.bc[bw_face] == MOCGPU_BC_VACUUM) << 3);
                 pl.rays.push_back(ri);
+                pl.ray_global.push_back(r);
                 std::vector<Event> ef, eb;
                 ray_events(p, cm, r, n, ef, eb);
                 for (int d = 0; d < 2; d++) {
@@ -403,5 +405,103 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
     return pl;
 }
 
+namespace {
+// Invariants of a plan against the problem it was built from (host only).
+std::string check_plan(const mocgpu_problem &p, const SweepPlan &pl, int band_rays)
+{
+    const int width = (band_rays <= 0 || band_rays > kMaxRaysPerBand) ? kMaxRaysPerBand : band_rays;
+    auto fail = [](const std::string &m) { throw std::runtime_error("plan check: " + m); };
+    std::vector<int> seen(p.n_ray, 0);
+    int64_t n_seg = 0;
+    for (size_t bi = 0; bi < pl.bands.size(); bi++) {
+        const BandInfo &B = pl.bands[bi];
+        if (B.n_rays < 1 || B.n_rays > width) fail("band width out of range");
+        if (B.n_local < 1 || B.n_local > kMaxLocal) fail("band-local FSR count out of range");
+        const int nchunk = (B.nmax + kChunk - 1) / kChunk;
+        for (int j = 0; j < B.n_rays; j++) {
+            const int r = pl.ray_global[B.ray_base + j];
+            if (r < 0 || r >= p.n_ray) fail("ray index");
+            seen[r]++;
+            if (r < p.azi_ray_off[B.azi] || r >= p.azi_ray_off[B.azi + 1]) fail("ray outside its band's azimuth");
+            const RayInfo &ri = pl.rays[B.ray_base + j];
+            const int64_t s0 = p.ray_seg_off[r];
+            const int n = (int)(p.ray_seg_off[r + 1] - s0);
+            if (ri.nseg != n || n > B.nmax) fail("segment count");
+            if (ri.bc0 != p.ray_bc[2 * r] || ri.bc1 != p.ray_bc[2 * r + 1]) fail("boundary indices");
+            n_seg += n;
+            for (int d = 0; d < 2; d++)
+                for (int t = 0; t < nchunk * kChunk; t++) {
+                    const size_t e = (size_t)B.seg_base[d] + ((size_t)(t / kChunk) * B.n_rays + j) * kChunk + t % kChunk;
+                    if (e >= pl.seg_loc.size()) fail("stream index");
+                    const int loc = pl.seg_loc[e];
+                    const float len = pl.seg_len.empty() ? 0.0f : pl.seg_len[e];
+                    if (t < n) {
+                        const int k = d == 0 ? t : n - 1 - t;
+                        if (loc >= B.n_local || pl.loc_fsr[B.loc_base + loc] != p.seg_fsr[s0 + k])
+                            fail("segment FSR in the stream");
+                        if (len != (float)p.seg_len[s0 + k]) fail("segment length in the stream");
+                    } else if (loc != B.n_local + j || len != 0.0f) {
+                        fail("padding step");
+                    }
+                }
+            for (int d = 0; d < 2; d++) {
+                const int64_t gi = 2 * ((int64_t)B.ray_base + j) + d;
+                for (int q = pl.ev_off[gi]; q < pl.ev_off[gi + 1]; q++) {
+                    const uint32_t ev = pl.ev[q];
+                    if ((int)(ev & 0xFFFFu) > n) fail("event step");
+                    if ((int)((ev >> 16) & 0x7FFFu) >= B.n_lsurf) fail("event surface");
+                }
+                if (pl.ev_off[gi + 1] < pl.ev_off[gi]) fail("event offsets");
+            }
+        }
+    }
+    for (int r = 0; r < p.n_ray; r++)
+        if (seen[r] != 1) fail("every ray in exactly one band");
+    if (n_seg != p.n_seg) fail("segment total");
+    std::vector<int> slot_seen(pl.loc_fsr.size(), 0);
+    for (int f = 0; f < p.n_fsr; f++)
+        for (int s = pl.fsr_slot_ptr[f]; s < pl.fsr_slot_ptr[f + 1]; s++) {
+            const int slot = pl.fsr_slot_idx[s];
+            if (slot < 0 || slot >= (int)pl.loc_fsr.size() || pl.loc_fsr[slot] != f) fail("FSR slot list");
+            if (s > pl.fsr_slot_ptr[f] && slot <= pl.fsr_slot_idx[s - 1]) fail("FSR slots not increasing");
+            slot_seen[slot]++;
+        }
+    for (int v : slot_seen)
+        if (v != 1) fail("every slot listed once");
+    std::vector<int> o1(pl.bands.size(), 0), o2(pl.bands.size(), 0);
+    for (size_t k = 0; k < pl.band_order.size(); k++) {
+        o1[pl.band_order[k]]++;
+        if (k && pl.bands[pl.band_order[k]].nmax > pl.bands[pl.band_order[k - 1]].nmax) fail("band order");
+    }
+    for (size_t k = 0; k < pl.band_order_gs.size(); k++) {
+        const int b = pl.band_order_gs[k];
+        o2[b]++;
+        if ((pl.bands[b].azi >= p.n_azi / 2) != ((int)k >= pl.n_bands_oct1)) fail("Gauss-Seidel phase order");
+    }
+    for (size_t b = 0; b < pl.bands.size(); b++)
+        if (o1[b] != 1 || o2[b] != 1) fail("band orders are permutations");
+    return "ok: " + std::to_string(pl.bands.size()) + " bands, " + std::to_string(pl.loc_fsr.size()) +
+           " slots, " + std::to_string(pl.n_seg_padded) + " stream steps";
+}
+}  // namespace
+
 }  // namespace gpu
 }  // namespace mocc_amd
+
+extern "C" int mocgpu_plan_check(const mocgpu_problem *p, int32_t band_rays, char *msg, int32_t cap)
+{
+    std::string m;
+    int rc = 0;
+    try {
+        const mocc_amd::gpu::SweepPlan pl = mocc_amd::gpu::build_plan(*p, band_rays);
+        m = mocc_amd::gpu::check_plan(*p, pl, band_rays);
+    } catch (const std::exception &e) {
+        m = e.what();
+        rc = MOCGPU_ERR_ARG;
+    }
+    if (msg && cap > 0) {
+        std::strncpy(msg, m.c_str(), (size_t)cap - 1);
+        msg[cap - 1] = 0;
+    }
+    return rc;
+}

@@ -80,6 +80,7 @@ struct SweepPlan {
     std::vector<float> seg_len;             // interleaved length
     std::vector<int32_t> loc_fsr;           // [total local] global FSR
     std::vector<RayInfo> rays;              // [sum n_rays]
+    std::vector<int32_t> ray_global;        // [sum n_rays] problem ray of each band lane (host only)
     std::vector<int32_t> fsr_slot_ptr, fsr_slot_idx;
     std::vector<AziInfo> azi;
     std::vector<float> ang_rsin, ang_wt;    // [A*NP]: 1/sin(theta), w*spacing*hz*sin(theta)*pi

@@ -1,0 +1,37 @@
+"""The band plan (the sweep's HBM layout) against the problem it encodes,
+built on the host: every segment exactly once per direction (backward
+reversed), padding steps in the lane's own dummy slot, band-local FSR slots
+and their per-FSR lists, longest-first and Gauss-Seidel band orders, current
+events. CPU only."""
+import ctypes as C
+
+import pytest
+
+from mocc_amd.native import gpu
+
+
+def _check(hp, width):
+    buf = C.create_string_buffer(512)
+    rc = gpu().mocgpu_plan_check(hp.view, width, buf, 512)
+    return rc, buf.value.decode()
+
+
+@pytest.mark.parametrize("width", [64, 48, 17, 1])
+def test_plan_c1(c1, width):
+    rc, msg = _check(c1, width)
+    assert rc == 0, msg
+    assert msg.startswith("ok")
+
+
+def test_plan_vacuum_and_cmfd_geometry(cmfd_decks):
+    for name in ("asm17", "core51"):
+        rc, msg = _check(cmfd_decks[name], 64)
+        assert rc == 0, (name, msg)
+
+
+def test_plan_c2(deck_dir):
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    hp = HostProblem.load(lg.write_config("c2", deck_dir))
+    rc, msg = _check(hp, 64)
+    assert rc == 0, msg
