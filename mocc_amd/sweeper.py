@@ -266,15 +266,25 @@ class GpuBatch:
     def bytes_used(self) -> int:
         return int(gpu().mocgpu_batch_bytes_used(self.b))
 
-    def solve_all(self, chunk: int = 4, max_outer: int = 100000):
-        """Run until every member has finished; returns {id: EigenState}."""
+    def solve_all(self, chunk: int = 4, max_outer: int = 100000, compact: float = 0.25):
+        """Run until every member has finished; returns {id: EigenState}.
+        Finished problems are taken out of the batch (their handles keep the
+        results) once they are at least a fraction `compact` of the problems
+        still in it, so the launches shrink with the work."""
         done = {}
+        pending = []
         outer = 0
-        while len(done) < len(self.members) and outer < max_outer:
+        total = len(self.members)
+        while len(done) < total and outer < max_outer:
             self.run(chunk)
             outer += chunk
             for cid, st in self.poll():
                 done[cid] = st
+                pending.append(cid)
+            if pending and len(done) < total and len(pending) >= compact * self.size():
+                for cid in pending:
+                    gpu().mocgpu_batch_remove(self.b, cid)
+                pending = []
         return done
 
     def close(self):
