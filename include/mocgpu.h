@@ -332,13 +332,15 @@ int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h);
 int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n_sweeps, double *ms_per_sweep,
                        double *kernel_ms);
 
-/* The attenuation table the sweep kernel reads: for j = 0..n-1 the pair
- * (H_j, H_{j+1} - H_j) with H_j = 1 - exp(-j/1000), so that
- * 1 - exp(-tau) ~ H_j + (H_{j+1} - H_j) * (1000 tau - j). For tau <= 10 this
- * is Exponential_Linear<10000> on [-10, 0] (src/core/exponential.hpp:56-112)
- * read from the other end. Fills up to cap pairs (2 * cap floats) and
- * returns the number of pairs n. Needs no GPU. */
-int32_t mocgpu_exp_table(float *out, int32_t cap);
+/* The sweep kernel's segment attenuation 1 - exp(-tau), evaluated on the
+ * current device for n host values tau[i] >= 0 into out[i]: the
+ * piecewise-linear interpolant of Exponential_Linear<10000> on [-10, 0]
+ * (src/core/exponential.hpp:56-112) in terms of w = 1000 tau, with nodes
+ * H_j = 1 - exp(-j/1000) and slopes computed in registers, continued past
+ * tau = 10 (where the reference calls exp) with the same nodes. Replaces the
+ * table lookup of moc_sweeper_kernel.inc.hpp:94-99 for testing. Returns 0 or
+ * MOCGPU_ERR_DEVICE. */
+int32_t mocgpu_attenuation(const float *tau, float *out, int32_t n);
 
 #ifdef __cplusplus
 }

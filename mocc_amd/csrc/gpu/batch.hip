@@ -5,12 +5,12 @@
 // every unfinished problem is a fixed sequence of launches, each covering all
 // problems at once:
 //
-//   fission source -> phi_old -> group sources (Jacobi in energy) ->
+//   k_b_pre: fission source, phi_old and the group sources (Jacobi in energy)
 //   n_inner x [ work-queue sweep of every (problem, band, group) item (two
 //               queues for Gauss-Seidel problems: octant-1 bands, then
-//               octant-2 bands) -> flux reduction -> boundary parity flip ] ->
-//   per-problem reductions (total fission + k, normalization, err_psi) ->
-//   per-problem convergence test.
+//               octant-2 bands) -> flux reduction + boundary parity flip ]
+//   k_b_post: total fission and k, normalization, err_psi and the
+//             convergence test, one workgroup per problem.
 //
 // The sequence is captured once per batch composition into a HIP graph and
 // replayed; problems that have converged stay in the graph and are skipped by
@@ -36,145 +36,124 @@ struct CaseDesc {
     const double *mat_nf, *mat_ch, *mat_sc, *vol;
     double *phi, *phi_old, *src, *fs, *fs_prev;
     double *scal;         // [S_COUNT]
-    uint64_t *parity;     // boundary buffer parity word
-    uint64_t gmask;       // bits of the groups
 };
 
 constexpr int kBT = 256;  // threads per block of the element-wise launches
 
 __device__ __forceinline__ bool finished(const int32_t *status, int c) { return status[2 * c + 1] != 0; }
 
-// TransportSweeper::calc_fission_source (transport_sweeper.cpp:119-134), fs
-// of every unfinished problem; copy_prev also stores fs_prev = fs.
-__global__ void k_b_fission(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
-                            const int32_t *__restrict__ status, int copy_prev)
+// The start of an outer iteration for one FSR of every unfinished problem:
+// fission source fs = (1/k) sum_g nuSf phi and fs_prev = fs
+// (transport_sweeper.cpp:119-134), phi_old = phi, and every group's source
+// S = chi fs + sum_{g' != g} Ss[g <- g'] phi_old (source.cpp:41-112).
+// Block 0 also zeroes the work-queue counters of the iteration's sweeps.
+__global__ void k_b_pre(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
+                        const int32_t *__restrict__ status, unsigned int *__restrict__ counters, int n_counters)
 {
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < n_counters; i += blockDim.x) counters[i] = 0u;
     const BlockMap b = bm[blockIdx.x];
     const int c = __builtin_amdgcn_readfirstlane(b.c);
     if (finished(status, c)) return;
     const CaseDesc &d = cs[c];
     const int f = b.first + threadIdx.x;
     if (f >= d.F) return;
-    const double rk = 1.0 / d.scal[S_K];
-    const int m = d.fsr_mat[f];
-    double s = 0.0;
-    for (int g = 0; g < d.G; g++) s += rk * d.mat_nf[m * d.G + g] * d.phi[(size_t)f * d.G + g];
-    d.fs[f] = s;
-    if (copy_prev) d.fs_prev[f] = s;
-}
-
-__global__ void k_b_copy_phi(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
-                             const int32_t *__restrict__ status)
-{
-    const BlockMap b = bm[blockIdx.x];
-    const int c = __builtin_amdgcn_readfirstlane(b.c);
-    if (finished(status, c)) return;
-    const CaseDesc &d = cs[c];
-    const int i = b.first + threadIdx.x;
-    if (i < d.F * d.G) d.phi_old[i] = d.phi[i];
-}
-
-// Source::initialize_group + fission + in_scatter (source.cpp:41-112), all
-// groups from the previous outer iteration's flux.
-__global__ void k_b_source(const CaseDesc *__restrict__ cs, const BlockMap *__restrict__ bm,
-                           const int32_t *__restrict__ status)
-{
-    const BlockMap b = bm[blockIdx.x];
-    const int c = __builtin_amdgcn_readfirstlane(b.c);
-    if (finished(status, c)) return;
-    const CaseDesc &d = cs[c];
-    const int i = b.first + threadIdx.x;
     const int G = d.G;
-    if (i >= d.F * G) return;
-    const int f = i / G, g = i % G;
     const int m = d.fsr_mat[f];
-    double s = d.mat_ch[m * G + g] * d.fs[f];
-    const double *row = d.mat_sc + ((size_t)m * G + g) * G;
-    for (int gg = 0; gg < G; gg++) {
-        if (gg == g) continue;
-        s += row[gg] * d.phi_old[(size_t)f * G + gg];
+    const double rk = 1.0 / d.scal[S_K];
+    const double *phi = d.phi + (size_t)f * G;
+    double *phi_old = d.phi_old + (size_t)f * G;
+    double fs = 0.0;
+    for (int g = 0; g < G; g++) {
+        const double v = phi[g];
+        fs += rk * d.mat_nf[m * G + g] * v;
+        phi_old[g] = v;
     }
-    d.src[i] = s;
+    d.fs[f] = fs;
+    d.fs_prev[f] = fs;
+    for (int g = 0; g < G; g++) {
+        double s = d.mat_ch[m * G + g] * fs;
+        const double *row = d.mat_sc + ((size_t)m * G + g) * G;
+        for (int gg = 0; gg < G; gg++) {
+            if (gg == g) continue;
+            s += row[gg] * phi_old[gg];
+        }
+        d.src[(size_t)f * G + g] = s;
+    }
 }
 
-__global__ void k_b_flip(const CaseDesc *__restrict__ cs, const int32_t *__restrict__ status,
-                         const int32_t *__restrict__ n_inner, int n_cases, int inner)
+__device__ __forceinline__ void block_sum_n(double *acc, int n, double (*sh)[4])
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cases || finished(status, c) || n_inner[c] <= inner) return;
-    *cs[c].parity ^= cs[c].gmask;
+    const int t = threadIdx.x;
+    __syncthreads();
+    for (int q = 0; q < n; q++) sh[t][q] = acc[q];
+    __syncthreads();
+    for (int w = kBT / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < n; q++) sh[t][q] += sh[t + w][q];
+        __syncthreads();
+    }
+    for (int q = 0; q < n; q++) acc[q] = sh[0][q];
 }
 
-__global__ void k_b_reset(unsigned int *counter)
-{
-    if (threadIdx.x == 0) *counter = 0u;
-}
-
-// The per-problem reductions of EigenSolver::step, one workgroup per problem
-// with a fixed tree (modes as the single-problem path: total fission and the
-// k update, Normalize, err_psi).
-__global__ __launch_bounds__(kBT) void k_b_reduce(const CaseDesc *__restrict__ cs,
-                                                  const int32_t *__restrict__ status, int mode)
+// The end of an outer iteration, one workgroup per problem (fixed reduction
+// trees): total fission and k <- k F_new / F_old (eigen_solver.cpp:242-247),
+// the new fission source, Normalize of fs and fs_prev (util/utils.hpp:28-51),
+// err_psi (eigen_solver.cpp:171-184) and EigenSolver::solve's test:
+// status[2c] counts steps, status[2c+1] becomes 1 (converged), 2 (max_iter)
+// or 3 (k not a number).
+__global__ __launch_bounds__(kBT) void k_b_post(const CaseDesc *__restrict__ cs, int32_t *__restrict__ status)
 {
     const int c = blockIdx.x;
     if (finished(status, c)) return;
     const CaseDesc &d = cs[c];
     __shared__ double sh[kBT][4];
-    double acc[4] = {0, 0, 0, 0};
     const int G = d.G;
     double *s = d.scal;
+    double acc[4] = {0, 0, 0, 0};
     for (int f = threadIdx.x; f < d.F; f += kBT) {
-        if (mode == RED_FISSION) {
-            const int m = d.fsr_mat[f];
-            for (int g = 0; g < G; g++) {
-                acc[0] += d.phi[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
-                acc[1] += d.phi_old[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
-            }
-        } else if (mode == RED_NORM) {
-            acc[0] += d.fs[f] > 0.0 ? 1.0 : 0.0;
-            acc[1] += d.fs[f];
-            acc[2] += d.fs_prev[f] > 0.0 ? 1.0 : 0.0;
-            acc[3] += d.fs_prev[f];
-        } else {
-            const double e = d.fs[f] * s[S_FA] - d.fs_prev[f] * s[S_FB];
-            acc[0] += e * e;
+        const int m = d.fsr_mat[f];
+        for (int g = 0; g < G; g++) {
+            acc[0] += d.phi[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
+            acc[1] += d.phi_old[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
         }
     }
-    const int t = threadIdx.x;
-    for (int q = 0; q < 4; q++) sh[t][q] = acc[q];
-    __syncthreads();
-    for (int w = kBT / 2; w > 0; w >>= 1) {
-        if (t < w)
-            for (int q = 0; q < 4; q++) sh[t][q] += sh[t + w][q];
-        __syncthreads();
+    block_sum_n(acc, 2, sh);
+    const double k_prev = s[S_K];
+    const double k = k_prev * acc[0] / acc[1];
+    const double rk = 1.0 / k;
+    double nrm[4] = {0, 0, 0, 0};
+    for (int f = threadIdx.x; f < d.F; f += kBT) {
+        const int m = d.fsr_mat[f];
+        double v = 0.0;
+        for (int g = 0; g < G; g++) v += rk * d.mat_nf[m * G + g] * d.phi[(size_t)f * G + g];
+        d.fs[f] = v;
+        const double vp = d.fs_prev[f];
+        nrm[0] += v > 0.0 ? 1.0 : 0.0;
+        nrm[1] += v;
+        nrm[2] += vp > 0.0 ? 1.0 : 0.0;
+        nrm[3] += vp;
     }
-    if (t != 0) return;
-    if (mode == RED_FISSION) {
-        s[S_TF_NEW] = sh[0][0];
-        s[S_TF_OLD] = sh[0][1];
-        s[S_KPREV] = s[S_K];
-        s[S_K] = s[S_K] * sh[0][0] / sh[0][1];
-        s[S_ERRK] = fabs(s[S_K] - s[S_KPREV]);
-    } else if (mode == RED_NORM) {
-        s[S_FA] = sh[0][0] / sh[0][1];
-        s[S_FB] = sh[0][2] / sh[0][3];
-    } else {
-        s[S_ERRPSI] = sqrt(sh[0][0] / s[S_NFISS]);
+    block_sum_n(nrm, 4, sh);
+    const double fa = nrm[0] / nrm[1], fb = nrm[2] / nrm[3];
+    double e2[1] = {0.0};
+    for (int f = threadIdx.x; f < d.F; f += kBT) {
+        const double e = d.fs[f] * fa - d.fs_prev[f] * fb;
+        e2[0] += e * e;
     }
-}
-
-// EigenSolver::solve's test after each step (eigen_solver.cpp:162-217):
-// status[2c] counts steps, status[2c+1] becomes 1 (converged), 2 (max_iter)
-// or 3 (k not a number).
-__global__ void k_b_check(const CaseDesc *__restrict__ cs, int32_t *__restrict__ status, int n_cases)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cases || finished(status, c)) return;
-    const CaseDesc &d = cs[c];
+    block_sum_n(e2, 1, sh);
+    if (threadIdx.x != 0) return;
+    s[S_TF_NEW] = acc[0];
+    s[S_TF_OLD] = acc[1];
+    s[S_KPREV] = k_prev;
+    s[S_K] = k;
+    s[S_ERRK] = fabs(k - k_prev);
+    s[S_FA] = fa;
+    s[S_FB] = fb;
+    s[S_ERRPSI] = sqrt(e2[0] / s[S_NFISS]);
     const int it = ++status[2 * c];
-    const double *s = d.scal;
     int fin = 0;
-    if (!(s[S_K] == s[S_K])) fin = 3;
+    if (!(k == k)) fin = 3;
     else if (s[S_ERRK] < d.k_tol && s[S_ERRPSI] < d.psi_tol && it - 1 >= d.min_iter) fin = 1;
     else if (it >= d.max_iter) fin = 2;
     status[2 * c + 1] = fin;
@@ -213,10 +192,10 @@ struct mocgpu_batch {
     DBuf<SweepArgs> d_sweep;
     DBuf<ReduceArgs> d_red;
     DBuf<int4> d_items;
-    DBuf<BlockMap> d_bmF, d_bmFG, d_bmR;
+    DBuf<BlockMap> d_bmF, d_bmR;
     DBuf<int32_t> d_status, d_ninner;
     DBuf<uint32_t> d_counter;
-    int n_bmF = 0, n_bmFG = 0, n_bmR = 0, n_items = 0, max_inner = 1, q_blocks = 1;
+    int n_bmF = 0, n_bmR = 0, n_items = 0, max_inner = 1, q_blocks = 1;
     int phase_first[2] = {0, 0}, phase_items[2] = {0, 0};
     size_t lds = 0;
     hipGraph_t graph = nullptr;
@@ -257,7 +236,7 @@ struct mocgpu_batch {
         std::vector<SweepArgs> sw(n);
         std::vector<ReduceArgs> rd(n);
         std::vector<int32_t> st(2 * n), ni(n);
-        std::vector<size_t> cF(n), cFG(n), cR(n);
+        std::vector<size_t> cF(n), cR(n);
         struct It {
             int64_t cost;
             int4 w;
@@ -285,21 +264,20 @@ struct mocgpu_batch {
             d.fs = h->d_fs.p;
             d.fs_prev = h->d_fs_prev.p;
             d.scal = h->d_scal.p;
-            d.parity = h->d_parity.p;
-            // Gauss-Seidel problems keep one boundary buffer: nothing to flip
-            d.gmask = h->opt.gs_boundary ? 0ull : (h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull));
             SweepArgs a = h->sweep_args(-1);
             a.parity_dev = h->d_parity.p;
             a.lds_lsurf = 0;
             sw[i] = a;
             lds = std::max(lds, sweep_lds_bytes(a));
             rd[i] = h->reduce_args(-1);
+            // Gauss-Seidel problems keep one boundary buffer: nothing to flip
+            rd[i].parity = h->d_parity.p;
+            rd[i].gmask = h->opt.gs_boundary ? 0ull : (h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull));
             st[2 * i] = cases[i].status[0];
             st[2 * i + 1] = cases[i].status[1];
             ni[i] = std::max(1, h->opt.n_inner);
             max_inner = std::max(max_inner, ni[i]);
             cF[i] = (size_t)h->F;
-            cFG[i] = (size_t)h->F * h->G;
             cR[i] = (size_t)h->F * h->G * reduce_flux_lanes();
             // phase 0: every band (Jacobi) or the octant-1 bands (Gauss-Seidel);
             // phase 1: the octant-2 bands of Gauss-Seidel problems
@@ -321,20 +299,18 @@ struct mocgpu_batch {
             phase_items[ph] = (int)items[ph].size();
         }
         n_items = (int)iv.size();
-        auto bF = block_map(cF), bFG = block_map(cFG), bR = block_map(cR);
+        auto bF = block_map(cF), bR = block_map(cR);
         n_bmF = (int)bF.size();
-        n_bmFG = (int)bFG.size();
         n_bmR = (int)bR.size();
         d_desc.upload(desc, stream);
         d_sweep.upload(sw, stream);
         d_red.upload(rd, stream);
         d_items.upload(iv, stream);
         d_bmF.upload(bF, stream);
-        d_bmFG.upload(bFG, stream);
         d_bmR.upload(bR, stream);
         d_status.upload(st, stream);
         d_ninner.upload(ni, stream);
-        if (!d_counter.p) d_counter.alloc(1);
+        d_counter.alloc(2 * (size_t)max_inner);  // one work-queue counter per (inner, phase)
         int dev = 0, ncu = 0;
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -348,32 +324,24 @@ struct mocgpu_batch {
     {
         const int n = (int)cases.size();
         hipStream_t s = stream;
-        hipLaunchKernelGGL(k_b_fission, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, 1);
-        hipLaunchKernelGGL(k_b_copy_phi, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
-        hipLaunchKernelGGL(k_b_source, dim3(n_bmFG), dim3(kBT), 0, s, d_desc.p, d_bmFG.p, d_status.p);
+        hipLaunchKernelGGL(k_b_pre, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, d_counter.p,
+                           2 * max_inner);
         for (int inner = 0; inner < max_inner; inner++) {
             for (int ph = 0; ph < 2; ph++) {
                 if (phase_items[ph] == 0) continue;
-                hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, d_counter.p);
                 QueueArgs q{};
                 q.cases = d_sweep.p;
                 q.items = d_items.p + phase_first[ph];
                 q.n_items = phase_items[ph];
                 q.inner = inner;
-                q.counter = d_counter.p;
+                q.counter = d_counter.p + 2 * inner + ph;
                 q.status = d_status.p;
                 q.n_inner = d_ninner.p;
                 launch_sweep_queue(q, NP, lds, std::min(q_blocks, phase_items[ph]), s);
             }
             launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
-            hipLaunchKernelGGL(k_b_flip, dim3(nblk(n, 64)), dim3(64), 0, s, d_desc.p, d_status.p, d_ninner.p, n,
-                               inner);
         }
-        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_FISSION);
-        hipLaunchKernelGGL(k_b_fission, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, 0);
-        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_NORM);
-        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p, (int)RED_PSIERR);
-        hipLaunchKernelGGL(k_b_check, dim3(nblk(n, 64)), dim3(64), 0, s, d_desc.p, d_status.p, n);
+        hipLaunchKernelGGL(k_b_post, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p);
     }
 
     void run(int n_outer)

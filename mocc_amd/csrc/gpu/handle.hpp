@@ -107,7 +107,7 @@ struct mocgpu_handle {
     DBuf<BandInfo> d_bands;
     DBuf<int32_t> d_order, d_order_gs, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
     DBuf<uint16_t> d_seg_loc;
-    DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw, d_exptab;
+    DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw;
     DBuf<RayInfo> d_rays;
     DBuf<AziInfo> d_azi;
     DBuf<uint32_t> d_ev;
@@ -161,7 +161,6 @@ struct mocgpu_handle {
         a.ang_wt = d_ang_wt.p;
         a.ang_cw = d_ang_cw.p;
         a.ang_fw = d_ang_fw.p;
-        a.exptab = (const float2 *)d_exptab.p;
         a.fsr_mat = d_fsr_mat.p;
         a.mat_tr = d_mat_tr.p;
         a.mat_ss = d_mat_ss.p;

@@ -34,7 +34,6 @@ struct SweepArgs {
     const RayInfo *rays;
     const AziInfo *azi;
     const float *ang_rsin, *ang_wt, *ang_cw, *ang_fw;
-    const float2 *exptab;
     const int32_t *fsr_mat;
     const double *mat_tr, *mat_ss;
     const double *src, *phi;
@@ -53,6 +52,8 @@ struct ReduceArgs {
     const double *mat_tr, *mat_ss, *vol;
     const double *src;
     double *phi;
+    uint64_t *parity;       // batch: boundary parity word, xor-ed with gmask after the reduction
+    uint64_t gmask;
 };
 
 // One persistent launch over a work queue of (case, band, group) items of many
@@ -78,6 +79,8 @@ size_t sweep_lds_bytes(const SweepArgs &a);
 // (band, group) grid of all groups.
 int sweep_grid_items(int n_bands, int G, bool all_groups);
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s);
+// out[i] = the sweep kernel's 1 - exp(-tau[i]) for n values.
+void launch_attenuation(const float *tau, float *out, int n, hipStream_t s);
 
 // Block -> (problem, first element) map of a launch over many problems; each
 // problem's elements are covered by whole 256-thread blocks.

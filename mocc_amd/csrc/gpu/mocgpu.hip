@@ -391,7 +391,6 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
             h->d_surf_ptr = like->d_surf_ptr;
             h->d_surf_idx = like->d_surf_idx;
             h->d_surf_area = like->d_surf_area;
-            h->d_exptab = like->d_exptab;
             h->h_fsr_mat = like->h_fsr_mat;
             h->d_fsr_mat = like->d_fsr_mat;
             h->d_vol = like->d_vol;
@@ -420,9 +419,6 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         h->d_surf_ptr.upload(pl.surf_slot_ptr, s);
         h->d_surf_idx.upload(pl.surf_slot_idx, s);
         h->d_surf_area.upload(pl.surf_area, s);
-        std::vector<float> tab;
-        exp_table_fill(tab);
-        h->d_exptab.upload(tab, s);
         h->h_fsr_mat.assign(p->fsr_mat, p->fsr_mat + F);
         h->d_fsr_mat.upload(h->h_fsr_mat, s);
         h->d_vol.upload(std::vector<double>(p->fsr_vol, p->fsr_vol + F), s);
@@ -617,7 +613,7 @@ int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
                      h->d_loc_fsr.bytes() + h->d_rays.bytes() + h->d_azi.bytes() + h->d_ang_rsin.bytes() +
                      h->d_ang_wt.bytes() + h->d_ang_cw.bytes() + h->d_ang_fw.bytes() + h->d_slot_ptr.bytes() +
                      h->d_slot_idx.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +
-                     h->d_surf_idx.bytes() + h->d_surf_area.bytes() + h->d_exptab.bytes() + h->d_fsr_mat.bytes() +
+                     h->d_surf_idx.bytes() + h->d_surf_area.bytes() + h->d_fsr_mat.bytes() +
                      h->d_vol.bytes() + h->d_area.bytes() + h->d_pin_off.bytes() + h->d_pin_cell.bytes() +
                      h->d_pin_area.bytes();
     return (int64_t)b;
@@ -869,13 +865,20 @@ int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current, cons
     });
 }
 
-int32_t mocgpu_exp_table(float *out, int32_t cap)
+int32_t mocgpu_attenuation(const float *tau, float *out, int32_t n)
 {
-    std::vector<float> t;
-    mocc_amd::gpu::exp_table_fill(t);
-    const int32_t n = (int32_t)(t.size() / 2);
-    if (out && cap > 0) std::memcpy(out, t.data(), sizeof(float) * 2 * (size_t)std::min(cap, n));
-    return n;
+    if (n <= 0) return 0;
+    if (!tau || !out) return MOCGPU_ERR_ARG;
+    float *d = nullptr;
+    if (hipMalloc(&d, 2 * sizeof(float) * (size_t)n) != hipSuccess) return MOCGPU_ERR_DEVICE;
+    hipError_t e = hipMemcpy(d, tau, sizeof(float) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        mocc_amd::gpu::launch_attenuation(d, d + n, n, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d + n, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? 0 : MOCGPU_ERR_DEVICE;
 }
 
 int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return 2LL * h->NP * h->n_seg * h->G; }

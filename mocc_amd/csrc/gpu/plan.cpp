@@ -88,29 +88,6 @@ void ray_events(const mocgpu_problem &p, const Coarse &cm, int r, int nseg, std:
 }
 }  // namespace
 
-// Exponential_Linear<10000> (exponential.hpp:56-112) interpolates exp on
-// the nodes x_i = -10 + i/1000 and the sweep uses e = 1 - exp(-tau). Writing
-// the same piecewise-linear interpolant in terms of w = 1000*tau with nodes
-// H_j = 1 - exp(x_{N-j}) gives e = H_j + (H_{j+1} - H_j) * (w - j), j = floor(w),
-// which FP32 evaluates without the cancellation of 1 - (value near 1).
-// Past tau = 10 the reference calls std::exp; the table continues there with
-// nodes of 1 - exp(-tau) itself (interpolation error < 6e-12) up to tau = 40,
-// beyond which 1 - exp(-tau) is 1 in FP32, so lookups never branch.
-void exp_table_fill(std::vector<float> &t)
-{
-    const int N = kExpN;
-    const double mn = -10.0, mx = 0.0;
-    const double space = (mx - mn) / (double)N;
-    std::vector<double> H(kExpTable + 1);
-    for (int j = 0; j <= kExpTable; j++)
-        H[j] = (j <= N) ? 1.0 - std::exp(mn + (N - j) * space) : 1.0 - std::exp(-(double)j * space);
-    t.assign(2 * kExpTable, 0.0f);
-    for (int j = 0; j < kExpTable; j++) {
-        t[2 * j] = (float)H[j];
-        t[2 * j + 1] = (float)(H[j + 1] - H[j]);
-    }
-}
-
 SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
 {
     if (max_rays <= 0 || max_rays > kMaxRaysPerBand) max_rays = kMaxRaysPerBand;

@@ -30,8 +30,6 @@ constexpr int kMaxLocal = 2047;  // band-local FSRs held in LDS
 constexpr int kMaxRaysPerBand = 64;
 constexpr int kDummySlots = kMaxRaysPerBand;  // padding slots after a band's n_local FSRs
 constexpr int kChunk = 8;         // segments per lane per vector load (chunked layout)
-constexpr int kExpN = 10000;      // Exponential_Linear<10000> on [-10, 0]
-constexpr int kExpTable = 40000;  // table nodes: the reference's 10000 + 1 - exp(-tau) to tau = 40
 constexpr int kMaxLSurf = 1024;   // band-local coarse surfaces held in LDS
 
 struct BandInfo {
@@ -99,10 +97,6 @@ struct SweepPlan {
 // Build the plan; throws std::runtime_error on inputs the kernels cannot take.
 SweepPlan build_plan(const mocgpu_problem &p, int max_rays_per_band);
 
-// 1 - exp(-j * dx) for j = 0..kExpN and the slopes between nodes: the
-// reference's Exponential_Linear<10000> interpolant of exp on [-10, 0]
-// rewritten in terms of tau (see exp_table_fill in plan.cpp).
-void exp_table_fill(std::vector<float> &h_and_slope);
 
 }  // namespace gpu
 }  // namespace mocc_amd

@@ -10,8 +10,9 @@
 //     (source_isotropic.cpp:21-57) are staged in LDS in FP32;
 //   * segments stream in chunks of kChunk steps per lane (three 16-byte loads),
 //     the next chunk in flight while the current one is swept; within a chunk
-//     all exponentials (independent of psi) are gathered before the psi
-//     recurrence runs;
+//     all attenuations (independent of psi) are evaluated in registers, from
+//     the reference's interpolated exponential, before the psi recurrence
+//     runs;
 //   * every tally contribution sum_p w_p * dpsi_p is rounded to a 64-bit
 //     fixed-point integer and added into a band-local LDS tally with integer
 //     atomics. Integer addition is associative, so the band's sums do not
@@ -30,9 +31,35 @@ namespace {
 constexpr double FPI = 4.0 * 3.1415926535897932;
 constexpr int kNumXcd = 8;  // MI355X: workgroup i is dispatched to XCD i % 8
 
-// Largest interpolation coordinate 1000 tau: the last table node. Beyond it
+// Largest interpolation coordinate 1000 tau: the last node used. Beyond it
 // 1 - exp(-tau) rounds to 1 in FP32.
-constexpr float kWMax = (float)(kExpTable - 1);
+constexpr float kWMax = 39999.0f;
+
+// 1 - exp(-tau) as Exponential_Linear<10000> on [-10, 0] evaluates exp(-tau)
+// (src/core/exponential.hpp:56-112), from the coordinate w = 1000 tau: with
+// j = floor(w), f = w - j and x = j / 1000 the interpolant is
+//   H_j + f (H_{j+1} - H_j),  H_j = 1 - e^{-x},  H_{j+1} - H_j = e^{-x} c,
+// c = 1 - e^{-1/1000}. e^{-x} is the hardware exp2; H_j is 1 - e^{-x} for
+// x >= 1/4 and the series of 1 - e^{-x} below (no cancellation near 1).
+// Past tau = 10 the reference calls exp itself; the same interpolant
+// continues there (error < 6e-12) up to w = kWMax. Relative error against the
+// FP64 interpolant: below 3e-7 for every tau.
+__device__ __forceinline__ float attenuation(float w)
+{
+    w = fminf(w, kWMax);
+    const float f = __builtin_amdgcn_fractf(w);
+    const float x = (w - f) * 1.0e-3f;
+    const float e = __builtin_amdgcn_exp2f(x * -1.4426950408889634f);
+    float s = 1.0f / 5040.0f;
+    s = fmaf(s, x, -1.0f / 720.0f);
+    s = fmaf(s, x, 1.0f / 120.0f);
+    s = fmaf(s, x, -1.0f / 24.0f);
+    s = fmaf(s, x, 1.0f / 6.0f);
+    s = fmaf(s, x, -0.5f);
+    s = fmaf(s, x, 1.0f);
+    const float h = x < 0.25f ? x * s : 1.0f - e;
+    return fmaf(e * 9.995001666250084e-4f, f, h);
+}
 
 // Power-of-two scale 2^k with bound < 2^(62 - k), so |sum| * 2^k < 2^62.
 __device__ __forceinline__ int fixed_exp(double bound)
@@ -123,7 +150,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const float *psi_in = A.psi + gbase + (size_t)par * A.bcpg;
     float *psi_out = A.psi + gbase + (size_t)(A.gs ? par : 1 - par) * A.bcpg;
 
-    // rk[p] = 1000 / sin(theta_p): tau * 1000 is the table coordinate
+    // rk[p] = 1000 / sin(theta_p): tau * 1000 is the interpolation coordinate
     float psi[NP], rk[NP], wt[NP];
     float cw[NP][2], fw[NP][2];
     RayInfo r{0, 0, 0, 0};
@@ -168,8 +195,6 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         }
     }
     const float csign = dir ? -1.0f : 1.0f;
-    // the table is addressed by 32-bit byte offsets from a uniform base
-    const char *__restrict__ tab = (const char *)A.exptab;
     const int n = r.nseg;
     const int nr = B.n_rays;
 
@@ -217,23 +242,16 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         len[5] = cb.y;
         len[6] = cb.z;
         len[7] = cb.w;
-        // Phase 1: every table gather of the chunk is issued before any is
-        // used (the exponentials do not depend on psi), then the next chunk's
-        // segment data, so waiting on the gathers never waits on it.
-        float q[kChunk], fr[kChunk][NP];
-        float2 hs[kChunk][NP];
+        // Phase 1: the chunk's attenuations (they do not depend on psi),
+        // then the next chunk's segment loads are issued.
+        float q[kChunk], at[kChunk][NP];
 #pragma unroll
         for (int u = 0; u < kChunk; u++) {
             const float2 e = sq[loc[u]];
             q[u] = e.y;
             const float tau0 = e.x * len[u];
 #pragma unroll
-            for (int p = 0; p < NP; p++) {
-                const float w = fminf(tau0 * rk[p], kWMax);
-                const unsigned int jj = (unsigned int)w;
-                fr[u][p] = __builtin_amdgcn_fractf(w);
-                hs[u][p] = *(const float2 *)(tab + (jj << 3));
-            }
+            for (int p = 0; p < NP; p++) at[u][p] = attenuation(tau0 * rk[p]);
         }
         if (c + 1 < nchunk) {
             const size_t ci = (size_t)(c + 1) * nr + j;
@@ -255,8 +273,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
             float cacc = 0.0f;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
-                const float ex = fmaf(hs[u][p].y, fr[u][p], hs[u][p].x);
-                const float d = (psi[p] - q[u]) * ex;
+                const float d = (psi[p] - q[u]) * at[u][p];
                 psi[p] = psi[p] - d;
                 cacc = fmaf(d, wt[p], cacc);
             }
@@ -381,7 +398,9 @@ __global__ void k_reduce_flux(ReduceArgs A)
 }
 
 // The same over many problems: block b covers elements [first, first + 256)
-// of problem bm[b].c (kRedLanes elements per (FSR, group) item).
+// of problem bm[b].c (kRedLanes elements per (FSR, group) item). The first
+// block of a problem also flips its boundary buffers (A.gmask; 0 for the
+// Gauss-Seidel update); the next sweep reads the parity word in a later launch.
 __global__ void k_reduce_flux_batch(const ReduceArgs *__restrict__ cases, const BlockMap *__restrict__ bm,
                                     const int32_t *__restrict__ status, const int32_t *__restrict__ n_inner,
                                     int inner)
@@ -390,11 +409,24 @@ __global__ void k_reduce_flux_batch(const ReduceArgs *__restrict__ cases, const 
     const int c = __builtin_amdgcn_readfirstlane(b.c);
     if (status[2 * c + 1] != 0 || n_inner[c] <= inner) return;
     const ReduceArgs A = cases[c];
-    const int gid = __builtin_amdgcn_readfirstlane(b.first) + threadIdx.x;
+    const int first = __builtin_amdgcn_readfirstlane(b.first);
+    if (first == 0 && threadIdx.x == 0 && A.gmask) *A.parity ^= A.gmask;
+    const int gid = first + threadIdx.x;
     reduce_flux_item(A, gid / kRedLanes, gid % kRedLanes);
 }
 
+__global__ void k_attenuation(const float *__restrict__ tau, float *__restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = attenuation(tau[i] * 1000.0f);
+}
+
 }  // namespace
+
+void launch_attenuation(const float *tau, float *out, int n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_attenuation, dim3((n + 255) / 256), dim3(256), 0, s, tau, out, n);
+}
 
 size_t sweep_lds_bytes(const SweepArgs &a)
 {

@@ -144,8 +144,8 @@ def gpu() -> C.CDLL:
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_plan_check.argtypes = [C.POINTER(MocgpuProblem), i32, C.c_char_p, i32]
-        L.mocgpu_exp_table.restype = i32
-        L.mocgpu_exp_table.argtypes = [P, i32]
+        L.mocgpu_attenuation.restype = i32
+        L.mocgpu_attenuation.argtypes = [P, P, i32]
         L.mocgpu_create_like.argtypes = [P, C.POINTER(MocgpuProblem), C.POINTER(MocgpuOptions), C.POINTER(P)]
         L.mocgpu_state_bytes.restype = i64
         L.mocgpu_state_bytes.argtypes = [P]

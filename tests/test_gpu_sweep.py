@@ -39,6 +39,34 @@ def _rel(a, b):
     return np.max(np.abs(a[m] / b[m] - 1.0))
 
 
+def test_attenuation_matches_reference_interpolant():
+    """The kernel's in-register 1 - exp(-tau) against the FP64 oracle of
+    Exponential_Linear<10000> (exponential.hpp:56-112; exp itself past
+    tau = 10), at the coordinate w = fl(1000 tau) the kernel interpolates at."""
+    import ctypes as C
+    import math
+    from mocc_amd.native import gpu
+    from oracle.binding import exp_table
+    rng = np.random.default_rng(7)
+    tau = np.concatenate([
+        rng.uniform(0, 10, 20000), rng.uniform(0, 0.01, 4000), rng.uniform(0, 1e-5, 500),
+        rng.uniform(10, 45, 1000), np.arange(0, 12000) / 1000.0,  # the nodes
+        [0.0, 1e-30, 0.2499, 0.25, 0.2501, 9.999, 10.0, 10.001, 39.999, 40.0, 1e3, 1e30],
+    ]).astype(np.float32)
+    out = np.zeros_like(tau)
+    assert gpu().mocgpu_attenuation(tau.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+                                    tau.size) == 0
+    w = np.minimum(tau * np.float32(1000.0), np.float32(39999.0)).astype(np.float64)
+    t = w / 1000.0
+    ref = np.array([1.0 - exp_table(-v) if v < 10.0 else 1.0 - math.exp(-v) for v in t])
+    e = out.astype(np.float64)
+    assert e[tau == 0].max() == 0.0
+    assert np.abs(e - ref).max() < 1.5e-7
+    m = ref > 1e-9  # below, the FP64 interpolant's own cancellation (~1e-16) is not small
+    assert np.max(np.abs(e[m] / ref[m] - 1.0)) < 3e-7
+    assert e[-1] == 1.0 and e[-2] == 1.0
+
+
 @pytest.mark.parametrize("gs", [True, False])
 @pytest.mark.parametrize("energy_mode", [1, 0])
 def test_c1_fss_step_parity(c1, energy_mode, gs):

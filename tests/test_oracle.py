@@ -55,30 +55,6 @@ def test_exp_table_10000():
     assert err <= 1.25e-7 * (1 + 1e-9)
 
 
-def test_device_exp_table():
-    """The table the sweep kernel reads (FP32 pairs H_j, H_{j+1} - H_j in
-    tau = j / 1000) reproduces the reference table read from the other end."""
-    n = gpu().mocgpu_exp_table(None, 0)
-    assert n == 40000
-    t = np.zeros(2 * n, np.float32)
-    assert gpu().mocgpu_exp_table(t.ctypes.data_as(C.c_void_p), n) == n
-    H, S = t[0::2].astype(np.float64), t[1::2].astype(np.float64)
-    # nodes: H_j = 1 - d[N - j] for j <= N
-    for j in [0, 1, 7, 500, 4999, 9999, 10000 - 1]:
-        assert abs(H[j] - (1.0 - exp_table_node(10000 - j))) <= 6e-8
-    # slopes are the node differences
-    np.testing.assert_allclose(H[:-1] + S[:-1], H[1:], rtol=0, atol=1.2e-7)
-    rng = np.random.default_rng(7)
-    tau = np.concatenate([rng.uniform(0, 10, 4000), rng.uniform(0, 0.01, 500), rng.uniform(10, 45, 500)])
-    # the kernel's FP32 evaluation: w = min(1000 tau, 39999), j = trunc(w), f = fract(w)
-    w = np.minimum(np.float32(tau) * np.float32(1000.0), np.float32(39999.0)).astype(np.float32)
-    j = w.astype(np.int64)
-    f = (w - j.astype(np.float32)).astype(np.float32)
-    e = (t[2 * j + 1] * f + t[2 * j]).astype(np.float64)
-    ref = np.array([1.0 - exp_table(-v) if v < 10.0 else 1.0 - math.exp(-v) for v in tau])
-    assert np.abs(e - ref).max() < 3e-7
-
-
 # --- sweeps against closed forms ------------------------------------------------
 
 @pytest.fixture(scope="module")
