@@ -27,7 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MOC segment-group updates/s; batched lattice solves/s + p99 at keff err≤1e-5"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes/clk x clock (non-packed FP32)
+# VALU issue: 256 CUs x 4 SIMDs, a wave64 VALU instruction occupies its SIMD
+# for 4 cycles (16 lanes per cycle; longer for transcendentals), 2.4 GHz.
+N_SIMD, VALU_CYCLES_PER_INST, CLOCK_HZ = 256 * 4, 4, 2.4e9
 
 
 def algorithmic_bytes(p) -> int:
@@ -39,21 +41,21 @@ def algorithmic_bytes(p) -> int:
     return 8 * S + 16 * F * G + 16 * R * P * G
 
 
-def read_traffic(path: str):
-    """HBM bytes per sweep-kernel dispatch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
-    if not os.path.exists(path):
-        return None
+def read_pmc(path: str) -> dict:
+    """Key: value lines of the committed rocprofv3 PMC summary of the sweep
+    kernel (HBM bytes per dispatch from FETCH_SIZE + WRITE_SIZE calibrated on a
+    copy kernel; VALU instructions per dispatch from SQ_INSTS_VALU)."""
     vals = {}
-    with open(path) as f:
-        for line in f:
-            if ":" in line and not line.startswith("#"):
-                k, v = line.split(":", 1)
-                vals[k.strip()] = v.strip()
-    try:
-        return float(vals["hbm_bytes_per_dispatch"])
-    except (KeyError, ValueError):
-        return None
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    try:
+                        vals[k.strip()] = float(v.strip())
+                    except ValueError:
+                        pass
+    return vals
 
 
 def cpu_baseline(hp, n_sweeps: int):
@@ -198,6 +200,21 @@ def bench_c3(work_dir: str, with_oracle: bool):
     return out
 
 
+def copy_calibration():
+    """A plain device copy of known size: 1 GiB read and 1 GiB written per
+    copy, three copies. Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives
+    the counters' KiB per GiB actually moved, which scales the sweep kernel's
+    readings into HBM bytes."""
+    import torch
+    n = 256 * 1024 * 1024
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,11 +229,16 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--c3", action="store_true", help="add the full C3 core solve with CMFD")
     ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on one GPU only "
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration()
+        return
 
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
@@ -252,11 +274,16 @@ def main():
     _, kernel_ms = dev.time_sweeps(max(10, min(args.steps, 50)))
     B = algorithmic_bytes(p)
     achieved = B / (kernel_ms * 1e-3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "sweep_c2_pmc.txt"))
+    pmc = read_pmc(os.path.join(ROOT, "profiles", "sweep_c2_pmc.txt"))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "k_sweep<3,false>", "kernel_ms": kernel_ms, "algorithmic_bytes": B,
-                "valu_frac_est": (U / (kernel_ms * 1e-3)) * 10.0 / VALU_PEAK_LANE_OPS}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": pmc.get("hbm_bytes_per_dispatch"),
+                "kernel": "k_sweep<3,false>", "kernel_ms": kernel_ms, "algorithmic_bytes": B}
+    insts = pmc.get("sq_insts_valu_per_dispatch")
+    if insts:
+        # the instruction-issue bound that actually limits this kernel
+        issue_ms = insts * VALU_CYCLES_PER_INST / N_SIMD / CLOCK_HZ * 1e3
+        roofline["valu_issue"] = {"insts_per_dispatch": insts, "issue_ms": issue_ms,
+                                  "frac": issue_ms / kernel_ms}
     line = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",

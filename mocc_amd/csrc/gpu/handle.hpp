@@ -105,7 +105,7 @@ struct mocgpu_handle {
 
     // plan
     DBuf<BandInfo> d_bands;
-    DBuf<int32_t> d_order, d_order_gs, d_loc_fsr, d_slot_ptr, d_slot_idx, d_ev_off, d_surf_ptr, d_surf_idx;
+    DBuf<int32_t> d_order, d_order_gs, d_loc_fsr, d_slot_ptr, d_slot_pos, d_ev_off, d_surf_ptr, d_surf_idx;
     DBuf<uint16_t> d_seg_loc;
     DBuf<float> d_seg_len, d_ang_rsin, d_ang_wt, d_ang_cw, d_ang_fw;
     DBuf<RayInfo> d_rays;
@@ -168,6 +168,7 @@ struct mocgpu_handle {
         a.phi = d_phi.p;
         a.psi = d_psi.p;
         a.psum = d_psum.p;
+        a.slot_pos = d_slot_pos.p;
         a.ev_off = d_ev_off.p;
         a.ev = d_ev.p;
         a.csum = d_csum.p;
@@ -183,7 +184,6 @@ struct mocgpu_handle {
         r.g_fixed = g_fixed;
         r.n_slots = (int)plan->loc_fsr.size();
         r.slot_ptr = d_slot_ptr.p;
-        r.slot_idx = d_slot_idx.p;
         r.psum = d_psum.p;
         r.fsr_mat = d_fsr_mat.p;
         r.mat_tr = d_mat_tr.p;

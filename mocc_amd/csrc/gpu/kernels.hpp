@@ -22,7 +22,7 @@ struct SweepArgs {
     int32_t G, F, bcpg;
     int32_t lds_local;      // LDS FSR slots per workgroup (max band-local FSRs + kDummySlots)
     int32_t lds_lsurf;      // LDS coarse-surface slots (0 without currents)
-    int32_t n_slots;        // partial slots per group (psum is [G][n_slots])
+    int32_t n_slots;        // partial slots per group
     int32_t n_lslots;       // current slots per group (csum/ssum are [G][n_lslots])
     uint64_t parity;        // bit g: which boundary buffer of group g is incoming
     const uint64_t *parity_dev;  // if set, the parity word is read from device memory instead
@@ -38,7 +38,8 @@ struct SweepArgs {
     const double *mat_tr, *mat_ss;
     const double *src, *phi;
     float *psi;             // [G][2][bcpg]
-    double *psum;           // [G][n_slots]
+    double *psum;           // [n_slots][G]: slot s of group g at psum[slot_pos[s] * G + g]
+    const int32_t *slot_pos;
     const int32_t *ev_off;
     const uint32_t *ev;
     double *csum, *ssum;    // [G][n_lslots]
@@ -46,7 +47,7 @@ struct SweepArgs {
 
 struct ReduceArgs {
     int32_t F, G, g_fixed, n_slots;
-    const int32_t *slot_ptr, *slot_idx;
+    const int32_t *slot_ptr;  // partials of FSR f: psum[s * G + g], s in [slot_ptr[f], slot_ptr[f + 1])
     const double *psum;
     const int32_t *fsr_mat;
     const double *mat_tr, *mat_ss, *vol;

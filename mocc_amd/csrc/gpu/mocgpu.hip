@@ -385,7 +385,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
             h->d_ang_cw = like->d_ang_cw;
             h->d_ang_fw = like->d_ang_fw;
             h->d_slot_ptr = like->d_slot_ptr;
-            h->d_slot_idx = like->d_slot_idx;
+            h->d_slot_pos = like->d_slot_pos;
             h->d_ev_off = like->d_ev_off;
             h->d_ev = like->d_ev;
             h->d_surf_ptr = like->d_surf_ptr;
@@ -413,7 +413,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         h->d_ang_cw.upload(pl.ang_cw, s);
         h->d_ang_fw.upload(pl.ang_fw, s);
         h->d_slot_ptr.upload(pl.fsr_slot_ptr, s);
-        h->d_slot_idx.upload(pl.fsr_slot_idx, s);
+        h->d_slot_pos.upload(pl.slot_pos, s);
         h->d_ev_off.upload(pl.ev_off, s);
         h->d_ev.upload(pl.ev, s);
         h->d_surf_ptr.upload(pl.surf_slot_ptr, s);
@@ -612,7 +612,7 @@ int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
     const size_t b = h->d_bands.bytes() + h->d_order.bytes() + h->d_order_gs.bytes() + h->d_seg_loc.bytes() + h->d_seg_len.bytes() +
                      h->d_loc_fsr.bytes() + h->d_rays.bytes() + h->d_azi.bytes() + h->d_ang_rsin.bytes() +
                      h->d_ang_wt.bytes() + h->d_ang_cw.bytes() + h->d_ang_fw.bytes() + h->d_slot_ptr.bytes() +
-                     h->d_slot_idx.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +
+                     h->d_slot_pos.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +
                      h->d_surf_idx.bytes() + h->d_surf_area.bytes() + h->d_fsr_mat.bytes() +
                      h->d_vol.bytes() + h->d_area.bytes() + h->d_pin_off.bytes() + h->d_pin_cell.bytes() +
                      h->d_pin_area.bytes();

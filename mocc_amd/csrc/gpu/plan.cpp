@@ -351,6 +351,8 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
         pl.fsr_slot_idx.assign(pl.loc_fsr.size(), 0);
         std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
         for (size_t s = 0; s < pl.loc_fsr.size(); s++) pl.fsr_slot_idx[fill[pl.loc_fsr[s]]++] = (int32_t)s;
+        pl.slot_pos.assign(pl.loc_fsr.size(), 0);
+        for (size_t q = 0; q < pl.fsr_slot_idx.size(); q++) pl.slot_pos[pl.fsr_slot_idx[q]] = (int32_t)q;
         for (int f = 0; f < pl.F; f++)
             if (pl.fsr_slot_ptr[f + 1] == pl.fsr_slot_ptr[f])
                 throw std::runtime_error("No rays passed through FSR " + std::to_string(f) +
@@ -441,6 +443,7 @@ std::string check_plan(const mocgpu_problem &p, const SweepPlan &pl, int band_ra
             const int slot = pl.fsr_slot_idx[s];
             if (slot < 0 || slot >= (int)pl.loc_fsr.size() || pl.loc_fsr[slot] != f) fail("FSR slot list");
             if (s > pl.fsr_slot_ptr[f] && slot <= pl.fsr_slot_idx[s - 1]) fail("FSR slots not increasing");
+            if (pl.slot_pos.size() != pl.loc_fsr.size() || pl.slot_pos[slot] != s) fail("slot positions");
             slot_seen[slot]++;
         }
     for (int v : slot_seen)

@@ -79,7 +79,11 @@ struct SweepPlan {
     std::vector<int32_t> loc_fsr;           // [total local] global FSR
     std::vector<RayInfo> rays;              // [sum n_rays]
     std::vector<int32_t> ray_global;        // [sum n_rays] problem ray of each band lane (host only)
-    std::vector<int32_t> fsr_slot_ptr, fsr_slot_idx;
+    // Partial slots of each FSR in increasing slot order: positions
+    // fsr_slot_ptr[f] .. fsr_slot_ptr[f + 1] - 1 hold slots fsr_slot_idx[..];
+    // slot_pos is the inverse map (slot -> position). The band tallies are
+    // stored at their positions, so each FSR's partials are contiguous.
+    std::vector<int32_t> fsr_slot_ptr, fsr_slot_idx, slot_pos;
     std::vector<AziInfo> azi;
     std::vector<float> ang_rsin, ang_wt;    // [A*NP]: 1/sin(theta), w*spacing*hz*sin(theta)*pi
     std::vector<float> ang_cw, ang_fw;      // [A*NP*2]: Current weights per normal

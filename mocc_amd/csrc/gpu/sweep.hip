@@ -31,33 +31,29 @@ namespace {
 constexpr double FPI = 4.0 * 3.1415926535897932;
 constexpr int kNumXcd = 8;  // MI355X: workgroup i is dispatched to XCD i % 8
 
-// Largest interpolation coordinate 1000 tau: the last node used. Beyond it
-// 1 - exp(-tau) rounds to 1 in FP32.
-constexpr float kWMax = 39999.0f;
-
 // 1 - exp(-tau) as Exponential_Linear<10000> on [-10, 0] evaluates exp(-tau)
 // (src/core/exponential.hpp:56-112), from the coordinate w = 1000 tau: with
 // j = floor(w), f = w - j and x = j / 1000 the interpolant is
 //   H_j + f (H_{j+1} - H_j),  H_j = 1 - e^{-x},  H_{j+1} - H_j = e^{-x} c,
 // c = 1 - e^{-1/1000}. e^{-x} is the hardware exp2; H_j is 1 - e^{-x} for
-// x >= 1/4 and the series of 1 - e^{-x} below (no cancellation near 1).
-// Past tau = 10 the reference calls exp itself; the same interpolant
-// continues there (error < 6e-12) up to w = kWMax. Relative error against the
-// FP64 interpolant: below 3e-7 for every tau.
+// j >= 250 (x >= 1/4) and x P(x) below (no cancellation near 1), P the
+// degree-4 minimax fit of (1 - e^{-x}) / x on [0, 1/4] in relative error
+// (2.5e-9), written in j (coefficient k times 1000^-(k+1)). Past tau = 10 the
+// reference calls exp itself; the same interpolant continues there (error
+// < 6e-12), and for large tau e^{-x} underflows to 0 and the result is 1.
+// Relative error against the FP64 interpolant: below 3e-7 for every finite
+// tau >= 0.
 __device__ __forceinline__ float attenuation(float w)
 {
-    w = fminf(w, kWMax);
     const float f = __builtin_amdgcn_fractf(w);
-    const float x = (w - f) * 1.0e-3f;
-    const float e = __builtin_amdgcn_exp2f(x * -1.4426950408889634f);
-    float s = 1.0f / 5040.0f;
-    s = fmaf(s, x, -1.0f / 720.0f);
-    s = fmaf(s, x, 1.0f / 120.0f);
-    s = fmaf(s, x, -1.0f / 24.0f);
-    s = fmaf(s, x, 1.0f / 6.0f);
-    s = fmaf(s, x, -0.5f);
-    s = fmaf(s, x, 1.0f);
-    const float h = x < 0.25f ? x * s : 1.0f - e;
+    const float j = w - f;
+    const float e = __builtin_amdgcn_exp2f(j * -1.4426950408889634e-3f);
+    float s = 7.510221563279629e-18f;
+    s = fmaf(s, j, -4.148895293474198e-14f);
+    s = fmaf(s, j, 1.6665081679821016e-10f);
+    s = fmaf(s, j, -4.999994933605194e-07f);
+    s = fmaf(s, j, 1.0e-3f);
+    const float h = j < 250.0f ? j * s : 1.0f - e;
     return fmaf(e * 9.995001666250084e-4f, f, h);
 }
 
@@ -297,8 +293,10 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     }
     __syncthreads();
     const double rscale = ldexp(1.0, -kx);
-    double *__restrict__ ps = A.psum + (size_t)g * A.n_slots + B.loc_base;
-    for (int l = tid; l < B.n_local; l += kWG) ps[l] = (double)(long long)tal[l] * rscale;
+    // each band slot goes to its FSR-major position; the G workgroups of a
+    // band run on one XCD, so its L2 joins their writes of a slot's groups
+    const int32_t *__restrict__ pos = A.slot_pos + B.loc_base;
+    for (int l = tid; l < B.n_local; l += kWG) A.psum[(size_t)pos[l] * A.G + g] = (double)(long long)tal[l] * rscale;
     if (CUR) {
         const double rc = ldexp(1.0, -kc);
         double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
@@ -358,10 +356,12 @@ __global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArg
 }
 
 // phi = T / (St V) + 4 pi q (moc_sweeper_kernel.inc.hpp:165-173). T is the
-// sum of the FSR's band slots: kRedLanes lanes per (FSR, group) take the
-// slots s = first + lane, first + lane + kRedLanes, ... and an xor butterfly
-// adds the lane sums; every lane pairs the same operands, so all lanes hold
-// the same T and the order is fixed.
+// sum of the FSR's band partials, stored contiguously ([position][group]):
+// kRedLanes lanes per (FSR, group) take the positions s = first + lane,
+// first + lane + kRedLanes, ... and an xor butterfly adds the lane sums;
+// every lane pairs the same operands, so all lanes hold the same T and the
+// order is fixed. The G items of an FSR are adjacent, so a wave's loads
+// cover consecutive addresses.
 constexpr int kRedLanes = 8;
 
 __device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, int lane)
@@ -378,9 +378,9 @@ __device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, 
     const int G = A.G;
     double T = 0.0;
     if (valid) {
-        const double *__restrict__ ps = A.psum + (size_t)g * A.n_slots;
+        const double *__restrict__ ps = A.psum + g;
         const int s1 = A.slot_ptr[f + 1];
-        for (int s = A.slot_ptr[f] + lane; s < s1; s += kRedLanes) T += ps[A.slot_idx[s]];
+        for (int s = A.slot_ptr[f] + lane; s < s1; s += kRedLanes) T += ps[(size_t)s * G];
     }
 #pragma unroll
     for (int o = 1; o < kRedLanes; o <<= 1) T += __shfl_xor(T, o, kRedLanes);
