@@ -43,8 +43,8 @@ def algorithmic_bytes(p) -> int:
 
 def read_pmc(path: str) -> dict:
     """Key: value lines of the committed rocprofv3 PMC summary of the sweep
-    kernel (HBM bytes per dispatch from FETCH_SIZE + WRITE_SIZE calibrated on a
-    copy kernel; VALU instructions per dispatch from SQ_INSTS_VALU)."""
+    kernel, per inner sweep (HBM bytes from FETCH_SIZE + WRITE_SIZE calibrated
+    on a copy kernel; VALU instructions from SQ_INSTS_VALU)."""
     vals = {}
     if os.path.exists(path):
         with open(path) as f:
@@ -276,13 +276,14 @@ def main():
     achieved = B / (kernel_ms * 1e-3) / 1e9
     pmc = read_pmc(os.path.join(ROOT, "profiles", "sweep_c2_pmc.txt"))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": pmc.get("hbm_bytes_per_dispatch"),
-                "kernel": "k_sweep<3,false>", "kernel_ms": kernel_ms, "algorithmic_bytes": B}
-    insts = pmc.get("sq_insts_valu_per_dispatch")
+                "frac": achieved / HBM_PEAK_GBS, "traffic": pmc.get("hbm_bytes_per_sweep"),
+                "kernel": "k_sweep<3,false>, both Gauss-Seidel phase launches of one sweep",
+                "kernel_ms": kernel_ms, "algorithmic_bytes": B}
+    insts = pmc.get("sq_insts_valu_per_sweep")
     if insts:
         # the instruction-issue bound that actually limits this kernel
         issue_ms = insts * VALU_CYCLES_PER_INST / N_SIMD / CLOCK_HZ * 1e3
-        roofline["valu_issue"] = {"insts_per_dispatch": insts, "issue_ms": issue_ms,
+        roofline["valu_issue"] = {"insts_per_sweep": insts, "issue_ms": issue_ms,
                                   "frac": issue_ms / kernel_ms}
     line = {
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,

@@ -327,8 +327,11 @@ int mocgpu_plan_check(const mocgpu_problem *p, int32_t band_rays, char *msg, int
 int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h);
 
 /* Time the sweep kernel: run n_sweeps all-group inner sweeps (Jacobi energy,
- * no currents) and report the mean kernel duration measured with HIP events
- * on the library's stream. */
+ * no currents, the handle's boundary update) and report the mean time per
+ * sweep of the sweep-kernel launches (both phases of the Gauss-Seidel update,
+ * the single launch of the Jacobi one), measured with HIP events on the
+ * library's stream, and the mean host time per sweep including the flux
+ * reduction. */
 int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n_sweeps, double *ms_per_sweep,
                        double *kernel_ms);
 

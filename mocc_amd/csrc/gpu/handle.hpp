@@ -216,6 +216,12 @@ struct mocgpu_handle {
     void build_source(int g_fixed, bool jacobi);
 
 
+    // The sweep-kernel launches of one inner sweep of group g (g < 0: all
+    // groups) on `stream`: one launch over every band (Jacobi update, the
+    // caller flips the parity), or the octant-1 bands, then the octant-2
+    // bands (Gauss-Seidel update).
+    void launch_sweeps(int g, bool with_current);
+
     // one inner iteration of sweep1g for group g (g < 0: all groups)
     void sweep_once(int g, bool with_current);
 

@@ -244,21 +244,25 @@ void mocgpu_handle::build_source(int g_fixed, bool jacobi)
                        d_src.p);
 }
 
-void mocgpu_handle::sweep_once(int g, bool with_current)
+void mocgpu_handle::launch_sweeps(int g, bool with_current)
 {
     SweepArgs a = sweep_args(g);
-    if (opt.gs_boundary) {
-        // Gauss-Seidel in angle: octant-1 bands, then octant-2 bands
-        const int n1 = plan->n_bands_oct1, n2 = (int)plan->bands.size() - n1;
-        a.order = d_order_gs.p;
-        a.n_bands = n1;
-        if (n1 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n1, G, g < 0), stream);
-        a.order = d_order_gs.p + n1;
-        a.n_bands = n2;
-        if (n2 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n2, G, g < 0), stream);
-    } else {
+    if (!opt.gs_boundary) {
         launch_sweep(a, NP, with_current, sweep_grid_items((int)plan->bands.size(), G, g < 0), stream);
+        return;
     }
+    const int n1 = plan->n_bands_oct1, n2 = (int)plan->bands.size() - n1;
+    a.order = d_order_gs.p;
+    a.n_bands = n1;
+    if (n1 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n1, G, g < 0), stream);
+    a.order = d_order_gs.p + n1;
+    a.n_bands = n2;
+    if (n2 > 0) launch_sweep(a, NP, with_current, sweep_grid_items(n2, G, g < 0), stream);
+}
+
+void mocgpu_handle::sweep_once(int g, bool with_current)
+{
+    launch_sweeps(g, with_current);
     launch_reduce_flux(reduce_args(g), stream);
     if (with_current) {
         int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
@@ -886,21 +890,21 @@ int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return 2LL * h->NP * 
 int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double *kernel_ms)
 {
     return guard(h, [&] {
-        SweepArgs a = h->sweep_args(-1);
-        a.gs = 0;  // one launch over every band, reading and writing separate buffers
-        int items = sweep_grid_items((int)h->plan->bands.size(), h->G, true);
-        // warm-up
-        launch_sweep(a, h->NP, false, items, h->stream);
-        launch_reduce_flux(h->reduce_args(-1), h->stream);
+        if (n <= 0) throw std::runtime_error("n_sweeps must be positive");
+        // the launches of sweep_once(-1, false), with events around the sweep kernels
+        auto launch = [&](bool timed) {
+            if (timed) HIP_CHECK(hipEventRecord(h->ev0, h->stream));
+            h->launch_sweeps(-1, false);
+            if (!h->opt.gs_boundary) h->parity ^= (h->G >= 64 ? ~0ull : ((1ull << h->G) - 1ull));
+            if (timed) HIP_CHECK(hipEventRecord(h->ev1, h->stream));
+            launch_reduce_flux(h->reduce_args(-1), h->stream);
+        };
+        launch(false);  // warm-up
         h->sync();
         float ksum = 0.0f;
         auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < n; i++) {
-            a.parity = h->parity;
-            HIP_CHECK(hipEventRecord(h->ev0, h->stream));
-            launch_sweep(a, h->NP, false, items, h->stream);
-            HIP_CHECK(hipEventRecord(h->ev1, h->stream));
-            launch_reduce_flux(h->reduce_args(-1), h->stream);
+            launch(true);
             HIP_CHECK(hipEventSynchronize(h->ev1));
             float ms = 0.0f;
             HIP_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));

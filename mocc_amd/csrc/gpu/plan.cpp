@@ -20,6 +20,16 @@ int reflect(int iang, int normal, int nd)
     return iang + (refl[normal][ioct] - ioct) * nd;
 }
 
+// The azimuth an exit of azimuth a enters through a reflective face
+// (either normal; both give the same one).
+int mirror_azimuth(int a, int np, int nd)
+{
+    const int i = a * np;
+    const int x = reflect(i, 0, nd) % (2 * nd), y = reflect(i, 1, nd) % (2 * nd);
+    if (x != y) throw std::runtime_error("x and y reflections of an azimuth differ");
+    return x / np;
+}
+
 // Mesh::coarse_surf / coarse_neighbor / surface_normal for one plane
 struct Coarse {
     int nx, ny;
@@ -457,6 +467,18 @@ std::string check_plan(const mocgpu_problem &p, const SweepPlan &pl, int band_ra
         const int b = pl.band_order_gs[k];
         o2[b]++;
         if ((pl.bands[b].azi >= p.n_azi / 2) != ((int)k >= pl.n_bands_oct1)) fail("Gauss-Seidel phase order");
+    }
+    // The two-phase Gauss-Seidel update needs every exit of an azimuth to
+    // land in the incoming slots of its mirror, an azimuth of the other octant.
+    const int np_ = p.n_polar, nd_ = p.n_azi * p.n_polar / 2;
+    for (int az = 0; az < p.n_azi; az++) {
+        const int m = mirror_azimuth(az, np_, nd_);
+        if (m < 0 || m >= p.n_azi || (m >= p.n_azi / 2) == (az >= p.n_azi / 2)) fail("mirror azimuth");
+        for (int n = 0; n < 2; n++)
+            for (int d = 0; d < 2; d++) {
+                const int out = pl.azi[az].out_off[d][n];
+                if (out != pl.azi[m].in_off[0] && out != pl.azi[m].in_off[1]) fail("exit offsets miss the mirror azimuth");
+            }
     }
     for (size_t b = 0; b < pl.bands.size(); b++)
         if (o1[b] != 1 || o2[b] != 1) fail("band orders are permutations");

@@ -68,10 +68,11 @@ struct SweepPlan {
     int max_local = 0, max_lsurf = 0;       // LDS sizing
     std::vector<BandInfo> bands;
     std::vector<int32_t> band_order;        // longest-first
-    // Gauss-Seidel in angle: reflection couples azimuth a only with its
-    // mirror A-1-a, so mocc's angle-ordered boundary update is two phases:
-    // the octant-1 bands (band_order_gs[0, n_bands_oct1)), then the octant-2
-    // bands, which read the boundary fluxes phase 1 wrote.
+    // Gauss-Seidel in angle: reflection couples octant-1 azimuth a only with
+    // its mirror a + A/2 (boundary_condition.cpp's reflect), so mocc's
+    // angle-ordered boundary update is two phases: the octant-1 bands
+    // (band_order_gs[0, n_bands_oct1)), then the octant-2 bands, which read
+    // the boundary fluxes phase 1 wrote.
     std::vector<int32_t> band_order_gs;
     int32_t n_bands_oct1 = 0;
     std::vector<uint16_t> seg_loc;          // interleaved band-local FSR id
