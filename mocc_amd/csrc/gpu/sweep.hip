@@ -57,28 +57,28 @@ __device__ __forceinline__ float attenuation(float w)
     return fmaf(e * 9.995001666250084e-4f, f, h);
 }
 
-// Power-of-two scale 2^k with bound < 2^(62 - k), so |sum| * 2^k < 2^62.
-__device__ __forceinline__ int fixed_exp(double bound)
+// Power-of-two scale 2^k of a band's fixed-point tallies. `single` bounds
+// one contribution and `count` the contributions any slot receives: every
+// contribution times 2^k stays below 2^51 (the range of to_fixed) and every
+// slot's sum below 2^62.
+__device__ __forceinline__ int fixed_exp(double single, double count)
 {
-    if (!(bound > 0.0)) return 0;
-    int e;
-    frexp(bound, &e);
-    return 62 - e;
+    if (!(single > 0.0)) return 0;
+    int e1, e2;
+    frexp(single, &e1);          // single < 2^e1
+    frexp(single * count, &e2);  // sum < 2^e2
+    return min(51 - e1, 62 - e2);
 }
 
-// v * 2^k as a two's-complement 64-bit integer, |v * 2^k| < 2^62, in FP32
-// arithmetic: x = v * 2^k is exact, hi = floor(x / 2^32) and
-// lo = x - hi * 2^32 in [0, 2^32). lo is exact except when -2^32 < x < 0,
-// where it is rounded to 24 bits and clamped below 2^32 (an error below 2^9
-// units of 2^-k).
-__device__ __forceinline__ unsigned long long to_fixed(float v, int k)
+// round(v * 2^k) as a two's-complement 64-bit integer for |v * 2^k| <= 2^51,
+// scale = 2^k. (double)v * 2^k is exact; adding M = 1.5 * 2^52 rounds it to
+// an integer n held in the low mantissa bits, so the bit pattern of the sum
+// is bits(M) + n. bits(M) = 0x4338000000000000 has a zero low word, so taking
+// it off changes the high word only (three VALU instructions in all).
+__device__ __forceinline__ unsigned long long to_fixed(float v, double scale)
 {
-    const float x = ldexpf(v, k);
-    const float hf = floorf(x * 2.3283064365386963e-10f);
-    const float lo = fminf(fmaf(-hf, 4294967296.0f, x), 4294967040.0f);
-    const unsigned int uhi = (unsigned int)(int)hf;
-    const unsigned int ulo = (unsigned int)lo;
-    return ((unsigned long long)uhi << 32) | ulo;
+    const double y = fma((double)v, scale, 6755399441055744.0);
+    return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
 }
 
 // One (band, group) item: the whole workgroup sweeps the band's rays in both
@@ -181,10 +181,13 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // |dpsi| <= 2 * max(|psi_in|, |q|) since psi relaxes towards q; the
     // per-contribution bound times the contribution count bounds every sum.
     const double psi_bound = (double)__uint_as_float(s_bound);
-    const int kx = fixed_exp(2.0 * psi_bound * (double)wsum * (double)B.max_contrib * 1.0001);
+    const int kx = fixed_exp(2.0 * psi_bound * (double)wsum * 1.0001, (double)B.max_contrib);
+    const double sx = ldexp(1.0, kx);
     int kc = 0, ep = 0, ee = 0;
+    double sc = 1.0;
     if (CUR) {
-        kc = fixed_exp(psi_bound * (double)cwsum * (double)B.max_surf_contrib * 1.0001);
+        kc = fixed_exp(psi_bound * (double)cwsum * 1.0001, (double)B.max_surf_contrib);
+        sc = ldexp(1.0, kc);
         if (active) {
             ep = A.ev_off[2 * (B.ray_base + j) + dir];
             ee = A.ev_off[2 * (B.ray_base + j) + dir + 1];
@@ -203,8 +206,8 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
             cv = fmaf(psi[p], nn ? cw[p][1] : cw[p][0], cv);
             sv = fmaf(psi[p], nn ? fw[p][1] : fw[p][0], sv);
         }
-        atomicAdd(&tcur[ls], to_fixed(csign * cv, kc));
-        atomicAdd(&tsfl[ls], to_fixed(sv, kc));
+        atomicAdd(&tcur[ls], to_fixed(csign * cv, sc));
+        atomicAdd(&tsfl[ls], to_fixed(sv, sc));
     };
 
     // chunked segment stream of this lane's ray and direction
@@ -273,7 +276,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
                 psi[p] = psi[p] - d;
                 cacc = fmaf(d, wt[p], cacc);
             }
-            atomicAdd(&tal[loc[u]], to_fixed(cacc, kx));
+            atomicAdd(&tal[loc[u]], to_fixed(cacc, sx));
         }
     }
     if (CUR && active) {
