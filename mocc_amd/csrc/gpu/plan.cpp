@@ -269,11 +269,11 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
                             int t = c * kChunk + u;
                             if (t < n) {
                                 int k = d == 0 ? t : n - 1 - t;
-                                pl.seg_loc.push_back((uint16_t)g2l[p.seg_fsr[s0 + k]]);
+                                pl.seg_loc.push_back((uint16_t)(g2l[p.seg_fsr[s0 + k]] * kSlotBytes));
                                 pl.seg_len.push_back((float)p.seg_len[s0 + k]);
                             } else {
                                 // padding: a zero-length step into the lane's own dummy slot
-                                pl.seg_loc.push_back((uint16_t)(n_local + j));
+                                pl.seg_loc.push_back((uint16_t)((n_local + j) * kSlotBytes));
                                 pl.seg_len.push_back(0.0f);
                             }
                         }
@@ -422,7 +422,8 @@ std::string check_plan(const mocgpu_problem &p, const SweepPlan &pl, int band_ra
                 for (int t = 0; t < nchunk * kChunk; t++) {
                     const size_t e = (size_t)B.seg_base[d] + ((size_t)(t / kChunk) * B.n_rays + j) * kChunk + t % kChunk;
                     if (e >= pl.seg_loc.size()) fail("stream index");
-                    const int loc = pl.seg_loc[e];
+                    if (pl.seg_loc[e] % kSlotBytes != 0) fail("stream offset not a slot boundary");
+                    const int loc = pl.seg_loc[e] / kSlotBytes;
                     const float len = pl.seg_len.empty() ? 0.0f : pl.seg_len[e];
                     if (t < n) {
                         const int k = d == 0 ? t : n - 1 - t;

@@ -29,6 +29,8 @@ namespace gpu {
 constexpr int kMaxLocal = 2047;  // band-local FSRs held in LDS
 constexpr int kMaxRaysPerBand = 64;
 constexpr int kDummySlots = kMaxRaysPerBand;  // padding slots after a band's n_local FSRs
+constexpr int kSlotBytes = 8;     // bytes per slot of the band's LDS tally and (Sigma_t, q) arrays
+static_assert((kMaxLocal + kDummySlots) * kSlotBytes < 65536, "stream byte offsets must fit 16 bits");
 constexpr int kChunk = 8;         // segments per lane per vector load (chunked layout)
 constexpr int kMaxLSurf = 1024;   // band-local coarse surfaces held in LDS
 
@@ -75,7 +77,7 @@ struct SweepPlan {
     // the boundary fluxes phase 1 wrote.
     std::vector<int32_t> band_order_gs;
     int32_t n_bands_oct1 = 0;
-    std::vector<uint16_t> seg_loc;          // interleaved band-local FSR id
+    std::vector<uint16_t> seg_loc;          // interleaved band-local FSR slot x kSlotBytes (LDS byte offset)
     std::vector<float> seg_len;             // interleaved length
     std::vector<int32_t> loc_fsr;           // [total local] global FSR
     std::vector<RayInfo> rays;              // [sum n_rays]

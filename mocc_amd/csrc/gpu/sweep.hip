@@ -217,10 +217,13 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const int nchunk = (n + kChunk - 1) / kChunk;
     uint4 cl = make_uint4(0, 0, 0, 0);
     float4 ca = make_float4(0, 0, 0, 0), cb = ca;
+    // lane's current chunk; chunk c of ray j is row c of nr rays
+    const uint4 *__restrict__ lp = locp + j;
+    const float4 *__restrict__ ap = lenp + 2 * j;
     if (active && nchunk > 0) {
-        cl = locp[j];
-        ca = lenp[2 * j];
-        cb = lenp[2 * j + 1];
+        cl = lp[0];
+        ca = ap[0];
+        cb = ap[1];
     }
     for (int c = 0; c < nchunk; c++) {
         unsigned int loc[kChunk];
@@ -246,18 +249,20 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         float q[kChunk], at[kChunk][NP];
 #pragma unroll
         for (int u = 0; u < kChunk; u++) {
-            const float2 e = sq[loc[u]];
+            const float2 e = *(const float2 *)((const char *)sq + loc[u]);
             q[u] = e.y;
             const float tau0 = e.x * len[u];
 #pragma unroll
             for (int p = 0; p < NP; p++) at[u][p] = attenuation(tau0 * rk[p]);
         }
-        if (c + 1 < nchunk) {
-            const size_t ci = (size_t)(c + 1) * nr + j;
-            cl = locp[ci];
-            ca = lenp[2 * ci];
-            cb = lenp[2 * ci + 1];
-        }
+        // the next chunk's loads, without a branch: the last chunk reloads
+        // itself (the pointer stays inside the lane's stream)
+        const int adv = c + 1 < nchunk ? nr : 0;
+        lp += adv;
+        ap += 2 * adv;
+        cl = lp[0];
+        ca = ap[0];
+        cb = ap[1];
         __builtin_amdgcn_sched_barrier(0);
         // Phase 2: the psi recurrence and the tallies.
 #pragma unroll
@@ -276,7 +281,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
                 psi[p] = psi[p] - d;
                 cacc = fmaf(d, wt[p], cacc);
             }
-            atomicAdd(&tal[loc[u]], to_fixed(cacc, sx));
+            atomicAdd((unsigned long long *)((char *)tal + loc[u]), to_fixed(cacc, sx));
         }
     }
     if (CUR && active) {
