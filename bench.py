@@ -138,10 +138,10 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
     }
 
 
-def bench_serving(work_dir: str, n_req: int, seed: int = 5):
+def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5):
     """Poisson arrivals of branch-case requests served by the continuous-
-    batching loop at 0.5x and 1.0x of its burst throughput, beside the same
-    requests solved one at a time."""
+    batching loop at the given fractions of its burst (saturation)
+    throughput, beside the same requests solved one at a time."""
     import numpy as np
     from mocc_amd.branch import BranchFactory, c5_request
     from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
@@ -155,7 +155,7 @@ def bench_serving(work_dir: str, n_req: int, seed: int = 5):
     out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
                        "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]",
            "burst": burst}
-    for frac in (0.5, 1.0):
+    for frac in rates:
         rate = frac * burst["solves_per_s"]
         out[f"poisson_{frac:g}x"] = {"rate_per_s": rate,
                                      **summarize(srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases))}
@@ -227,6 +227,8 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--serial-cases", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rates", default="0.5,1.0",
+                    help="comma-separated arrival rates as fractions of the burst throughput")
     ap.add_argument("--c3", action="store_true", help="add the full C3 core solve with CMFD")
     ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
     ap.add_argument("--copy-calib", action="store_true",
@@ -301,7 +303,8 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 6)
     if not args.no_serve:
-        line["serving"] = bench_serving(deck_dir, args.serve_requests)
+        rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
+        line["serving"] = bench_serving(deck_dir, args.serve_requests, rates)
     if args.c3:
         line["c3"] = bench_c3(deck_dir, args.c3_oracle)
     print(json.dumps(line), flush=True)

@@ -152,6 +152,7 @@ struct mocgpu_handle {
         a.parity = parity;
         a.gs = opt.gs_boundary ? 1 : 0;
         a.interleave = opt.lane_map == 0 ? 0 : 1;
+        a.wg_rays = plan->max_band_rays <= 32 ? 32 : kRays;
         a.seg_loc = d_seg_loc.p;
         a.seg_len = d_seg_len.p;
         a.loc_fsr = d_loc_fsr.p;

@@ -28,6 +28,7 @@ struct SweepArgs {
     const uint64_t *parity_dev;  // if set, the parity word is read from device memory instead
     int32_t gs;             // 1: Gauss-Seidel boundary update (exit fluxes go to the incoming buffer)
     int32_t interleave;     // 1: each wave takes even rays one way and odd rays the other
+    int32_t wg_rays;        // rays per direction of a workgroup: 64 (two waves) or 32 (one wave); the plan's widest band fits
     const uint16_t *seg_loc;
     const float *seg_len;
     const int32_t *loc_fsr;

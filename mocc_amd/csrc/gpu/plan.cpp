@@ -349,6 +349,7 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
     pl.n_seg_padded = (int64_t)pl.seg_loc.size();
     for (const BandInfo &b : pl.bands) {
         pl.max_local = std::max(pl.max_local, b.n_local);
+        pl.max_band_rays = std::max(pl.max_band_rays, b.n_rays);
         pl.max_lsurf = std::max(pl.max_lsurf, b.n_lsurf);
     }
 

@@ -68,6 +68,7 @@ struct SweepPlan {
     int n_surf = 0;
     int64_t n_seg_padded = 0;
     int max_local = 0, max_lsurf = 0;       // LDS sizing
+    int max_band_rays = 0;                  // widest band (workgroup width)
     std::vector<BandInfo> bands;
     std::vector<int32_t> band_order;        // longest-first
     // Gauss-Seidel in angle: reflection couples octant-1 azimuth a only with

@@ -84,7 +84,7 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale)
 // One (band, group) item: the whole workgroup sweeps the band's rays in both
 // directions for group g. Ends after the band's partial tallies are stored;
 // the caller synchronizes before reusing smem.
-template <int NP, bool CUR>
+template <int NP, bool CUR, int W>
 __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, unsigned long long *smem,
                                            unsigned int &s_bound)
 {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
 
     // stage the band's cross sections and transport source
     unsigned int lmax = 0u;
-    for (int l = tid; l < B.n_local; l += kWG) {
+    for (int l = tid; l < B.n_local; l += 2 * W) {
         const int f = A.loc_fsr[B.loc_base + l];
         const int m = A.fsr_mat[f];
         const double tr = A.mat_tr[m * G + g];
@@ -112,12 +112,12 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         tal[l] = 0ull;
         lmax = max(lmax, __float_as_uint(fabsf(q32)));
     }
-    if (tid < kDummySlots) {
-        sq[B.n_local + tid] = make_float2(0.0f, 0.0f);
-        tal[B.n_local + tid] = 0ull;
+    for (int t = tid; t < kDummySlots; t += 2 * W) {
+        sq[B.n_local + t] = make_float2(0.0f, 0.0f);
+        tal[B.n_local + t] = 0ull;
     }
     if (CUR) {
-        for (int l = tid; l < B.n_lsurf; l += kWG) {
+        for (int l = tid; l < B.n_lsurf; l += 2 * W) {
             tcur[l] = 0ull;
             tsfl[l] = 0ull;
         }
@@ -128,8 +128,8 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // odd rays in the other (the other wave the reverse), so neighbouring
     // rays enter a given FSR at different steps and fewer lanes of a wave hit
     // the same LDS tally address in the same step.
-    const int j = tid % kRays;
-    const int dir = A.interleave ? ((j & 1) ^ (tid / kRays)) : (tid / kRays);
+    const int j = tid % W;
+    const int dir = A.interleave ? ((j & 1) ^ (tid / W)) : (tid / W);
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
     // select (not index) the per-direction offsets: indexing a local array
@@ -304,20 +304,21 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // each band slot goes to its FSR-major position; the G workgroups of a
     // band run on one XCD, so its L2 joins their writes of a slot's groups
     const int32_t *__restrict__ pos = A.slot_pos + B.loc_base;
-    for (int l = tid; l < B.n_local; l += kWG) A.psum[(size_t)pos[l] * A.G + g] = (double)(long long)tal[l] * rscale;
+    for (int l = tid; l < B.n_local; l += 2 * W) A.psum[(size_t)pos[l] * A.G + g] = (double)(long long)tal[l] * rscale;
     if (CUR) {
         const double rc = ldexp(1.0, -kc);
         double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
         double *__restrict__ ss = A.ssum + (size_t)g * A.n_lslots + B.lsurf_base;
-        for (int l = tid; l < B.n_lsurf; l += kWG) {
+        for (int l = tid; l < B.n_lsurf; l += 2 * W) {
             cs[l] = (double)(long long)tcur[l] * rc;
             ss[l] = (double)(long long)tsfl[l] * rc;
         }
     }
 }
 
-template <int NP, bool CUR>
-__global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
+// W rays per direction: 2W lanes per workgroup (one wave for W = 32).
+template <int NP, bool CUR, int W>
+__global__ __launch_bounds__(2 * W) void k_sweep(SweepArgs A)
 {
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned int s_bound;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(kWG) void k_sweep(SweepArgs A)
         g = s % A.G;
         band = A.order[rank];
     }
-    sweep_band<NP, CUR>(A, band, g, smem, s_bound);
+    sweep_band<NP, CUR, W>(A, band, g, smem, s_bound);
 }
 
 // Persistent work-queue sweep over many problems. Thread 0 takes the next
@@ -357,8 +358,8 @@ __global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArg
         const int c = __builtin_amdgcn_readfirstlane(w.x);
         if (Q.status[2 * c + 1] != 0 || Q.n_inner[c] <= Q.inner) continue;
         const SweepArgs A = cases[c];
-        sweep_band<NP, false>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z), smem,
-                              s_bound);
+        sweep_band<NP, false, kRays>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
+                                     smem, s_bound);
         __syncthreads();
     }
 }
@@ -446,11 +447,17 @@ void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStr
     SweepArgs b = a;
     if (!cur) b.lds_lsurf = 0;
     const size_t lds = sweep_lds_bytes(b);
-    dim3 grid(n_items), block(kWG);
-#define MOCGPU_SWEEP_CASE(NPV)                                                     \
-    case NPV:                                                                      \
-        if (cur) hipLaunchKernelGGL((k_sweep<NPV, true>), grid, block, lds, s, b); \
-        else hipLaunchKernelGGL((k_sweep<NPV, false>), grid, block, lds, s, b);    \
+    const bool narrow = a.wg_rays == 32;
+    dim3 grid(n_items), block(narrow ? 64 : kWG);
+#define MOCGPU_SWEEP_CASE(NPV)                                                                 \
+    case NPV:                                                                                  \
+        if (narrow) {                                                                          \
+            if (cur) hipLaunchKernelGGL((k_sweep<NPV, true, 32>), grid, block, lds, s, b);     \
+            else hipLaunchKernelGGL((k_sweep<NPV, false, 32>), grid, block, lds, s, b);        \
+        } else {                                                                               \
+            if (cur) hipLaunchKernelGGL((k_sweep<NPV, true, kRays>), grid, block, lds, s, b);  \
+            else hipLaunchKernelGGL((k_sweep<NPV, false, kRays>), grid, block, lds, s, b);     \
+        }                                                                                      \
         break;
     switch (n_polar) {
         MOCGPU_SWEEP_CASE(1)

@@ -15,12 +15,12 @@ pytestmark = pytest.mark.gpu
 FLUX_RTOL = 1e-4
 
 
-def _setup_pair(hp, energy_mode, seed=0, gs=False):
+def _setup_pair(hp, energy_mode, seed=0, gs=False, band_rays=0):
     from mocc_amd.sweeper import GpuMoCSweeper
     F, G = hp.n_fsr, hp.n_group
     rng = np.random.default_rng(seed)
     phi0 = rng.uniform(0.5, 2.0, (F, G))
-    dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=energy_mode, gs_boundary=gs)
+    dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=energy_mode, gs_boundary=gs, band_rays=band_rays)
     ora = Oracle(hp, n_inner=1, gs_boundary=gs, energy_mode=energy_mode)
     for s in (dev, ora):
         s.initialize()
@@ -86,6 +86,25 @@ def test_c1_fss_step_parity(c1, energy_mode, gs):
         scale = np.abs(co[n0:n1]).max()
         assert np.abs(cd[n0:n1] - co[n0:n1]).max() <= 1e-5 * scale
         assert _rel(sd[n0:n1], so[n0:n1]) < 1e-5
+
+
+@pytest.mark.parametrize("gs", [True, False])
+def test_c1_one_wave_workgroup_parity(c1, gs):
+    """Bands of at most 32 rays run the one-wave workgroup (32 rays per
+    direction, 64 lanes): the same parity against the oracle, currents
+    included."""
+    dev, ora = _setup_pair(c1, 1, seed=5, gs=gs, band_rays=32)
+    assert dev.options().band_rays == 32
+    for s in (dev, ora):
+        s.fss_step(with_current=True)
+    assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
+    assert _rel(dev.boundary(), ora.boundary()) < 1e-5
+    p = c1.p
+    n0, n1 = p.nx * p.ny, p.nx * p.ny + (p.nx + 1) * p.ny + (p.ny + 1) * p.nx
+    for g in range(c1.n_group):
+        cd, _ = dev.currents(g)
+        co, _ = ora.currents(g)
+        assert np.abs(cd[n0:n1] - co[n0:n1]).max() <= 1e-5 * np.abs(co[n0:n1]).max()
 
 
 def test_c1_reproducible(c1):
