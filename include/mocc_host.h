@@ -77,6 +77,32 @@ int mocc_host_reload_xs(mocc_host_problem *h, const char *xsl_path);
  * FLAT correction preserves). */
 int mocc_host_fsr_areas(const mocc_host_problem *h, double *out, int32_t cap);
 
+/* What mocc's HDF5 output needs beyond the mocgpu_problem view. */
+typedef struct mocc_host_output_info {
+    int32_t n_polar, n_azimuthal; /* <ang_quad> attributes (AngularQuadrature::output,
+                                     angular_quadrature.cpp:213-249)                    */
+    int32_t dump_fsr_flux;        /* <sweeper dump_fsr_flux> (moc_sweeper.cpp:103)      */
+    int32_t dump_current;         /* <cmfd dump_current> (cmfd.cpp:161-163)             */
+    int32_t nx, ny, nz;           /* pin mesh; nz = 1 (single-plane decks only)         */
+    int32_t n_fuel_2d;            /* fuel pins of the plane (CoreMesh::n_fuel_2d)       */
+    int32_t n_group;
+    int32_t cmfd;                 /* <solver cmfd>: CMFD::output runs only with CMFD     */
+} mocc_host_output_info;
+int mocc_host_get_output_info(const mocc_host_problem *h, mocc_host_output_info *out);
+
+/* Energy-group upper bounds of the XSL library (MaterialLib::g_bounds,
+ * written as "eubounds" by FixedSourceSolver::output). Returns ng. */
+int mocc_host_group_bounds(const mocc_host_problem *h, double *out, int32_t cap);
+
+/* Per pin in FSR order: 1 if the pin is fuel (Pin::is_fuel: a fissile
+ * material or the pin's fuel attribute, pin.cpp:67-82). Returns the pin count. */
+int mocc_host_pin_fuel(const mocc_host_problem *h, int32_t *out, int32_t cap);
+
+/* Coarse surface index of every (cell, surface) for the six surfaces in
+ * AllSurfaces order (EAST, NORTH, WEST, SOUTH, TOP, BOTTOM; constants.cpp:23)
+ * as Mesh::coarse_surf numbers them: out[6 * cell + s]. Returns the cell count. */
+int mocc_host_coarse_surfaces(const mocc_host_problem *h, int32_t *out, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

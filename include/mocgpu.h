@@ -271,6 +271,14 @@ typedef struct mocgpu_eigen_state {
 int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol,
                        int32_t max_iter, int32_t min_iter,
                        mocgpu_eigen_state *state);
+/* EigenSolver::solve recording its convergence history, as EigenSolver keeps
+ * convergence_ and iteration_times_ for its output (eigen_solver.cpp:181-187,
+ * 293-315): after outer iteration n, history[4n .. 4n+3] = (k, error_k,
+ * error_psi, seconds since the solve began), for n < cap. The number of rows
+ * written is state->iterations (at most cap). */
+int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol,
+                               int32_t max_iter, int32_t min_iter,
+                               mocgpu_eigen_state *state, double *history, int32_t cap);
 /* One outer iteration (EigenSolver::step + the convergence bookkeeping of solve). */
 int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
 

@@ -3,6 +3,7 @@
 ``build_all()`` compiles:
   * libmocc_host.so  -- C++17 host front-end (g++),
   * libmocgpu.so     -- HIP for gfx950 (hipcc),
+  * libmocc_h5.so    -- mocc's HDF5 output file (g++, the HDF5 C library),
   * oracle/_build/liboracle.so -- the FP64 checker (test infrastructure).
 """
 from __future__ import annotations
@@ -57,6 +58,28 @@ def build_gpu():
     os.replace(out + ".tmp", out)
 
 
+# The HDF5 C library of this image (1.10); the output library is built only
+# where its header is found. It is not put on the run path (that directory
+# also holds an older libstdc++): callers load libhdf5 first by its full
+# path (mocc_amd.output does).
+HDF5_PREFIX = os.environ.get("MOCC_HDF5_PREFIX", "/opt/conda")
+
+
+def build_h5():
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"libmocc_h5.so skipped: no hdf5.h under {inc}", file=sys.stderr)
+        return
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libmocc_h5.so")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I" + inc,
+          os.path.join(PKG, "csrc", "h5", "output_h5.cpp"), "-o", out + ".tmp",
+          "-L" + LIB, "-lmocc_host", "-L" + lib, "-lhdf5",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -64,11 +87,12 @@ def build_oracle():
 def build_all():
     build_host()
     build_gpu()
+    build_h5()
     build_oracle()
 
 
 if __name__ == "__main__":
     which = sys.argv[1:] or ["all"]
     for w in which:
-        {"host": build_host, "gpu": build_gpu, "oracle": build_oracle, "all": build_all}[w]()
+        {"host": build_host, "gpu": build_gpu, "h5": build_h5, "oracle": build_oracle, "all": build_all}[w]()
     print("built:", ", ".join(which))

@@ -964,6 +964,14 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
 int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter, int32_t min_iter,
                        mocgpu_eigen_state *st)
 {
+    return mocgpu_eigen_solve_history(h, k_tol, psi_tol, max_iter, min_iter, st, nullptr, 0);
+}
+
+int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter,
+                               int32_t min_iter, mocgpu_eigen_state *st, double *history, int32_t cap)
+{
+    if (!st || (cap > 0 && !history)) return MOCGPU_ERR_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
     int rc = mocgpu_initialize(h);
     if (rc) return rc;
     *st = mocgpu_eigen_state{};
@@ -982,6 +990,13 @@ int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol, int32_t m
     for (int n = 0; n < max_iter; n++) {
         rc = mocgpu_eigen_iterate(h, st);
         if (rc) return rc;
+        if (n < cap) {
+            double *row = history + 4 * (size_t)n;
+            row[0] = st->k;
+            row[1] = st->err_k;
+            row[2] = st->err_psi;
+            row[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
         if (st->err_k < k_tol && st->err_psi < psi_tol && n >= min_iter) {
             st->converged = 1;
             return 0;

@@ -207,4 +207,54 @@ int mocc_host_fsr_areas(const mocc_host_problem *h, double *out, int32_t cap)
     return (int)a.size();
 }
 
+int mocc_host_get_output_info(const mocc_host_problem *h, mocc_host_output_info *out)
+{
+    if (!h || !h->prob || !out) return MOCGPU_ERR_ARG;
+    const SolverParams &p = h->prob->params();
+    const CoreMesh &m = h->prob->mesh();
+    out->n_polar = p.n_polar;
+    out->n_azimuthal = p.n_azimuthal;
+    out->dump_fsr_flux = p.dump_fsr_flux ? 1 : 0;
+    out->dump_current = p.dump_current ? 1 : 0;
+    out->nx = m.nx;
+    out->ny = m.ny;
+    out->nz = 1;
+    out->n_fuel_2d = m.n_fuel;
+    out->n_group = h->prob->view().n_group;
+    out->cmfd = p.cmfd ? 1 : 0;
+    return 0;
+}
+
+int mocc_host_group_bounds(const mocc_host_problem *h, double *out, int32_t cap)
+{
+    if (!h || !h->prob) return MOCGPU_ERR_ARG;
+    const VecF &b = h->prob->mesh().mat_lib.g_bounds;
+    if ((int)b.size() > cap) return MOCGPU_ERR_CAPACITY;
+    for (size_t i = 0; i < b.size(); i++) out[i] = b[i];
+    return (int)b.size();
+}
+
+int mocc_host_pin_fuel(const mocc_host_problem *h, int32_t *out, int32_t cap)
+{
+    if (!h) return MOCGPU_ERR_ARG;
+    const CoreMesh &m = h->mesh();
+    if ((int)m.core_pins.size() > cap) return MOCGPU_ERR_CAPACITY;
+    for (size_t i = 0; i < m.core_pins.size(); i++) out[i] = m.core_pins[i]->is_fuel ? 1 : 0;
+    return (int)m.core_pins.size();
+}
+
+int mocc_host_coarse_surfaces(const mocc_host_problem *h, int32_t *out, int32_t cap)
+{
+    if (!h) return MOCGPU_ERR_ARG;
+    return guard([&] {
+        const CoreMesh &m = h->mesh();
+        const int n = m.nx * m.ny;
+        if (6 * n > cap) return (int)MOCGPU_ERR_CAPACITY;
+        static const Surface order[6] = {EAST, NORTH, WEST, SOUTH, TOP, BOTTOM};
+        for (int c = 0; c < n; c++)
+            for (int s = 0; s < 6; s++) out[6 * c + s] = m.coarse_surf(c, order[s]);
+        return n;
+    });
+}
+
 }  // extern "C"

@@ -78,6 +78,7 @@ void Problem::build(const XmlNode &doc, const std::string &base_dir)
             opts_.cmfd_resid_reduction = c->attr_double("residual_reduction", opts_.cmfd_resid_reduction);
             opts_.cmfd_max_iter = c->attr_int("max_iter", opts_.cmfd_max_iter);
             if (c->has_attr("enabled") && !c->attr_bool("enabled", true)) params_.cmfd = false;
+            params_.dump_current = c->attr_bool("dump_current", false);
         }
     }
     int n_inner = sweeper->attr_int("n_inner", -1);
@@ -93,10 +94,13 @@ void Problem::build(const XmlNode &doc, const std::string &base_dir)
         else throw Error("Unrecognized boundary update option.");
     }
     opts_.gs_boundary = params_.gauss_seidel_boundary ? 1 : 0;
+    params_.dump_fsr_flux = sweeper->attr_bool("dump_fsr_flux", false);
     const XmlNode *rays = sweeper->child("rays");
     if (!rays) throw Error("No input privided for ray spacing.");
     AngularQuadrature quad(*find_angquad(sweeper));
     ndir_oct_ = quad.ndir_oct();
+    params_.n_polar = quad.n_polar();
+    params_.n_azimuthal = quad.n_azimuthal();
     rays_ = std::make_unique<RayData>(*rays, quad, *mesh_);
     fill_view();
 }

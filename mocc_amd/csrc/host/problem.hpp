@@ -17,6 +17,9 @@ struct SolverParams {
     int n_inner = 1;
     bool gauss_seidel_boundary = true;  // <sweeper boundary_update>
     bool cmfd = false;
+    bool dump_fsr_flux = false;         // <sweeper dump_fsr_flux> (moc_sweeper.cpp:103)
+    bool dump_current = false;          // <cmfd dump_current> (cmfd.cpp:161-163)
+    int n_polar = 0, n_azimuthal = 0;   // <ang_quad> attributes, as AngularQuadrature::output writes them
 };
 
 // Reference defaults for mocgpu_options (cmfd.cpp:62-102).

@@ -60,6 +60,11 @@ class EigenState(C.Structure):
                 ("iterations", i32), ("converged", i32)]
 
 
+class HostOutputInfo(C.Structure):
+    _fields_ = [("n_polar", i32), ("n_azimuthal", i32), ("dump_fsr_flux", i32), ("dump_current", i32),
+                ("nx", i32), ("ny", i32), ("nz", i32), ("n_fuel_2d", i32), ("n_group", i32), ("cmfd", i32)]
+
+
 class HostParams(C.Structure):
     _fields_ = [("k_tol", f64), ("psi_tol", f64), ("max_iter", i32), ("min_iter", i32),
                 ("n_inner", i32), ("gs_boundary", i32), ("cmfd", i32), ("ndir_oct", i32)]
@@ -99,6 +104,10 @@ def host() -> C.CDLL:
         L.mocc_host_trace_ray.argtypes = [P, f64, f64, f64, f64, P, P, i32, P, P, i32]
         L.mocc_host_fsr_areas.argtypes = [P, P, i32]
         L.mocc_host_reload_xs.argtypes = [P, C.c_char_p]
+        L.mocc_host_get_output_info.argtypes = [P, C.POINTER(HostOutputInfo)]
+        L.mocc_host_group_bounds.argtypes = [P, P, i32]
+        L.mocc_host_pin_fuel.argtypes = [P, P, i32]
+        L.mocc_host_coarse_surfaces.argtypes = [P, P, i32]
         _host = L
     return _host
 
@@ -142,6 +151,7 @@ def gpu() -> C.CDLL:
         L.mocgpu_cmfd_flux.argtypes = [P, P]
         L.mocgpu_cmfd_matrix.argtypes = [P, i32, P, P, P, i32]
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
+        L.mocgpu_eigen_solve_history.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState), P, i32]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_plan_check.argtypes = [C.POINTER(MocgpuProblem), i32, C.c_char_p, i32]
         L.mocgpu_attenuation.restype = i32

@@ -32,6 +32,20 @@ def test_exports(header):
     assert not missing, missing
 
 
+def test_h5_exports():
+    """libmocc_h5 (loaded after the HDF5 C library it links) exports every
+    function mocc_h5.h declares."""
+    from mocc_amd import output
+    try:
+        L = output.h5lib()
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    names = declared("mocc_h5.h", "mocc_h5_")
+    assert len(names) == 5
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
 def test_bad_arguments_are_status_codes():
     from mocc_amd.native import MocgpuOptions, gpu
     o = MocgpuOptions()

@@ -1,0 +1,58 @@
+"""The driver's run on the device: EigenSolver::solve with its convergence
+history (mocgpu_eigen_solve_history), then mocc's output file
+(driver.cpp:51-57), checked against the same solve without history and the
+FP64 oracle's eigenvalue."""
+import numpy as np
+import pytest
+
+from oracle.binding import Oracle
+
+pytestmark = pytest.mark.gpu
+
+K_TOL, PSI_TOL, MAX_IT = 1e-7, 1e-6, 500
+
+
+def test_solve_history_and_output_file(tmp_path, deck_dir):
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd import output as h5
+    from mocc_amd.native import HostProblem
+    from mocc_amd.sweeper import GpuMoCSweeper
+    path = lg.write_config("c1", deck_dir)
+    xml = open(path).read().replace('<sweeper type="moc"', '<sweeper type="moc" dump_fsr_flux="t"', 1)
+    xml = xml.replace('cmfd="f"', 'cmfd="t"', 1)
+    xml = xml.replace("</solver>", '  <cmfd dump_current="t" />\n</solver>', 1)
+    deck = path[:-4] + "_out.xml"  # beside its cross-section library
+    with open(deck, "w") as f:
+        f.write(xml)
+    hp = HostProblem.load(deck)
+    assert h5.output_info(hp).cmfd == 1
+    print("c1 solve with CMFD and history", flush=True)
+    dev = GpuMoCSweeper(hp, n_inner=1)
+    out = str(tmp_path / "c1.h5")
+    rc, st, hist = h5.solve_and_write(dev, out, K_TOL, PSI_TOL, MAX_IT)
+    assert rc == 0 and st.converged
+    # the same solve without history: same iterations and bitwise the same k
+    ref = GpuMoCSweeper(hp, n_inner=1)
+    rc2, st2 = ref.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    assert rc2 == 0 and st2.iterations == st.iterations and st2.k == st.k
+    assert hist.shape == (st.iterations, 4)
+    assert hist[-1, 0] == st.k and hist[-1, 1] < K_TOL and hist[-1, 2] < PSI_TOL
+    assert np.all(np.diff(hist[:, 3]) >= 0)
+    k = h5.read(out, "convergence/k")
+    assert np.array_equal(k, hist[:, 0])
+    _, st_o, _ = Oracle(hp, n_inner=1, use_cmfd=True).eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    assert abs(k[-1] - st_o.k) < 1e-5
+    # flux/NNN is the device's pin flux, normalized
+    G = hp.n_group
+    pins = np.stack([dev.get_pin_flux_1g(g) for g in range(G)])
+    f = np.count_nonzero(pins > 0) / pins.sum()
+    got = np.stack([h5.read(out, f"flux/{g + 1:03d}").reshape(-1) for g in range(G)])
+    np.testing.assert_allclose(got, f * pins, rtol=1e-12)
+    phi = dev.flux()
+    for g in range(G):
+        assert np.array_equal(h5.read(out, f"fsr_flux/{g + 1:03d}")[:, 0], phi[:, g])
+    cur = dev.currents(0)[0]
+    east = h5.read(out, "current/1/east")
+    assert east.shape == (1, hp.p.ny, hp.p.nx) and np.abs(east).max() > 0
+    assert h5.read_string(out, "current/source") == "CMFD"
+    assert cur.size == dev.currents(1)[0].size

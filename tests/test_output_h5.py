@@ -1,0 +1,201 @@
+"""mocc's HDF5 output file (include/mocc_h5.h), written from FP64 oracle
+results on the CPU and read back through the HDF5 C library.
+
+What pins each dataset:
+  * names, ranks and shapes: the datasets MoCSweeper::output,
+    FixedSourceSolver::output, EigenSolver::output, AngularQuadrature::output
+    and CMFD::output create (moc_sweeper.cpp:534-585, fixed_source_solver.cpp:
+    119-128, eigen_solver.cpp:293-322, angular_quadrature.cpp:213-249,
+    cmfd.cpp:668-696);
+  * flux/NNN: the oracle's own pin flux (get_pin_flux_1g restated
+    separately) times mocc's Normalize factor (count of positive entries over
+    their sum), and the invariant that the entries then sum to that count;
+  * pin_powers: the synthetic libraries have Sigma_f = nu Sigma_f / 2.43 in
+    every group, so pin power is proportional to the oracle's fission source
+    times volume summed over the pin; the total equals the fuel-pin count;
+    a flat flux gives exactly 1 in every fuel pin and 0 in guide tubes;
+  * eubounds: the XSL file's group-bound line;
+  * ang_quad: unit direction vectors, omega_z = cos(theta), weights summing
+    to 8 over the sphere (angular_quadrature.hpp:37-39);
+  * current/*: a surface shared by two cells is one value (east of cell x is
+    west of cell x + 1, north of y is south of y + 1).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import DECKS  # noqa: F401  (conftest puts the repository on the path)
+
+h5 = pytest.importorskip("mocc_amd.output")
+
+
+def _have_h5():
+    try:
+        h5.h5lib()
+        return True
+    except (RuntimeError, OSError):
+        return False
+
+
+pytestmark = pytest.mark.skipif(not _have_h5(), reason="libmocc_h5 / the HDF5 C library is not available")
+
+
+def _deck(tmp_path, key, fsr=False, current=False):
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    path = lg.write_config(key, str(tmp_path))
+    xml = open(path).read()
+    if fsr:
+        xml = xml.replace('<sweeper type="moc"', '<sweeper type="moc" dump_fsr_flux="t"', 1)
+    if current:
+        xml = xml.replace('cmfd="f"', 'cmfd="t"', 1)
+        xml = xml.replace("</solver>", '  <cmfd dump_current="t" />\n</solver>', 1)
+    with open(path, "w") as f:
+        f.write(xml)
+    return HostProblem.load(path), path
+
+
+def _solve_oracle(hp, n_outer=4):
+    from oracle.binding import Oracle
+    o = Oracle(hp, n_inner=1, gs_boundary=True, energy_mode=0)
+    rc, st, khist = o.eigen_solve(1e-12, 1e-12, n_outer)
+    hist = np.stack([khist, np.full_like(khist, 1e-3), np.full_like(khist, 2e-3),
+                     np.arange(1, khist.size + 1) * 0.5], axis=1)
+    return o, hist
+
+
+def test_datasets_and_shapes(tmp_path):
+    hp, _ = _deck(tmp_path, "c1", fsr=True)
+    o, hist = _solve_oracle(hp)
+    out = str(tmp_path / "c1.h5")
+    h5.write(hp, out, o.flux(), hist)
+    G, F = hp.n_group, hp.n_fsr
+    nx, ny = hp.p.nx, hp.p.ny
+    names = set(h5.datasets(out))
+    want = {"ng", "eubounds", "pin_powers",
+            "convergence/k", "convergence/error_k", "convergence/error_psi",
+            "convergence/iteration_time", "convergence/abscissae",
+            "ang_quad/n_polar", "ang_quad/n_azimuthal", "ang_quad/alpha", "ang_quad/theta",
+            "ang_quad/omega_x", "ang_quad/omega_y", "ang_quad/omega_z", "ang_quad/weight"}
+    want |= {f"flux/{g + 1:03d}" for g in range(G)} | {f"fsr_flux/{g + 1:03d}" for g in range(G)}
+    assert names == want
+    assert h5.read(out, "ng").tolist() == [G]
+    assert h5.read(out, "eubounds").shape == (G,)
+    for g in range(G):
+        assert h5.read(out, f"flux/{g + 1:03d}").shape == (1, ny, nx)
+        fsr = h5.read(out, f"fsr_flux/{g + 1:03d}")
+        assert fsr.shape == (F, 1)
+        assert np.array_equal(fsr[:, 0], o.flux()[:, g])  # not normalized (moc_sweeper.cpp:557-559)
+    assert h5.read(out, "pin_powers").shape == (1, ny, nx)
+    n = hist.shape[0]
+    for i, name in enumerate(("k", "error_k", "error_psi", "iteration_time")):
+        v = h5.read(out, f"convergence/{name}")
+        assert v.shape == (n,) and np.array_equal(v, hist[:, i])
+    assert np.array_equal(h5.read(out, "convergence/abscissae"), hist[:, 3])
+
+
+def test_no_optional_groups_by_default(tmp_path):
+    hp, _ = _deck(tmp_path, "c1")
+    o, hist = _solve_oracle(hp, 2)
+    out = str(tmp_path / "plain.h5")
+    h5.write(hp, out, o.flux(), hist)
+    names = h5.datasets(out)
+    assert not any(n.startswith("fsr_flux/") or n.startswith("current/") for n in names)
+
+
+def test_pin_flux_is_normalized_oracle_pin_flux(tmp_path):
+    hp, _ = _deck(tmp_path, "c1")
+    o, hist = _solve_oracle(hp)
+    out = str(tmp_path / "c1.h5")
+    h5.write(hp, out, o.flux(), hist)
+    G, nx, ny = hp.n_group, hp.p.nx, hp.p.ny
+    pins = np.stack([o.pin_flux(g) for g in range(G)])      # [G, cells], cell = x + nx*y
+    f = np.count_nonzero(pins > 0) / pins.sum()
+    got = np.stack([h5.read(out, f"flux/{g + 1:03d}").reshape(-1) for g in range(G)])
+    np.testing.assert_allclose(got, f * pins, rtol=1e-13, atol=0)
+    assert got.sum() == pytest.approx(np.count_nonzero(got > 0), rel=1e-13)
+    # the reflective 3x3 lattice is symmetric: corners agree, edges agree
+    p = got[0].reshape(ny, nx)
+    assert p[0, 0] == pytest.approx(p[2, 2], rel=1e-9) and p[0, 1] == pytest.approx(p[1, 0], rel=1e-9)
+
+
+def test_pin_powers_follow_the_fission_source(tmp_path):
+    hp, _ = _deck(tmp_path, "c1")
+    o, hist = _solve_oracle(hp)
+    out = str(tmp_path / "c1.h5")
+    h5.write(hp, out, o.flux(), hist)
+    fs = np.asarray(o.calc_fission_source(1.0))            # (1/k) sum_g nu Sigma_f phi
+    vol = hp.array("fsr_vol", hp.n_fsr)
+    off = hp.array("pin_fsr_off", hp.p.nx * hp.p.ny + 1, np.int32)
+    cell = hp.array("pin_cell", hp.p.nx * hp.p.ny, np.int32)
+    rate = np.zeros(hp.p.nx * hp.p.ny)
+    for i in range(cell.size):
+        rate[cell[i]] += float(np.sum(fs[off[i]:off[i + 1]] * vol[off[i]:off[i + 1]]))
+    info = h5.output_info(hp)
+    want = rate * info.n_fuel_2d / rate.sum()
+    got = h5.read(out, "pin_powers").reshape(-1)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+    assert got.sum() == pytest.approx(info.n_fuel_2d, rel=1e-13)
+
+
+def test_flat_flux_closed_forms(tmp_path, deck_dir):
+    """17x17 with guide tubes, flat flux: every pin flux is 1 after Normalize,
+    every fuel pin has power 1 and every guide tube 0."""
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    hp = HostProblem.load(lg.write_deck(lg.pin_lattice(17, 2, 4, 0.1, "h5_asm17", guide_tubes=True), deck_dir, 2))
+    F, G = hp.n_fsr, hp.n_group
+    out = str(tmp_path / "flat.h5")
+    h5.write(hp, out, np.ones((F, G)), np.zeros((0, 4)))
+    for g in range(G):
+        np.testing.assert_allclose(h5.read(out, f"flux/{g + 1:03d}"), 1.0, rtol=1e-14)
+    pw = h5.read(out, "pin_powers").reshape(-1)
+    info = h5.output_info(hp)
+    assert info.n_fuel_2d == 289 - 25
+    assert np.count_nonzero(pw == 0.0) == 25
+    np.testing.assert_allclose(pw[pw > 0], 1.0, rtol=1e-12)
+    assert h5.read(out, "convergence/k").shape == (0,)
+
+
+def test_eubounds_and_quadrature(tmp_path):
+    hp, path = _deck(tmp_path, "c1")
+    o, hist = _solve_oracle(hp, 1)
+    out = str(tmp_path / "q.h5")
+    h5.write(hp, out, o.flux(), hist)
+    xsl = os.path.join(os.path.dirname(path), [f for f in os.listdir(os.path.dirname(path)) if f.endswith(".xsl")
+                                               and os.path.basename(path)[:-4] in f][0])
+    bounds = [float(v) for v in open(xsl).read().split("\n")[2].split()]
+    np.testing.assert_array_equal(h5.read(out, "eubounds"), bounds)
+    a, t = h5.read(out, "ang_quad/alpha"), h5.read(out, "ang_quad/theta")
+    ox, oy, oz = (h5.read(out, f"ang_quad/omega_{c}") for c in "xyz")
+    w = h5.read(out, "ang_quad/weight")
+    assert a.size == t.size == w.size == 8 * hp.params().ndir_oct
+    np.testing.assert_allclose(ox ** 2 + oy ** 2 + oz ** 2, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(oz, np.cos(t), atol=1e-12)
+    assert w.sum() == pytest.approx(8.0, rel=1e-12)
+    assert h5.read(out, "ang_quad/n_polar").tolist() == [3]
+    assert h5.read(out, "ang_quad/n_azimuthal").tolist() == [h5.output_info(hp).n_azimuthal]
+
+
+def test_cmfd_currents_layout(tmp_path):
+    hp, _ = _deck(tmp_path, "c1", current=True)
+    from oracle.binding import Oracle
+    o = Oracle(hp, n_inner=1, gs_boundary=True, energy_mode=0)
+    o.initialize()
+    o.calc_fission_source(1.0)
+    o.fss_step(with_current=True)
+    G, nx, ny = hp.n_group, hp.p.nx, hp.p.ny
+    cur = np.stack([o.currents(g)[0] for g in range(G)])
+    out = str(tmp_path / "cur.h5")
+    assert h5.output_info(hp).cmfd == 1
+    with pytest.raises(RuntimeError):
+        h5.write(hp, out, o.flux(), np.zeros((0, 4)))  # dump_current needs the currents
+    h5.write(hp, out, o.flux(), np.zeros((0, 4)), cur)
+    assert h5.read_string(out, "current/source") == "CMFD"
+    for g in range(G):
+        d = {s: h5.read(out, f"current/{g + 1}/{s}")[0] for s in ("east", "north", "west", "south", "top", "bottom")}
+        assert all(v.shape == (ny, nx) for v in d.values())
+        np.testing.assert_array_equal(d["east"][:, :-1], d["west"][:, 1:])
+        np.testing.assert_array_equal(d["north"][:-1, :], d["south"][1:, :])
+        assert np.abs(d["east"]).max() > 0
