@@ -187,6 +187,8 @@ def bench_c3(work_dir: str, with_oracle: bool):
            "n_fsr": hp.n_fsr, "n_seg": hp.p.n_seg, "k": st.k, "outer_iterations": st.iterations,
            "converged": bool(st.converged), "solve_s": t_solve, "trace_s": t_trace, "plan_upload_s": t_plan,
            "updates_per_s": dev.updates_per_sweep() * par.n_inner * st.iterations / t_solve,
+           "last_cmfd_power_iterations": dev.cmfd_iterations(),
+           "last_cmfd_bicgstab_iterations": dev.cmfd_bicgstab_iterations(),
            "k_tol": par.k_tol, "psi_tol": par.psi_tol}
     dev.close()
     if with_oracle:

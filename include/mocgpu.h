@@ -248,6 +248,10 @@ int mocgpu_cmfd_step(mocgpu_handle *h, double *k);
 
 /* Outer (coarse power) iterations of the last CMFD solve. */
 int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h);
+/* BiCGSTAB iterations of the last CMFD solve, summed over its power
+ * iterations and groups (each group solve stops at the tolerance of
+ * cmfd.cpp:202-204 or at 150 iterations, cmfd.cpp:547). */
+int32_t mocgpu_cmfd_bicgstab_iterations(const mocgpu_handle *h);
 
 /* CMFD::flux() (src/core/cmfd.hpp:76-79): the coarse flux of the last solve,
  * [n_cell][G] with n_cell = nx * ny. */

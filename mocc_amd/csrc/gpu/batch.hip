@@ -82,18 +82,26 @@ __global__ void k_b_pre(const CaseDesc *__restrict__ cs, const BlockMap *__restr
     }
 }
 
-__device__ __forceinline__ void block_sum_n(double *acc, int n, double (*sh)[4])
+constexpr int kPostT = 1024;  // threads of k_b_post: one workgroup per problem
+constexpr int kPostWaves = kPostT / 64;
+
+// Sum of n values over the workgroup in a fixed order: an xor butterfly in
+// each wave, then every lane adds the wave sums in wave order. red holds
+// kPostWaves x 4 doubles; the leading barrier lets a caller reuse it.
+__device__ __forceinline__ void block_sum_n(double *acc, int n, double (*red)[4])
 {
-    const int t = threadIdx.x;
+    for (int q = 0; q < n; q++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
     __syncthreads();
-    for (int q = 0; q < n; q++) sh[t][q] = acc[q];
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < n; q++) red[threadIdx.x >> 6][q] = acc[q];
     __syncthreads();
-    for (int w = kBT / 2; w > 0; w >>= 1) {
-        if (t < w)
-            for (int q = 0; q < n; q++) sh[t][q] += sh[t + w][q];
-        __syncthreads();
+    for (int q = 0; q < n; q++) {
+        double s = 0.0;
+        for (int w = 0; w < kPostWaves; w++) s += red[w][q];
+        acc[q] = s;
     }
-    for (int q = 0; q < n; q++) acc[q] = sh[0][q];
 }
 
 // The end of an outer iteration, one workgroup per problem (fixed reduction
@@ -102,16 +110,16 @@ __device__ __forceinline__ void block_sum_n(double *acc, int n, double (*sh)[4])
 // err_psi (eigen_solver.cpp:171-184) and EigenSolver::solve's test:
 // status[2c] counts steps, status[2c+1] becomes 1 (converged), 2 (max_iter)
 // or 3 (k not a number).
-__global__ __launch_bounds__(kBT) void k_b_post(const CaseDesc *__restrict__ cs, int32_t *__restrict__ status)
+__global__ __launch_bounds__(kPostT) void k_b_post(const CaseDesc *__restrict__ cs, int32_t *__restrict__ status)
 {
     const int c = blockIdx.x;
     if (finished(status, c)) return;
     const CaseDesc &d = cs[c];
-    __shared__ double sh[kBT][4];
+    __shared__ double sh[kPostWaves][4];
     const int G = d.G;
     double *s = d.scal;
     double acc[4] = {0, 0, 0, 0};
-    for (int f = threadIdx.x; f < d.F; f += kBT) {
+    for (int f = threadIdx.x; f < d.F; f += kPostT) {
         const int m = d.fsr_mat[f];
         for (int g = 0; g < G; g++) {
             acc[0] += d.phi[(size_t)f * G + g] * d.vol[f] * d.mat_nf[m * G + g];
@@ -123,7 +131,7 @@ __global__ __launch_bounds__(kBT) void k_b_post(const CaseDesc *__restrict__ cs,
     const double k = k_prev * acc[0] / acc[1];
     const double rk = 1.0 / k;
     double nrm[4] = {0, 0, 0, 0};
-    for (int f = threadIdx.x; f < d.F; f += kBT) {
+    for (int f = threadIdx.x; f < d.F; f += kPostT) {
         const int m = d.fsr_mat[f];
         double v = 0.0;
         for (int g = 0; g < G; g++) v += rk * d.mat_nf[m * G + g] * d.phi[(size_t)f * G + g];
@@ -137,7 +145,7 @@ __global__ __launch_bounds__(kBT) void k_b_post(const CaseDesc *__restrict__ cs,
     block_sum_n(nrm, 4, sh);
     const double fa = nrm[0] / nrm[1], fb = nrm[2] / nrm[3];
     double e2[1] = {0.0};
-    for (int f = threadIdx.x; f < d.F; f += kBT) {
+    for (int f = threadIdx.x; f < d.F; f += kPostT) {
         const double e = d.fs[f] * fa - d.fs_prev[f] * fb;
         e2[0] += e * e;
     }
@@ -341,7 +349,7 @@ struct mocgpu_batch {
             }
             launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
         }
-        hipLaunchKernelGGL(k_b_post, dim3(n), dim3(kBT), 0, s, d_desc.p, d_status.p);
+        hipLaunchKernelGGL(k_b_post, dim3(n), dim3(kPostT), 0, s, d_desc.p, d_status.p);
     }
 
     void run(int n_outer)

@@ -278,9 +278,11 @@ __global__ void k_set_pin_flux_all(CmfdView v, const double *cflux, int *bad)
     for (int r = v.pin_off[ip]; r < v.pin_off[ip + 1]; r++) v.phi[(size_t)r * v.G + g] *= f;
 }
 
+// info[0]: coarse power iterations, info[1]: non-positive flux flag of the
+// rescale, info[2]: BiCGSTAB iterations summed over the solve
 __global__ void k_zero_info(int *info)
 {
-    if (threadIdx.x < 2) info[threadIdx.x] = 0;
+    if (threadIdx.x < 3) info[threadIdx.x] = 0;
 }
 
 // --- the coarse power iteration ------------------------------------------------
@@ -378,31 +380,33 @@ __device__ __forceinline__ void spmv(const SolveArgs &A, int g, const Lane<K> &L
 // restart when rho vanishes (van der Vorst 1992, the iteration of Eigen's
 // BiCGSTAB<SparseMatrix<double>> that cmfd.cpp:291 calls). The preconditioned
 // directions y = M^-1 p and z = M^-1 s are read back from the product's LDS
-// operand, so x is updated as (x + alpha y) + w z.
+// operand, so x is updated as (x + alpha y) + w z. The loop-invariant vectors
+// (the inverse diagonal, the shadow residual r0 and b) live in LDS slots that
+// only the lane owning the cell touches, which keeps the rest in registers.
 template <int K>
-__device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, const double (&b)[K], double (&x)[K],
-                        double tol, int max_iter, double *buf, double *red)
+__device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)[K], double tol, int max_iter,
+                        double *buf, double *red, double *sinv, double *sr0, const double *sb)
 {
     const size_t pl = (size_t)A.G * A.nc;
-    double inv[K], r[K], r0[K], p[K], v[K], t[K];
+    double r[K], p[K], v[K], t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        inv[k] = 1.0;
         if (L.on[k]) {
             const double d = A.coef[2 * pl + (size_t)g * A.nc + L.c[k]];
-            if (d != 0.0) inv[k] = 1.0 / d;
+            sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
         }
     }
     spmv<K>(A, g, L, x, t, buf);
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        r[k] = b[k] - t[k];
-        r0[k] = r[k];
+        const double bk = L.on[k] ? sb[L.c[k]] : 0.0;
+        r[k] = bk - t[k];
+        if (L.on[k]) sr0[L.c[k]] = r[k];
         p[k] = 0.0;
         v[k] = 0.0;
-        lr0 += r0[k] * r0[k];
-        lb += b[k] * b[k];
+        lr0 += r[k] * r[k];
+        lb += bk * bk;
     }
     double r0_sq, rhs_sq;
     block_sum2(lr0, lb, red, r0_sq, rhs_sq);
@@ -421,7 +425,7 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, const doubl
 #pragma unroll
         for (int k = 0; k < K; k++) {
             lrr += r[k] * r[k];
-            lrho += r0[k] * r[k];
+            if (L.on[k]) lrho += sr0[L.c[k]] * r[k];
         }
         double rr, rho_new;
         block_sum2(lrr, lrho, red, rr, rho_new);
@@ -433,8 +437,8 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, const doubl
             double l = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                r[k] = b[k] - t[k];
-                r0[k] = r[k];
+                r[k] = (L.on[k] ? sb[L.c[k]] : 0.0) - t[k];
+                if (L.on[k]) sr0[L.c[k]] = r[k];
                 l += r[k] * r[k];
             }
             rho = r0_sq = block_sum(l, red);
@@ -444,18 +448,19 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, const doubl
 #pragma unroll
         for (int k = 0; k < K; k++) {
             p[k] = r[k] + beta * (p[k] - w * v[k]);
-            t[k] = inv[k] * p[k];  // y
+            t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * p[k];  // y
         }
         spmv<K>(A, g, L, t, v, buf);
         double l = 0.0;
 #pragma unroll
-        for (int k = 0; k < K; k++) l += r0[k] * v[k];
+        for (int k = 0; k < K; k++)
+            if (L.on[k]) l += sr0[L.c[k]] * v[k];
         alpha = rho / block_sum(l, red);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (L.on[k]) x[k] += alpha * buf[L.c[k]];
-            r[k] = r[k] - alpha * v[k];  // s
-            t[k] = inv[k] * r[k];        // z
+            r[k] = r[k] - alpha * v[k];                     // s
+            t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * r[k];  // z
         }
         spmv<K>(A, g, L, t, t, buf);  // t = A z
         double ltt = 0.0, lts = 0.0;
@@ -533,8 +538,11 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Lan
 template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(SolveArgs A)
 {
     extern __shared__ double lds[];
-    double *buf = lds;            // [nc] operand of the matrix-vector products
-    double *fs_old = lds + A.nc;  // [nc] previous coarse fission source
+    double *buf = lds;                // [nc] operand of the matrix-vector products
+    double *fs_old = lds + A.nc;      // [nc] previous coarse fission source
+    double *sinv = lds + 2 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
+    double *sr0 = lds + 3 * A.nc;     // [nc] BiCGSTAB: shadow residual
+    double *sb = lds + 4 * A.nc;      // [nc] BiCGSTAB: right-hand side
     __shared__ double red[2 * kCmfdWaves];
     Lane<K> L;
 #pragma unroll
@@ -553,7 +561,7 @@ template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(So
     }
     r0 = sqrt(r0) / (G * A.nc);
     const double tol = A.rr * r0;
-    int iter = 0;
+    int iter = 0, n_bicg = 0;
     while (true) {
         iter++;
 #pragma unroll
@@ -565,7 +573,10 @@ template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(So
         for (int g = 0; g < G; g++) {
             cmfd_source<K>(A, g, L, fsc, b);
             ri += block_sum(residual2<K>(A, g, L, b, x, t, buf), red);
-            bicgstab<K>(A, g, L, b, x, tol, 150, buf, red);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (L.on[k]) sb[L.c[k]] = b[k];
+            n_bicg += bicgstab<K>(A, g, L, x, tol, 150, buf, red, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
@@ -597,6 +608,7 @@ template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(So
     if (threadIdx.x == 0) {
         A.k[0] = k_eff;
         A.info[0] = iter;
+        A.info[2] = n_bicg;
     }
 }
 
@@ -642,6 +654,16 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     for (int i = 0; i < 6; i++)
         if (bc_[i] != MOCGPU_BC_VACUUM && bc_[i] != MOCGPU_BC_REFLECT)
             throw std::runtime_error("CMFD supports vacuum and reflective boundaries only");
+    {
+        // the solve keeps five cell vectors in LDS (k_cmfd_solve)
+        int dev = 0, lds_max = 0;
+        CMFD_CHECK(hipGetDevice(&dev));
+        CMFD_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t need = 5 * (size_t)nc_ * sizeof(double) + 4 * kCmfdWaves * sizeof(double);
+        if (need > (size_t)lds_max)
+            throw std::runtime_error("CMFD mesh of " + std::to_string(nc_) + " cells needs " + std::to_string(need) +
+                                     " bytes of LDS per workgroup; the device allows " + std::to_string(lds_max));
+    }
     std::vector<double> dx(nx_), dy(ny_), cv(nc_);
     for (int i = 0; i < nx_; i++) dx[i] = p.x_vec[i + 1] - p.x_vec[i];
     for (int j = 0; j < ny_; j++) dy[j] = p.y_vec[j + 1] - p.y_vec[j];
@@ -665,7 +687,7 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     dalloc(d_dhat_, (size_t)n_surf_ * G_);
     dalloc(d_shat_, (size_t)n_surf_ * G_);
     dalloc(d_coef_, 5 * ncg);
-    dalloc(d_info_, 2);
+    dalloc(d_info_, 3);
     initialize(s);
 }
 
@@ -722,7 +744,7 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     a.cflux = d_cflux_;
     a.k = v.k;
     a.info = d_info_;
-    const size_t lds = 2 * (size_t)nc_ * sizeof(double);
+    const size_t lds = 5 * (size_t)nc_ * sizeof(double);  // buf, fs_old and the BiCGSTAB vectors
     const int kneed = (nc_ + kCmfdThreads - 1) / kCmfdThreads;
     if (kneed <= 1) hipLaunchKernelGGL(k_cmfd_solve<1>, dim3(1), dim3(kCmfdThreads), lds, s, a);
     else if (kneed <= 2) hipLaunchKernelGGL(k_cmfd_solve<2>, dim3(1), dim3(kCmfdThreads), lds, s, a);
@@ -734,15 +756,16 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     n_solve_++;
 }
 
-void CmfdSolver::read_info(hipStream_t s, int *iterations, int *bad_flux)
+void CmfdSolver::read_info(hipStream_t s, int *iterations, int *bad_flux, int *bicgstab_iterations)
 {
-    int info[2] = {0, 0};
+    int info[3] = {0, 0, 0};
     if (d_info_) {
         CMFD_CHECK(hipMemcpyAsync(info, d_info_, sizeof(info), hipMemcpyDeviceToHost, s));
         CMFD_CHECK(hipStreamSynchronize(s));
     }
     if (iterations) *iterations = info[0];
     if (bad_flux) *bad_flux = info[1];
+    if (bicgstab_iterations) *bicgstab_iterations = info[2];
 }
 
 void CmfdSolver::matrix_csr(int g, hipStream_t s, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,

@@ -38,7 +38,9 @@ public:
     void enqueue_step(const CmfdView &v, hipStream_t s);
     // Outer iterations of the last solve and the non-positive-flux flag of
     // the last rescale (read back; synchronizes s).
-    void read_info(hipStream_t s, int *iterations, int *bad_flux);
+    // iterations of the last solve: coarse power iterations, the rescale's
+    // non-positive flux flag, BiCGSTAB iterations summed over all groups
+    void read_info(hipStream_t s, int *iterations, int *bad_flux, int *bicgstab_iterations = nullptr);
 
     int n_cell() const { return nc_; }
     // Assembled coefficients of group g after the last solve, rows in cell

@@ -100,7 +100,7 @@ struct mocgpu_handle {
     uint64_t parity = 0;
     bool has_fs = false;
     bool has_radial = false;
-    int cmfd_iterations = 0;
+    int cmfd_iterations = 0, cmfd_bicgstab = 0;
     double tuned_ms = 0.0;  // sweep kernel time of the autotuned lane map
 
     // plan
@@ -286,7 +286,7 @@ struct mocgpu_handle {
     {
         cmfd->enqueue_step(cmfd_view(), stream);
         int bad = 0;
-        cmfd->read_info(stream, iterations, &bad);
+        cmfd->read_info(stream, iterations, &bad, &cmfd_bicgstab);
         if (bad) {
             err = "Negative or zero input flux in the CMFD rescale";
             return MOCGPU_ERR_NONPOS_FLUX;

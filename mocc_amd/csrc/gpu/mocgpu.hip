@@ -941,7 +941,7 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
         h->sync();
         if (h->opt.use_cmfd) {
             int bad = 0;
-            h->cmfd->read_info(h->stream, &h->cmfd_iterations, &bad);
+            h->cmfd->read_info(h->stream, &h->cmfd_iterations, &bad, &h->cmfd_bicgstab);
             if (bad) {
                 h->err = "Negative or zero input flux in the CMFD rescale";
                 return MOCGPU_ERR_NONPOS_FLUX;
@@ -1020,6 +1020,8 @@ int mocgpu_cmfd_step(mocgpu_handle *h, double *k)
 }
 
 int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h) { return h->cmfd_iterations; }
+
+int32_t mocgpu_cmfd_bicgstab_iterations(const mocgpu_handle *h) { return h ? h->cmfd_bicgstab : 0; }
 
 int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
 {

@@ -185,12 +185,20 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const double sx = ldexp(1.0, kx);
     int kc = 0, ep = 0, ee = 0;
     double sc = 1.0;
+    // the lane's next coarse-surface event, held in registers so that steps
+    // without one do not wait on a load; nstep = 0x10000 is past every step
+    uint32_t nev = 0u;
+    int nstep = 0x10000;
     if (CUR) {
         kc = fixed_exp(psi_bound * (double)cwsum * 1.0001, (double)B.max_surf_contrib);
         sc = ldexp(1.0, kc);
         if (active) {
             ep = A.ev_off[2 * (B.ray_base + j) + dir];
             ee = A.ev_off[2 * (B.ray_base + j) + dir + 1];
+            if (ep < ee) {
+                nev = A.ev[ep];
+                nstep = (int)(nev & 0xFFFFu);
+            }
         }
     }
     const float csign = dir ? -1.0f : 1.0f;
@@ -269,9 +277,14 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         for (int u = 0; u < kChunk; u++) {
             if (CUR) {
                 const int t = c * kChunk + u;
-                while (ep < ee && (int)(A.ev[ep] & 0xFFFFu) == t) {
-                    tally_current(A.ev[ep]);
+                while (nstep == t) {
+                    tally_current(nev);
                     ep++;
+                    nstep = 0x10000;
+                    if (ep < ee) {
+                        nev = A.ev[ep];
+                        nstep = (int)(nev & 0xFFFFu);
+                    }
                 }
             }
             float cacc = 0.0f;
@@ -285,9 +298,15 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         }
     }
     if (CUR && active) {
-        while (ep < ee) {
-            tally_current(A.ev[ep]);
+        // events at the exit step (after the last segment)
+        while (nstep != 0x10000) {
+            tally_current(nev);
             ep++;
+            nstep = 0x10000;
+            if (ep < ee) {
+                nev = A.ev[ep];
+                nstep = (int)(nev & 0xFFFFu);
+            }
         }
     }
     if (active) {

@@ -148,6 +148,8 @@ def gpu() -> C.CDLL:
         L.mocgpu_cmfd_step.argtypes = [P, P]
         L.mocgpu_cmfd_iterations.restype = i32
         L.mocgpu_cmfd_iterations.argtypes = [P]
+        L.mocgpu_cmfd_bicgstab_iterations.restype = i32
+        L.mocgpu_cmfd_bicgstab_iterations.argtypes = [P]
         L.mocgpu_cmfd_flux.argtypes = [P, P]
         L.mocgpu_cmfd_matrix.argtypes = [P, i32, P, P, P, i32]
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]

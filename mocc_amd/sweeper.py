@@ -167,6 +167,10 @@ class GpuMoCSweeper:
     def cmfd_iterations(self) -> int:
         return int(gpu().mocgpu_cmfd_iterations(self.h))
 
+    def cmfd_bicgstab_iterations(self) -> int:
+        """BiCGSTAB iterations of the last CMFD solve (all groups)."""
+        return int(gpu().mocgpu_cmfd_bicgstab_iterations(self.h))
+
     def cmfd_flux(self) -> np.ndarray:
         """CMFD::flux(): coarse flux [n_cell, G] of the last solve."""
         out = np.zeros((self.npin, self.G))
