@@ -295,8 +295,10 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
  * Every unfinished problem advances by whole outer iterations of
  * EigenSolver::step (eigen_solver.cpp:225-253) with Jacobi energy sweeps and
  * the handle's n_inner; the convergence test is EigenSolver::solve's
- * (k_tol, psi_tol, min_iter, max_iter). A problem's result is bitwise the same
- * whichever other problems share the batch. CMFD is not used in a batch.
+ * (k_tol, psi_tol, min_iter, max_iter). A handle with CMFD (options.use_cmfd)
+ * runs its CMFD step at the start of every outer iteration, as do_cmfd does,
+ * and tallies currents on its last inner sweep. A problem's result is bitwise
+ * the same whichever other problems share the batch.
  * ------------------------------------------------------------------------- */
 typedef struct mocgpu_batch mocgpu_batch;
 
@@ -318,8 +320,10 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
 int mocgpu_batch_run(mocgpu_batch *b, int32_t n_outer);
 
 /* Wait for the enqueued work; report problems that finished since the last
- * poll (converged, max_iter or NaN), up to cap. Returns the count. The flux of
- * a finished problem stays readable with mocgpu_get_flux on its handle. */
+ * poll (converged, max_iter, k not a number, or a non-positive coarse flux in
+ * the CMFD rescale), up to cap; state.converged is 1 only for the first.
+ * Returns the count. The flux of a finished problem stays readable with
+ * mocgpu_get_flux on its handle. */
 int mocgpu_batch_poll(mocgpu_batch *b, int32_t *ids, mocgpu_eigen_state *states, int32_t cap);
 
 /* Remove a problem (finished or not); its handle is the caller's again. */

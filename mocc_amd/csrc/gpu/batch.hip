@@ -12,6 +12,11 @@
 //   k_b_post: total fission and k, normalization, err_psi and the
 //             convergence test, one workgroup per problem.
 //
+// Problems with CMFD first run the CMFD step (CmfdBatch: coarse flux,
+// homogenization, coupling, assembly, one solve workgroup per problem,
+// currents and the pin rescale), and their last inner sweep tallies
+// coarse-surface currents, reduced per problem before k_b_post.
+//
 // The sequence is captured once per batch composition into a HIP graph and
 // replayed; problems that have converged stay in the graph and are skipped by
 // every kernel from their status word. Each problem's arithmetic (fixed-point
@@ -32,6 +37,7 @@ namespace {
 struct CaseDesc {
     int32_t F, G, max_iter, min_iter;
     double k_tol, psi_tol;
+    const int *cmfd_bad;  // the CMFD rescale's non-positive flux flag, or null
     const int32_t *fsr_mat;
     const double *mat_nf, *mat_ch, *mat_sc, *vol;
     double *phi, *phi_old, *src, *fs, *fs_prev;
@@ -127,6 +133,7 @@ __global__ __launch_bounds__(kPostT) void k_b_post(const CaseDesc *__restrict__ 
         }
     }
     block_sum_n(acc, 2, sh);
+    // keff_prev_ is k after the step's CMFD solve (eigen_solver.cpp:246-247)
     const double k_prev = s[S_K];
     const double k = k_prev * acc[0] / acc[1];
     const double rk = 1.0 / k;
@@ -161,7 +168,8 @@ __global__ __launch_bounds__(kPostT) void k_b_post(const CaseDesc *__restrict__ 
     s[S_ERRPSI] = sqrt(e2[0] / s[S_NFISS]);
     const int it = ++status[2 * c];
     int fin = 0;
-    if (!(k == k)) fin = 3;
+    if (d.cmfd_bad && *d.cmfd_bad) fin = 4;  // set_pin_flux_1g throws (moc_sweeper.cpp:399-402)
+    else if (!(k == k)) fin = 3;
     else if (s[S_ERRK] < d.k_tol && s[S_ERRPSI] < d.psi_tol && it - 1 >= d.min_iter) fin = 1;
     else if (it >= d.max_iter) fin = 2;
     status[2 * c + 1] = fin;
@@ -200,12 +208,17 @@ struct mocgpu_batch {
     DBuf<SweepArgs> d_sweep;
     DBuf<ReduceArgs> d_red;
     DBuf<int4> d_items;
-    DBuf<BlockMap> d_bmF, d_bmR;
-    DBuf<int32_t> d_status, d_ninner;
+    DBuf<BlockMap> d_bmF, d_bmR, d_bmJ;
+    DBuf<int32_t> d_status, d_ninner, d_wantcur;
+    DBuf<CurrentArgs> d_cura;
     DBuf<uint32_t> d_counter;
-    int n_bmF = 0, n_bmR = 0, n_items = 0, max_inner = 1, q_blocks = 1;
+    int n_bmF = 0, n_bmR = 0, n_bmJ = 0, n_items = 0, max_inner = 1, q_blocks = 1, q_blocks_cur = 1;
     int phase_first[2] = {0, 0}, phase_items[2] = {0, 0};
-    size_t lds = 0;
+    size_t lds = 0, lds_cur = 0;
+    // CMFD problems: their CMFD step runs first in every outer iteration, and
+    // the inner iterations marked in cur_inner tally currents
+    CmfdBatch cmfd;
+    std::vector<char> cur_inner;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -244,13 +257,17 @@ struct mocgpu_batch {
         std::vector<SweepArgs> sw(n);
         std::vector<ReduceArgs> rd(n);
         std::vector<int32_t> st(2 * n), ni(n);
-        std::vector<size_t> cF(n), cR(n);
+        std::vector<size_t> cF(n), cR(n), cJ(n, 0);
+        std::vector<int32_t> wc(n, 0);
+        std::vector<CurrentArgs> ja(n);
+        std::vector<CmfdBatchEntry> cm;
         struct It {
             int64_t cost;
             int4 w;
         };
         std::vector<It> items[2];
         lds = 0;
+        lds_cur = 0;
         max_inner = 1;
         for (int i = 0; i < n; i++) {
             mocgpu_handle *h = cases[i].h;
@@ -272,11 +289,26 @@ struct mocgpu_batch {
             d.fs = h->d_fs.p;
             d.fs_prev = h->d_fs_prev.p;
             d.scal = h->d_scal.p;
+            d.cmfd_bad = nullptr;
             SweepArgs a = h->sweep_args(-1);
             a.parity_dev = h->d_parity.p;
-            a.lds_lsurf = 0;
+            if (h->opt.use_cmfd) {
+                // CMFD: currents on the last inner iteration, the CMFD step first
+                wc[i] = 1;
+                d.cmfd_bad = h->cmfd->bad_flux_flag();
+                lds_cur = std::max(lds_cur, sweep_lds_bytes(a));
+                ja[i] = h->current_args(-1);
+                ja[i].radial_flag = h->d_radial.p;
+                cJ[i] = (size_t)(ja[i].n1 - ja[i].n0) * h->G;
+                cm.push_back(CmfdBatchEntry{i, h->cmfd.get(), h->cmfd_view(), h->d_radial.p});
+            } else {
+                a.lds_lsurf = 0;
+            }
             sw[i] = a;
-            lds = std::max(lds, sweep_lds_bytes(a));
+            SweepArgs plain = a;
+            plain.lds_lsurf = 0;
+            lds = std::max(lds, sweep_lds_bytes(plain));
+            lds_cur = std::max(lds_cur, sweep_lds_bytes(plain));
             rd[i] = h->reduce_args(-1);
             // Gauss-Seidel problems keep one boundary buffer: nothing to flip
             rd[i].parity = h->d_parity.p;
@@ -307,9 +339,14 @@ struct mocgpu_batch {
             phase_items[ph] = (int)items[ph].size();
         }
         n_items = (int)iv.size();
-        auto bF = block_map(cF), bR = block_map(cR);
+        auto bF = block_map(cF), bR = block_map(cR), bJ = block_map(cJ);
         n_bmF = (int)bF.size();
         n_bmR = (int)bR.size();
+        n_bmJ = (int)bJ.size();
+        // inner iterations on which some CMFD problem tallies currents (its last)
+        cur_inner.assign(max_inner, 0);
+        for (int i = 0; i < n; i++)
+            if (wc[i]) cur_inner[ni[i] - 1] = 1;
         d_desc.upload(desc, stream);
         d_sweep.upload(sw, stream);
         d_red.upload(rd, stream);
@@ -318,11 +355,16 @@ struct mocgpu_batch {
         d_bmR.upload(bR, stream);
         d_status.upload(st, stream);
         d_ninner.upload(ni, stream);
+        d_wantcur.upload(wc, stream);
+        d_cura.upload(ja, stream);
+        d_bmJ.upload(bJ, stream);
+        cmfd.build(cm, stream);
         d_counter.alloc(2 * (size_t)max_inner);  // one work-queue counter per (inner, phase)
         int dev = 0, ncu = 0;
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        q_blocks = std::max(1, ncu * queue_blocks_per_cu(NP, lds));
+        q_blocks = std::max(1, ncu * queue_blocks_per_cu(NP, lds, false));
+        q_blocks_cur = cm.empty() ? q_blocks : std::max(1, ncu * queue_blocks_per_cu(NP, lds_cur, true));
         sync();
         dirty = false;
     }
@@ -332,9 +374,12 @@ struct mocgpu_batch {
     {
         const int n = (int)cases.size();
         hipStream_t s = stream;
+        // EigenSolver::step: do_cmfd first (eigen_solver.cpp:228-230)
+        cmfd.enqueue(d_status.p, s);
         hipLaunchKernelGGL(k_b_pre, dim3(n_bmF), dim3(kBT), 0, s, d_desc.p, d_bmF.p, d_status.p, d_counter.p,
                            2 * max_inner);
         for (int inner = 0; inner < max_inner; inner++) {
+            const bool cur = cur_inner[inner] != 0;
             for (int ph = 0; ph < 2; ph++) {
                 if (phase_items[ph] == 0) continue;
                 QueueArgs q{};
@@ -345,9 +390,12 @@ struct mocgpu_batch {
                 q.counter = d_counter.p + 2 * inner + ph;
                 q.status = d_status.p;
                 q.n_inner = d_ninner.p;
-                launch_sweep_queue(q, NP, lds, std::min(q_blocks, phase_items[ph]), s);
+                q.want_cur = d_wantcur.p;
+                launch_sweep_queue(q, NP, cur ? lds_cur : lds, std::min(cur ? q_blocks_cur : q_blocks, phase_items[ph]),
+                                   s, cur);
             }
             launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
+            if (cur) launch_reduce_current_batch(d_cura.p, d_bmJ.p, n_bmJ, d_status.p, d_ninner.p, inner, s);
         }
         hipLaunchKernelGGL(k_b_post, dim3(n), dim3(kPostT), 0, s, d_desc.p, d_status.p);
     }
@@ -427,10 +475,6 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
 {
     if (!b || !h || !id) return MOCGPU_ERR_ARG;
     return bguard(b, [&] {
-        if (h->opt.use_cmfd) {
-            b->err = "the batch engine runs problems without CMFD";
-            return (int)MOCGPU_ERR_UNSUPPORTED;
-        }
         if (!b->cases.empty() && h->NP != b->NP) {
             b->err = "all problems of a batch must have the same number of polar angles";
             return (int)MOCGPU_ERR_UNSUPPORTED;
@@ -459,6 +503,8 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
         HIP_CHECK(hipMemcpyAsync(h->d_scal.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, b->stream));
         const uint64_t zero = 0;
         HIP_CHECK(hipMemcpyAsync(h->d_parity.p, &zero, 8, hipMemcpyHostToDevice, b->stream));
+        const int32_t no_radial = 0;  // the first CMFD solve has no MoC currents yet (D-hat = 0)
+        HIP_CHECK(hipMemcpyAsync(h->d_radial.p, &no_radial, 4, hipMemcpyHostToDevice, b->stream));
         b->sync();
         mocgpu_batch::Case c{};
         c.h = h;

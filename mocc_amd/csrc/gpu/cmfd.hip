@@ -109,10 +109,10 @@ __device__ __forceinline__ double thickness(const Grid &m, int cell, int normal)
 
 // --- set-up kernels --------------------------------------------------------------
 
-// MoCSweeper::get_pin_flux_1g (PIN_PLANE, moc_sweeper.cpp:293-326) for all groups.
-__global__ void k_pin_flux_all(CmfdView v, double *cflux)
+// MoCSweeper::get_pin_flux_1g (PIN_PLANE, moc_sweeper.cpp:293-326) for all
+// groups: item t = (pin, group).
+__device__ __forceinline__ void pin_flux_item(const CmfdView &v, double *cflux, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= v.npin * v.G) return;
     const int ip = t / v.G, g = t % v.G;
     double vs = 0.0, pf = 0.0;
@@ -123,14 +123,22 @@ __global__ void k_pin_flux_all(CmfdView v, double *cflux)
     cflux[(size_t)v.pin_cell[ip] * v.G + g] = pf / vs;
 }
 
+__global__ void k_pin_flux_all(CmfdView v, double *cflux)
+{
+    pin_flux_item(v, cflux, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // XSMeshHomogenized::homogenize_region_flux: flux-area weighted Sigma_t,
 // nu Sigma_f, Sigma_f; chi weighted by the fission source; Sigma_s[g <- g']
 // weighted by phi_g'. One thread per (pin, group).
-__global__ void k_homogenize(CmfdView v, double *htr, double *hnf, double *hf, double *hch, double *hrm,
-                             double *hsc)
+struct HomogArrays {
+    double *htr, *hnf, *hf, *hch, *hrm, *hsc;
+};
+
+__device__ __forceinline__ void homogenize_item(const CmfdView &v, const HomogArrays &h, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= v.npin * v.G) return;
+    double *htr = h.htr, *hnf = h.hnf, *hf = h.hf, *hch = h.hch, *hrm = h.hrm, *hsc = h.hsc;
     const int G = v.G;
     const int ip = t / G, g = t % G;
     const int cell = v.pin_cell[ip];
@@ -171,19 +179,25 @@ __global__ void k_homogenize(CmfdView v, double *htr, double *hnf, double *hf, d
     hrm[(size_t)cell * G + g] = htr_g - self;
 }
 
+__global__ void k_homogenize(CmfdView v, HomogArrays h)
+{
+    homogenize_item(v, h, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 struct CoupleArgs {
     Grid m;
     int G, n_surf, has_radial;
+    const int32_t *radial_flag;  // if set, read instead of has_radial (batch graphs)
     int bc_lo[3], bc_hi[3];  // [normal]: low-side and high-side boundary kinds
     const double *htr, *cflux, *cur, *sfl;
     double *dtil, *stil, *dhat, *shat;
 };
 
-// CMFD::setup_solve, surface couplings (cmfd.cpp:423-490)
-__global__ void k_couple(CoupleArgs a)
+// CMFD::setup_solve, surface couplings (cmfd.cpp:423-490): item t = (surface, group)
+__device__ __forceinline__ void couple_item(const CoupleArgs &a, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n_surf * a.G) return;
+    const int has_radial = a.radial_flag ? *a.radial_flag : a.has_radial;
     const int is = t / a.G, g = t % a.G, G = a.G;
     int c1, c2;
     neigh_cells(a.m, is, c1, c2);
@@ -196,7 +210,7 @@ __global__ void k_couple(CoupleArgs a)
     a.dtil[(size_t)is * G + g] = dt;
     a.stil[(size_t)is * G + g] = st;
     double dh = 0.0, sh = 0.0;
-    if (norm != 2 && a.has_radial) {
+    if (norm != 2 && has_radial) {
         const double j = a.cur[(size_t)is * G + g];
         const double sf = a.sfl[(size_t)is * G + g];
         const double fl = c1 >= 0 ? a.cflux[(size_t)c1 * G + g] : 0.0;
@@ -209,10 +223,15 @@ __global__ void k_couple(CoupleArgs a)
     a.shat[(size_t)is * G + g] = sh;
 }
 
-// CMFD::setup_solve, matrix rows (cmfd.cpp:506-544): one thread per (cell, g)
-__global__ void k_assemble(Grid m, int G, const double *hrm, const double *dtil, const double *dhat, double *coef)
+__global__ void k_couple(CoupleArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    couple_item(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// CMFD::setup_solve, matrix rows (cmfd.cpp:506-544): item t = (group, cell)
+__device__ __forceinline__ void assemble_item(const Grid &m, int G, const double *hrm, const double *dtil,
+                                              const double *dhat, double *coef, int t)
+{
     const int nc = m.nx * m.ny;
     if (t >= nc * G) return;
     const int g = t / nc, c = t % nc;
@@ -240,11 +259,15 @@ __global__ void k_assemble(Grid m, int G, const double *hrm, const double *dtil,
     coef[4 * pl + i] = iy < m.ny - 1 ? off[1] : 0.0;  // north
 }
 
+__global__ void k_assemble(Grid m, int G, const double *hrm, const double *dtil, const double *dhat, double *coef)
+{
+    assemble_item(m, G, hrm, dtil, dhat, coef, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // CMFD::store_currents (cmfd.cpp:555-638): the net current and surface flux
 // the coarse solution implies.
-__global__ void k_store_currents(CoupleArgs a, double *cur, double *sfl)
+__device__ __forceinline__ void store_currents_item(const CoupleArgs &a, double *cur, double *sfl, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n_surf * a.G) return;
     const int is = t / a.G, g = t % a.G, G = a.G;
     int c1, c2;
@@ -259,11 +282,15 @@ __global__ void k_store_currents(CoupleArgs a, double *cur, double *sfl)
     cur[(size_t)is * G + g] = j;
 }
 
+__global__ void k_store_currents(CoupleArgs a, double *cur, double *sfl)
+{
+    store_currents_item(a, cur, sfl, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // MoCSweeper::set_pin_flux_1g (PIN_PLANE, moc_sweeper.cpp:360-433) for all
 // groups; a non-positive coarse flux raises the flag (the reference throws).
-__global__ void k_set_pin_flux_all(CmfdView v, const double *cflux, int *bad)
+__device__ __forceinline__ void set_pin_flux_item(const CmfdView &v, const double *cflux, int *bad, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= v.npin * v.G) return;
     const int ip = t / v.G, g = t % v.G;
     const double pn = cflux[(size_t)v.pin_cell[ip] * v.G + g];
@@ -276,6 +303,11 @@ __global__ void k_set_pin_flux_all(CmfdView v, const double *cflux, int *bad)
     fm /= v.pin_area[ip];
     const double f = pn / fm;
     for (int r = v.pin_off[ip]; r < v.pin_off[ip + 1]; r++) v.phi[(size_t)r * v.G + g] *= f;
+}
+
+__global__ void k_set_pin_flux_all(CmfdView v, const double *cflux, int *bad)
+{
+    set_pin_flux_item(v, cflux, bad, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // info[0]: coarse power iterations, info[1]: non-positive flux flag of the
@@ -534,16 +566,15 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Lan
     return s;
 }
 
-// CMFD::solve (cmfd.cpp:171-279)
-template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(SolveArgs A)
+// CMFD::solve (cmfd.cpp:171-279) by one workgroup of kCmfdThreads lanes;
+// lds holds 5 * nc doubles, red 2 * kCmfdWaves.
+template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
-    extern __shared__ double lds[];
     double *buf = lds;                // [nc] operand of the matrix-vector products
     double *fs_old = lds + A.nc;      // [nc] previous coarse fission source
     double *sinv = lds + 2 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
     double *sr0 = lds + 3 * A.nc;     // [nc] BiCGSTAB: shadow residual
     double *sb = lds + 4 * A.nc;      // [nc] BiCGSTAB: right-hand side
-    __shared__ double red[2 * kCmfdWaves];
     Lane<K> L;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -610,6 +641,13 @@ template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(So
         A.info[0] = iter;
         A.info[2] = n_bicg;
     }
+}
+
+template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(SolveArgs A)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[2 * kCmfdWaves];
+    cmfd_solve<K>(A, lds, red);
 }
 
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -698,52 +736,76 @@ void CmfdSolver::initialize(hipStream_t s)
     hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(64), 0, s, d_info_);
 }
 
+// The arguments of one problem's CMFD kernels (shared by enqueue_step and the
+// batch launches over many problems).
+struct CmfdAccess {
+    static CoupleArgs couple(const CmfdSolver &c, const CmfdView &v)
+    {
+        CoupleArgs ca{};
+        ca.m = Grid{c.nx_, c.ny_, c.hz_, c.d_dx_, c.d_dy_};
+        ca.G = c.G_;
+        ca.n_surf = c.n_surf_;
+        ca.has_radial = v.has_radial;
+        ca.radial_flag = nullptr;
+        ca.bc_lo[0] = c.bc_[WEST];
+        ca.bc_hi[0] = c.bc_[EAST];
+        ca.bc_lo[1] = c.bc_[SOUTH];
+        ca.bc_hi[1] = c.bc_[NORTH];
+        ca.bc_lo[2] = c.bc_[BOTTOM];
+        ca.bc_hi[2] = c.bc_[TOP];
+        ca.htr = c.d_htr_;
+        ca.cflux = c.d_cflux_;
+        ca.cur = v.cur;
+        ca.sfl = v.sfl;
+        ca.dtil = c.d_dtil_;
+        ca.stil = c.d_stil_;
+        ca.dhat = c.d_dhat_;
+        ca.shat = c.d_shat_;
+        return ca;
+    }
+    static SolveArgs solve(const CmfdSolver &c, const CmfdView &v)
+    {
+        SolveArgs a{};
+        a.nc = c.nc_;
+        a.nx = c.nx_;
+        a.G = c.G_;
+        a.max_iter = c.opt_.cmfd_max_iter;
+        a.k_tol = c.opt_.cmfd_k_tol;
+        a.psi_tol = c.opt_.cmfd_psi_tol;
+        a.rr = c.opt_.cmfd_resid_reduction;
+        a.coef = c.d_coef_;
+        a.hnf = c.d_hnf_;
+        a.hch = c.d_hch_;
+        a.hsc = c.d_hsc_;
+        a.cvol = c.d_cvol_;
+        a.cflux = c.d_cflux_;
+        a.k = v.k;
+        a.info = c.d_info_;
+        return a;
+    }
+    static HomogArrays homog(const CmfdSolver &c)
+    {
+        return HomogArrays{c.d_htr_, c.d_hnf_, c.d_hf_, c.d_hch_, c.d_hrm_, c.d_hsc_};
+    }
+    static int n_cell(const CmfdSolver &c) { return c.nc_; }
+    static int n_surf(const CmfdSolver &c) { return c.n_surf_; }
+    static double *hrm(const CmfdSolver &c) { return c.d_hrm_; }
+    static double *coef(const CmfdSolver &c) { return c.d_coef_; }
+    static bool enabled(const CmfdSolver &c) { return c.d_cflux_ != nullptr; }
+};
+
 void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
 {
     if (!d_cflux_) throw std::runtime_error("CMFD is not enabled for this handle (use_cmfd = 0)");
     const int npg = v.npin * v.G;
     hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(64), 0, s, d_info_);
     hipLaunchKernelGGL(k_pin_flux_all, dim3(nblk(npg)), dim3(256), 0, s, v, d_cflux_);
-    hipLaunchKernelGGL(k_homogenize, dim3(nblk(npg)), dim3(256), 0, s, v, d_htr_, d_hnf_, d_hf_, d_hch_, d_hrm_,
-                       d_hsc_);
-    CoupleArgs ca{};
-    ca.m = Grid{nx_, ny_, hz_, d_dx_, d_dy_};
-    ca.G = G_;
-    ca.n_surf = n_surf_;
-    ca.has_radial = v.has_radial;
-    ca.bc_lo[0] = bc_[WEST];
-    ca.bc_hi[0] = bc_[EAST];
-    ca.bc_lo[1] = bc_[SOUTH];
-    ca.bc_hi[1] = bc_[NORTH];
-    ca.bc_lo[2] = bc_[BOTTOM];
-    ca.bc_hi[2] = bc_[TOP];
-    ca.htr = d_htr_;
-    ca.cflux = d_cflux_;
-    ca.cur = v.cur;
-    ca.sfl = v.sfl;
-    ca.dtil = d_dtil_;
-    ca.stil = d_stil_;
-    ca.dhat = d_dhat_;
-    ca.shat = d_shat_;
+    hipLaunchKernelGGL(k_homogenize, dim3(nblk(npg)), dim3(256), 0, s, v, CmfdAccess::homog(*this));
+    const CoupleArgs ca = CmfdAccess::couple(*this, v);
     hipLaunchKernelGGL(k_couple, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca);
     hipLaunchKernelGGL(k_assemble, dim3(nblk((size_t)nc_ * G_)), dim3(256), 0, s, ca.m, G_, d_hrm_, d_dtil_,
                        d_dhat_, d_coef_);
-    SolveArgs a{};
-    a.nc = nc_;
-    a.nx = nx_;
-    a.G = G_;
-    a.max_iter = opt_.cmfd_max_iter;
-    a.k_tol = opt_.cmfd_k_tol;
-    a.psi_tol = opt_.cmfd_psi_tol;
-    a.rr = opt_.cmfd_resid_reduction;
-    a.coef = d_coef_;
-    a.hnf = d_hnf_;
-    a.hch = d_hch_;
-    a.hsc = d_hsc_;
-    a.cvol = d_cvol_;
-    a.cflux = d_cflux_;
-    a.k = v.k;
-    a.info = d_info_;
+    const SolveArgs a = CmfdAccess::solve(*this, v);
     const size_t lds = 5 * (size_t)nc_ * sizeof(double);  // buf, fs_old and the BiCGSTAB vectors
     const int kneed = (nc_ + kCmfdThreads - 1) / kCmfdThreads;
     if (kneed <= 1) hipLaunchKernelGGL(k_cmfd_solve<1>, dim3(1), dim3(kCmfdThreads), lds, s, a);
@@ -797,6 +859,161 @@ void CmfdSolver::coarse_flux(hipStream_t s, std::vector<double> &out)
     out.assign((size_t)nc_ * G_, 0.0);
     CMFD_CHECK(hipMemcpyAsync(out.data(), d_cflux_, out.size() * 8, hipMemcpyDeviceToHost, s));
     CMFD_CHECK(hipStreamSynchronize(s));
+}
+
+// --- the CMFD step of many problems (batch engine) ------------------------------
+
+namespace {
+
+struct CmfdCase {
+    int32_t bc;             // index of the problem in the batch status array
+    int32_t npg, nsg, ncg;  // elements: pins x G, surfaces x G, cells x G
+    CmfdView v;
+    CoupleArgs ca;
+    SolveArgs a;
+    HomogArrays h;
+    double *hrm, *coef;
+    int *info;
+};
+
+struct CaseBlock {
+    int32_t c, first;
+};
+
+__device__ __forceinline__ bool case_done(const CmfdCase &d, const int32_t *status)
+{
+    return status[2 * d.bc + 1] != 0;
+}
+
+__global__ void k_cmfd_b_zero(const CmfdCase *__restrict__ cs, int n, const int32_t *__restrict__ status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || case_done(cs[i], status)) return;
+    for (int q = 0; q < 3; q++) cs[i].info[q] = 0;
+}
+
+// stage 0: pin flux, 1: homogenize, 5: rescale (pins x G);
+// 2: couple, 4: store currents (surfaces x G); 3: assemble (cells x G)
+template <int STAGE>
+__global__ void k_cmfd_b_elem(const CmfdCase *__restrict__ cs, const CaseBlock *__restrict__ bm,
+                              const int32_t *__restrict__ status)
+{
+    const CaseBlock b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    const CmfdCase &d = cs[c];
+    if (case_done(d, status)) return;
+    const int t = __builtin_amdgcn_readfirstlane(b.first) + threadIdx.x;
+    if (STAGE == 0) pin_flux_item(d.v, d.a.cflux, t);
+    if (STAGE == 1) homogenize_item(d.v, d.h, t);
+    if (STAGE == 2) couple_item(d.ca, t);
+    if (STAGE == 3) assemble_item(d.ca.m, d.ca.G, d.hrm, d.ca.dtil, d.ca.dhat, d.coef, t);
+    if (STAGE == 4) store_currents_item(d.ca, d.v.cur, d.v.sfl, t);
+    if (STAGE == 5) set_pin_flux_item(d.v, d.a.cflux, d.info + 1, t);
+}
+
+template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_b_solve(const CmfdCase *__restrict__ cs,
+                                                                              const int32_t *__restrict__ status)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[2 * kCmfdWaves];
+    const CmfdCase &d = cs[blockIdx.x];
+    if (case_done(d, status)) return;
+    cmfd_solve<K>(d.a, lds, red);
+}
+
+std::vector<CaseBlock> case_blocks(const std::vector<int> &counts)
+{
+    std::vector<CaseBlock> m;
+    for (size_t c = 0; c < counts.size(); c++)
+        for (int first = 0; first < counts[c]; first += 256) m.push_back(CaseBlock{(int32_t)c, first});
+    return m;
+}
+
+}  // namespace
+
+struct CmfdBatch::Impl {
+    CmfdCase *cases = nullptr;
+    CaseBlock *bm[3] = {nullptr, nullptr, nullptr};  // pins, surfaces, cells
+    int nbm[3] = {0, 0, 0};
+    int kmax = 1;
+    size_t lds = 0;
+    void release()
+    {
+        if (cases) (void)hipFree(cases);
+        for (auto &p : bm)
+            if (p) (void)hipFree(p);
+        cases = nullptr;
+        for (auto &p : bm) p = nullptr;
+    }
+    ~Impl() { release(); }
+};
+
+CmfdBatch::CmfdBatch() : impl_(new Impl) {}
+CmfdBatch::~CmfdBatch() { delete impl_; }
+
+void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
+{
+    CMFD_CHECK(hipStreamSynchronize(s));
+    impl_->release();
+    n_ = 0;
+    if (entries.empty()) return;
+    std::vector<CmfdCase> host;
+    std::vector<int> np, ns, nc;
+    int ncmax = 0;
+    for (const CmfdBatchEntry &e : entries) {
+        const CmfdSolver &sv = *e.solver;
+        if (!CmfdAccess::enabled(sv)) throw std::runtime_error("CMFD is not enabled for a batch problem");
+        CmfdCase d{};
+        d.bc = e.batch_index;
+        d.v = e.view;
+        d.ca = CmfdAccess::couple(sv, e.view);
+        d.ca.radial_flag = e.radial_flag;
+        d.a = CmfdAccess::solve(sv, e.view);
+        d.h = CmfdAccess::homog(sv);
+        d.hrm = CmfdAccess::hrm(sv);
+        d.coef = CmfdAccess::coef(sv);
+        d.info = d.a.info;
+        d.npg = e.view.npin * e.view.G;
+        d.nsg = CmfdAccess::n_surf(sv) * e.view.G;
+        d.ncg = CmfdAccess::n_cell(sv) * e.view.G;
+        host.push_back(d);
+        np.push_back(d.npg);
+        ns.push_back(d.nsg);
+        nc.push_back(d.ncg);
+        ncmax = std::max(ncmax, CmfdAccess::n_cell(sv));
+    }
+    n_ = (int)host.size();
+    dalloc(impl_->cases, host.size());
+    CMFD_CHECK(hipMemcpyAsync(impl_->cases, host.data(), host.size() * sizeof(CmfdCase), hipMemcpyHostToDevice, s));
+    const std::vector<int> *cnt[3] = {&np, &ns, &nc};
+    std::vector<CaseBlock> maps[3];
+    for (int k = 0; k < 3; k++) {
+        maps[k] = case_blocks(*cnt[k]);
+        impl_->nbm[k] = (int)maps[k].size();
+        dalloc(impl_->bm[k], maps[k].size());
+        CMFD_CHECK(hipMemcpyAsync(impl_->bm[k], maps[k].data(), maps[k].size() * sizeof(CaseBlock),
+                                  hipMemcpyHostToDevice, s));
+    }
+    impl_->kmax = (ncmax + kCmfdThreads - 1) / kCmfdThreads;
+    impl_->lds = 5 * (size_t)ncmax * sizeof(double);
+    CMFD_CHECK(hipStreamSynchronize(s));
+}
+
+void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
+{
+    if (n_ == 0) return;
+    const Impl &m = *impl_;
+    hipLaunchKernelGGL(k_cmfd_b_zero, dim3(nblk(n_, 64)), dim3(64), 0, s, m.cases, n_, status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<0>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<1>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<2>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<3>, dim3(m.nbm[2]), dim3(256), 0, s, m.cases, m.bm[2], status);
+    if (m.kmax <= 1) hipLaunchKernelGGL(k_cmfd_b_solve<1>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
+    else if (m.kmax <= 2) hipLaunchKernelGGL(k_cmfd_b_solve<2>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
+    else if (m.kmax <= 4) hipLaunchKernelGGL(k_cmfd_b_solve<4>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
+    else hipLaunchKernelGGL(k_cmfd_b_solve<6>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<4>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
+    hipLaunchKernelGGL(k_cmfd_b_elem<5>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
 }
 
 }  // namespace gpu

@@ -43,6 +43,8 @@ public:
     void read_info(hipStream_t s, int *iterations, int *bad_flux, int *bicgstab_iterations = nullptr);
 
     int n_cell() const { return nc_; }
+    // device word raised when the last rescale met a non-positive coarse flux
+    const int *bad_flux_flag() const { return d_info_ ? d_info_ + 1 : nullptr; }
     // Assembled coefficients of group g after the last solve, rows in cell
     // order with the columns of each row sorted (CSR, for tests).
     void matrix_csr(int g, hipStream_t s, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
@@ -50,6 +52,7 @@ public:
     void coarse_flux(hipStream_t s, std::vector<double> &out);
 
 private:
+    friend struct CmfdAccess;
     void free_all();
     int nx_ = 0, ny_ = 0, nc_ = 0, G_ = 0, n_surf_ = 0;
     double hz_ = 0.0;
@@ -63,6 +66,37 @@ private:
     double *d_coef_ = nullptr;  // [5][G][nc]: south, west, diagonal, east, north
     int *d_info_ = nullptr;     // [0] outer iterations, [1] non-positive pin flux seen
     int n_solve_ = 0;
+};
+
+// One problem of a batch that runs CMFD: its index in the batch's status
+// array, its solver, the view of its sweeper's arrays (k and the fine flux
+// on the device) and a device flag that turns 1 once radial currents exist.
+struct CmfdBatchEntry {
+    int batch_index;
+    CmfdSolver *solver;
+    CmfdView view;
+    int32_t *radial_flag;
+};
+
+// The CMFD step (CmfdSolver::enqueue_step) of many problems in a fixed
+// sequence of launches, each covering every problem: the element-wise
+// set-up kernels over (problem, element) block maps and one solve
+// workgroup per problem. Problems whose status is finished are skipped, so
+// the sequence can be captured once and replayed.
+class CmfdBatch {
+public:
+    CmfdBatch();
+    ~CmfdBatch();
+    CmfdBatch(const CmfdBatch &) = delete;
+    CmfdBatch &operator=(const CmfdBatch &) = delete;
+    void build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s);
+    void enqueue(const int32_t *status, hipStream_t s) const;
+    bool empty() const { return n_ == 0; }
+
+private:
+    struct Impl;
+    Impl *impl_ = nullptr;
+    int n_ = 0;
 };
 
 }  // namespace gpu

@@ -119,9 +119,10 @@ struct mocgpu_handle {
     DBuf<double> d_phi, d_phi_old, d_src, d_fs, d_fs_prev, d_psum, d_csum, d_ssum, d_cur, d_sfl;
     DBuf<float> d_psi;
     DBuf<double> d_partial, d_scal, d_red_out, d_pin, d_e2;
-    // batch mode: device parity word and (iterations, finished) status
+    // batch mode: device parity word, (iterations, finished) status and the
+    // flag that radial currents exist (read by the CMFD coupling in replays)
     DBuf<uint64_t> d_parity;
-    DBuf<int32_t> d_status;
+    DBuf<int32_t> d_status, d_radial;
     std::unique_ptr<CmfdSolver> cmfd;
 
     // host copies
@@ -193,6 +194,25 @@ struct mocgpu_handle {
         r.src = d_src.p;
         r.phi = d_phi.p;
         return r;
+    }
+
+    // coarse-surface current reduction of the radial surfaces (g < 0: all groups)
+    CurrentArgs current_args(int g_fixed) const
+    {
+        CurrentArgs c{};
+        c.n0 = nx * ny;
+        c.n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
+        c.G = G;
+        c.g_fixed = g_fixed;
+        c.n_lslots = (int)std::max<size_t>(1, plan->lsurf_glob.size());
+        c.ptr = d_surf_ptr.p;
+        c.idx = d_surf_idx.p;
+        c.csum = d_csum.p;
+        c.ssum = d_ssum.p;
+        c.area = d_surf_area.p;
+        c.cur = d_cur.p;
+        c.sfl = d_sfl.p;
+        return c;
     }
 
     RedArgs red_args() const

@@ -58,6 +58,12 @@ struct ReduceArgs {
     uint64_t gmask;
 };
 
+// Block -> (problem, first element) map of a launch over many problems; each
+// problem's elements are covered by whole 256-thread blocks.
+struct BlockMap {
+    int32_t c, first;
+};
+
 // One persistent launch over a work queue of (case, band, group) items of many
 // problems: workgroups take items from *counter until n_items is reached.
 // Items of cases whose status[2 * case + 1] is non-zero (finished) or whose
@@ -70,12 +76,25 @@ struct QueueArgs {
     unsigned int *counter;
     const int32_t *status;    // [2 * case + 1]: non-zero once the case has finished
     const int32_t *n_inner;   // [case]
+    const int32_t *want_cur;  // [case] or null: tally currents on the case's last inner iteration
+};
+
+// Coarse-surface current reduction of one problem (surfaces [n0, n1)).
+struct CurrentArgs {
+    int32_t n0, n1, G, g_fixed, n_lslots;
+    const int32_t *ptr, *idx;  // surface -> current slots
+    const double *csum, *ssum, *area;
+    double *cur, *sfl;         // [n_surf][G]
+    int32_t *radial_flag;      // batch: set to 1 once currents exist
 };
 
 void launch_sweep(const SweepArgs &a, int n_polar, bool with_current, int n_items, hipStream_t s);
 // Workgroups resident per CU for the queue kernel with lds bytes of LDS.
-int queue_blocks_per_cu(int n_polar, size_t lds);
-void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s);
+int queue_blocks_per_cu(int n_polar, size_t lds, bool cur);
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur);
+void launch_reduce_current(const CurrentArgs &a, size_t n_items, hipStream_t s);
+void launch_reduce_current_batch(const CurrentArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
+                                 const int32_t *n_inner, int inner, hipStream_t s);
 size_t sweep_lds_bytes(const SweepArgs &a);
 // Workgroups of a sweep launch: one per band (one group), or the XCD-aware
 // (band, group) grid of all groups.
@@ -84,11 +103,6 @@ void launch_reduce_flux(const ReduceArgs &a, hipStream_t s);
 // out[i] = the sweep kernel's 1 - exp(-tau[i]) for n values.
 void launch_attenuation(const float *tau, float *out, int n, hipStream_t s);
 
-// Block -> (problem, first element) map of a launch over many problems; each
-// problem's elements are covered by whole 256-thread blocks.
-struct BlockMap {
-    int32_t c, first;
-};
 // Lanes per (FSR, group) item of the flux reduction.
 int reduce_flux_lanes();
 void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,

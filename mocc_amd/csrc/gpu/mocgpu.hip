@@ -162,34 +162,6 @@ __global__ void k_fill_f(float *a, float v, size_t n)
     if (i < n) a[i] = v;
 }
 
-// Currents of the coarse surfaces: sum the band slots, normalize by area
-// (moc::Current::post_sweep, moc_current_worker.hpp:272-318). Surfaces no ray
-// crosses end up zero, as after CoarseData::zero_data_radial.
-__global__ void k_reduce_current(int n_surf_xy0, int n_surf_xy1, int G, int g_fixed, int n_lslots, const int32_t *ptr,
-                                 const int32_t *idx, const double *csum, const double *ssum, const double *area,
-                                 double *cur, double *sfl)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int s, g;
-    const int n = n_surf_xy1 - n_surf_xy0;
-    if (g_fixed >= 0) {
-        s = t;
-        g = g_fixed;
-    } else {
-        s = t / G;
-        g = t % G;
-    }
-    if (s >= n) return;
-    s += n_surf_xy0;
-    double c = 0.0, f = 0.0;
-    for (int k = ptr[s]; k < ptr[s + 1]; k++) {
-        c += csum[(size_t)g * n_lslots + idx[k]];
-        f += ssum[(size_t)g * n_lslots + idx[k]];
-    }
-    cur[(size_t)s * G + g] = c / area[s];
-    sfl[(size_t)s * G + g] = f / area[s];
-}
-
 // MoCSweeper::get_pin_flux_1g, PIN_PLANE (moc_sweeper.cpp:293-326)
 __global__ void k_pin_flux(int npin, int G, int g, const int32_t *pin_off, const int32_t *pin_cell,
                            const double *vol, const double *phi, double *pin)
@@ -267,9 +239,7 @@ void mocgpu_handle::sweep_once(int g, bool with_current)
     if (with_current) {
         int n0 = nx * ny, n1 = nx * ny + (nx + 1) * ny + (ny + 1) * nx;
         size_t n = (size_t)(n1 - n0) * (g >= 0 ? 1 : G);
-        int nls = (int)std::max<size_t>(1, plan->lsurf_glob.size());
-        hipLaunchKernelGGL(k_reduce_current, dim3(nblk(n)), dim3(256), 0, stream, n0, n1, G, g, nls,
-                           d_surf_ptr.p, d_surf_idx.p, d_csum.p, d_ssum.p, d_surf_area.p, d_cur.p, d_sfl.p);
+        launch_reduce_current(current_args(g), n, stream);
         has_radial = true;
     }
     if (!opt.gs_boundary) {
@@ -505,6 +475,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         }
         h->d_parity.alloc(1);
         h->d_status.alloc(2);
+        h->d_radial.alloc(1);
         h->cmfd = std::make_unique<CmfdSolver>();
         h->cmfd->setup(*p, h->opt, s);
         h->sync();
@@ -606,7 +577,8 @@ int64_t mocgpu_state_bytes(const mocgpu_handle *h)
                      h->d_src.bytes() + h->d_fs.bytes() + h->d_fs_prev.bytes() + h->d_psum.bytes() +
                      h->d_csum.bytes() + h->d_ssum.bytes() + h->d_cur.bytes() + h->d_sfl.bytes() +
                      h->d_psi.bytes() + h->d_partial.bytes() + h->d_scal.bytes() + h->d_red_out.bytes() +
-                     h->d_pin.bytes() + h->d_e2.bytes() + h->d_parity.bytes() + h->d_status.bytes();
+                     h->d_pin.bytes() + h->d_e2.bytes() + h->d_parity.bytes() + h->d_status.bytes() +
+                     h->d_radial.bytes();
     return (int64_t)b;
 }
 

@@ -96,6 +96,9 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const BandInfo B = A.bands[band];
     const int tid = threadIdx.x;
     const int G = A.G;
+    // currents are tallied when the instantiation has them and this problem
+    // wants them (the batch queue mixes problems with and without CMFD)
+    const bool cur = CUR && A.lds_lsurf > 0;
     if (tid == 0) s_bound = 0u;
     __syncthreads();
 
@@ -116,7 +119,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         sq[B.n_local + t] = make_float2(0.0f, 0.0f);
         tal[B.n_local + t] = 0ull;
     }
-    if (CUR) {
+    if (cur) {
         for (int l = tid; l < B.n_lsurf; l += 2 * W) {
             tcur[l] = 0ull;
             tsfl[l] = 0ull;
@@ -189,7 +192,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // without one do not wait on a load; nstep = 0x10000 is past every step
     uint32_t nev = 0u;
     int nstep = 0x10000;
-    if (CUR) {
+    if (cur) {
         kc = fixed_exp(psi_bound * (double)cwsum * 1.0001, (double)B.max_surf_contrib);
         sc = ldexp(1.0, kc);
         if (active) {
@@ -324,7 +327,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // band run on one XCD, so its L2 joins their writes of a slot's groups
     const int32_t *__restrict__ pos = A.slot_pos + B.loc_base;
     for (int l = tid; l < B.n_local; l += 2 * W) A.psum[(size_t)pos[l] * A.G + g] = (double)(long long)tal[l] * rscale;
-    if (CUR) {
+    if (cur) {
         const double rc = ldexp(1.0, -kc);
         double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
         double *__restrict__ ss = A.ssum + (size_t)g * A.n_lslots + B.lsurf_base;
@@ -361,7 +364,9 @@ __global__ __launch_bounds__(2 * W) void k_sweep(SweepArgs A)
 // Persistent work-queue sweep over many problems. Thread 0 takes the next
 // item index with a (vector) global atomic and shares it through LDS; every
 // workgroup leaves once the counter passes n_items, so the grid always drains.
-template <int NP>
+// The CUR instantiation tallies coarse-surface currents for the problems
+// whose want_cur is set (CMFD problems on their last inner iteration).
+template <int NP, bool CUR>
 __global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArgs *__restrict__ cases)
 {
     extern __shared__ unsigned long long smem[];
@@ -376,9 +381,10 @@ __global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArg
         const int4 w = Q.items[it];
         const int c = __builtin_amdgcn_readfirstlane(w.x);
         if (Q.status[2 * c + 1] != 0 || Q.n_inner[c] <= Q.inner) continue;
-        const SweepArgs A = cases[c];
-        sweep_band<NP, false, kRays>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
-                                     smem, s_bound);
+        SweepArgs A = cases[c];
+        if (!CUR || !(Q.want_cur && Q.want_cur[c] && Q.inner == Q.n_inner[c] - 1)) A.lds_lsurf = 0;
+        sweep_band<NP, CUR, kRays>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
+                                   smem, s_bound);
         __syncthreads();
     }
 }
@@ -417,6 +423,53 @@ __device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, 
     const double tr = A.mat_tr[m * G + g];
     const double q = (A.src[(size_t)f * G + g] + A.phi[(size_t)f * G + g] * A.mat_ss[m * G + g]) * (1.0 / (tr * FPI));
     A.phi[(size_t)f * G + g] = T / (tr * A.vol[f]) + q * FPI;
+}
+
+// Net current and surface flux of a coarse surface: the sum of the band
+// slots that tally it, over its area (moc::Current::post_sweep,
+// moc_current_worker.hpp:272-318). Surfaces no ray crosses end up zero, as
+// after CoarseData::zero_data_radial. Item t covers (surface n0 + t / G,
+// group t % G), or surface n0 + t of group g_fixed.
+__device__ __forceinline__ void reduce_current_item(const CurrentArgs &A, int t)
+{
+    int s, g;
+    if (A.g_fixed >= 0) {
+        s = t;
+        g = A.g_fixed;
+    } else {
+        s = t / A.G;
+        g = t % A.G;
+    }
+    if (s >= A.n1 - A.n0) return;
+    s += A.n0;
+    double c = 0.0, f = 0.0;
+    for (int k = A.ptr[s]; k < A.ptr[s + 1]; k++) {
+        c += A.csum[(size_t)g * A.n_lslots + A.idx[k]];
+        f += A.ssum[(size_t)g * A.n_lslots + A.idx[k]];
+    }
+    A.cur[(size_t)s * A.G + g] = c / A.area[s];
+    A.sfl[(size_t)s * A.G + g] = f / A.area[s];
+}
+
+__global__ void k_reduce_current(CurrentArgs A)
+{
+    reduce_current_item(A, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The same over the problems of a batch whose last inner iteration this is
+// and that tally currents (CMFD); the first block of a problem raises its
+// flag that radial currents exist.
+__global__ void k_reduce_current_batch(const CurrentArgs *__restrict__ cases, const BlockMap *__restrict__ bm,
+                                       const int32_t *__restrict__ status, const int32_t *__restrict__ n_inner,
+                                       int inner)
+{
+    const BlockMap b = bm[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(b.c);
+    if (status[2 * c + 1] != 0 || n_inner[c] - 1 != inner) return;
+    const CurrentArgs A = cases[c];
+    const int first = __builtin_amdgcn_readfirstlane(b.first);
+    if (first == 0 && threadIdx.x == 0 && A.radial_flag) *A.radial_flag = 1;
+    reduce_current_item(A, first + threadIdx.x);
 }
 
 __global__ void k_reduce_flux(ReduceArgs A)
@@ -487,28 +540,42 @@ void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStr
 #undef MOCGPU_SWEEP_CASE
 }
 
-int queue_blocks_per_cu(int n_polar, size_t lds)
+int queue_blocks_per_cu(int n_polar, size_t lds, bool cur)
 {
     int nb = 0;
     hipError_t e = hipSuccess;
+#define MOCGPU_QOCC(NPV)                                                                                     \
+    case NPV:                                                                                                \
+        e = cur ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, true>, kWG, lds)      \
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, false>, kWG, lds);    \
+        break;
     switch (n_polar) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<1>, kWG, lds); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<2>, kWG, lds); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<3>, kWG, lds); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<4>, kWG, lds); break;
+        MOCGPU_QOCC(1)
+        MOCGPU_QOCC(2)
+        MOCGPU_QOCC(3)
+    default:
+        MOCGPU_QOCC(4)
     }
+#undef MOCGPU_QOCC
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s)
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur)
 {
     dim3 grid(n_blocks), block(kWG);
+#define MOCGPU_QLAUNCH(NPV)                                                                             \
+    case NPV:                                                                                           \
+        if (cur) hipLaunchKernelGGL((k_sweep_queue<NPV, true>), grid, block, lds, s, q, q.cases);        \
+        else hipLaunchKernelGGL((k_sweep_queue<NPV, false>), grid, block, lds, s, q, q.cases);           \
+        break;
     switch (n_polar) {
-    case 1: hipLaunchKernelGGL(k_sweep_queue<1>, grid, block, lds, s, q, q.cases); break;
-    case 2: hipLaunchKernelGGL(k_sweep_queue<2>, grid, block, lds, s, q, q.cases); break;
-    case 3: hipLaunchKernelGGL(k_sweep_queue<3>, grid, block, lds, s, q, q.cases); break;
-    default: hipLaunchKernelGGL(k_sweep_queue<4>, grid, block, lds, s, q, q.cases); break;
+        MOCGPU_QLAUNCH(1)
+        MOCGPU_QLAUNCH(2)
+        MOCGPU_QLAUNCH(3)
+    default:
+        MOCGPU_QLAUNCH(4)
     }
+#undef MOCGPU_QLAUNCH
 }
 
 int sweep_grid_items(int n_bands, int G, bool all_groups)
@@ -516,6 +583,18 @@ int sweep_grid_items(int n_bands, int G, bool all_groups)
     if (!all_groups) return n_bands;
     const int per_xcd = (n_bands + kNumXcd - 1) / kNumXcd;
     return per_xcd * G * kNumXcd;
+}
+
+void launch_reduce_current(const CurrentArgs &a, size_t n_items, hipStream_t s)
+{
+    if (n_items > 0) hipLaunchKernelGGL(k_reduce_current, dim3((n_items + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_reduce_current_batch(const CurrentArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
+                                 const int32_t *n_inner, int inner, hipStream_t s)
+{
+    if (n_blocks > 0)
+        hipLaunchKernelGGL(k_reduce_current_batch, dim3(n_blocks), dim3(256), 0, s, cases, bm, status, n_inner, inner);
 }
 
 void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,

@@ -131,3 +131,61 @@ def test_create_like_shares_geometry(batch_decks):
     assert np.array_equal(b.flux(), c.flux())
     with pytest.raises(DeviceError):
         GpuMoCSweeper(batch_decks["pin1"], energy_mode=1, like=a)
+
+
+def _cmfd_sweepers(batch_decks, names):
+    from mocc_amd.sweeper import GpuMoCSweeper
+    return {n: GpuMoCSweeper(batch_decks[n], energy_mode=1, n_inner=5, use_cmfd=True) for n in names}
+
+
+def test_batch_cmfd_matches_single(batch_decks):
+    """Problems with CMFD (5 inner iterations, as mocc's CMFD decks use):
+    the batch runs each one's CMFD step (one solve workgroup per problem)
+    and its last inner sweep tallies currents; each converges like its
+    one-at-a-time device solve."""
+    names = ["c1", "c1pert", "asm17"]
+    sw = _cmfd_sweepers(batch_decks, names)
+    res = _solve_batch(sw, names)
+    for n in names:
+        print(f"cmfd case {n}", flush=True)
+        single = _cmfd_sweepers(batch_decks, [n])[n]
+        rc, st1 = single.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+        assert rc == 0 and st1.converged
+        st = res[n]
+        assert st.converged, (n, st.iterations)
+        assert st.iterations < 40, st.iterations  # CMFD: tens of outer iterations, not hundreds
+        assert abs(st.k - st1.k) < 1e-9, (n, st.k, st1.k)
+        assert abs(st.iterations - st1.iterations) <= 1
+        f_b, f_s = sw[n].flux(), single.flux()
+        assert np.abs(f_b / f_s - 1.0).max() < 1e-5
+
+
+def test_batch_cmfd_alone_vs_mixed_bitwise(batch_decks):
+    """A CMFD problem mixed with problems without CMFD: bitwise the same."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    target = "c1pert"
+    alone = _cmfd_sweepers(batch_decks, [target])
+    r_alone = _solve_batch(alone, [target])[target]
+    f_alone = alone[target].flux()
+    sw = _sweepers(batch_decks, ["asm17", "pin1"])
+    sw[target] = GpuMoCSweeper(batch_decks[target], energy_mode=1, n_inner=5, use_cmfd=True)
+    r = _solve_batch(sw, ["asm17", target, "pin1"])[target]
+    assert r.k == r_alone.k and r.iterations == r_alone.iterations and r.err_psi == r_alone.err_psi
+    assert np.array_equal(sw[target].flux(), f_alone)
+
+
+def test_batch_cmfd_graph_matches_eager(batch_decks):
+    from mocc_amd.sweeper import GpuBatch
+    names = ["c1", "asm17"]
+    sg, se = _cmfd_sweepers(batch_decks, names), _cmfd_sweepers(batch_decks, names)
+    bg, be = GpuBatch(use_graph=True), GpuBatch(use_graph=False)
+    for n in names:
+        bg.add(sg[n], K_TOL, PSI_TOL, MAX_IT)
+        be.add(se[n], K_TOL, PSI_TOL, MAX_IT)
+    for rep in range(6):
+        bg.run(1)
+        be.run(1)
+        bg.poll()
+        be.poll()
+        for n in names:
+            assert np.array_equal(sg[n].flux(), se[n].flux()), (rep, n)
