@@ -1139,6 +1139,41 @@ void CoreMesh::trace(std::vector<Point2> &ps) const
     }
     std::sort(ps.begin(), ps.end());
     ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+    snap_corners(ps);
+}
+
+// A ray that passes within rounding distance of a pin corner crosses the
+// corner's x line and y line at two points a few ULPs apart (one of them may
+// be the ray's end on the domain boundary). Point2 equality (200 ULPs per
+// coordinate) may keep both while coarse_norm_point classifies one of them
+// as the corner (both lines) and the other as a crossing of one line: the
+// coarse data then count one surface twice and the pin-cell currents no
+// longer balance. Crossings of the pin lines closer than kCornerSnap can only
+// be such a pair, so they become one point exactly on the corner (a ray end
+// stays the ray end), which coarse_norm_point takes as one corner crossing.
+// A point that it takes as a corner anyway is also moved onto the corner, so
+// that no sliver of a segment is traced in the neighbouring pin.
+void CoreMesh::snap_corners(std::vector<Point2> &ps) const
+{
+    auto nearest = [](const VecF &v, double x) {
+        size_t i = (size_t)(std::lower_bound(v.begin(), v.end(), x) - v.begin());
+        if (i == v.size() || (i > 0 && x - v[i - 1] < v[i] - x)) i--;
+        return v[i];
+    };
+    auto corner = [&](const Point2 &p) { return Point2(nearest(x_vec, p.x), nearest(y_vec, p.y)); };
+    const size_t n = ps.size();
+    std::vector<Point2> out;
+    out.reserve(n);
+    for (size_t k = 0; k < n; k++) {
+        if (k + 1 < n && std::hypot(ps[k + 1].x - ps[k].x, ps[k + 1].y - ps[k].y) < kCornerSnap) {
+            out.push_back(corner(ps[k]));
+            k++;  // the pair is one crossing
+            continue;
+        }
+        const Point2 c = corner(ps[k]);
+        out.push_back((fp_equiv(ps[k].x, c.x) && fp_equiv(ps[k].y, c.y)) ? c : ps[k]);
+    }
+    ps.swap(out);
 }
 
 // ---------------------------------------------------------------------------

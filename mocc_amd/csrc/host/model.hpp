@@ -6,6 +6,10 @@
 // The geometric arithmetic (intersections, sorting, tie-breaking) follows the
 // reference operation for operation, because its golden ray tests compare
 // segment lengths at 1e-15 (src/sweepers/moc/tests/test_Ray.cpp:281-305).
+// One step is added: CoreMesh::snap_corners merges the two crossings of a ray
+// that grazes a pin corner, where the reference's fuzzy point equality and
+// corner classification disagree and count one coarse surface twice. Rays
+// that pass no closer than 1e-9 cm to a corner are traced exactly as before.
 #pragma once
 
 #include <array>
@@ -225,8 +229,12 @@ public:
     // CoreMesh::get_pinmesh: locate the pin holding p, return its mesh, set
     // p to the pin centre and add the pin's first region to first_reg.
     const PinMesh *get_pinmesh(Point2 &p, int &first_reg) const;
-    // Trace the core lines (Mesh::trace, core/mesh.cpp:321-344).
+    // Trace the core lines (Mesh::trace, core/mesh.cpp:321-344), then merge
+    // the two crossings of a ray that grazes a pin corner.
     void trace(std::vector<Point2> &ps) const;
+    void snap_corners(std::vector<Point2> &ps) const;
+    // distance (cm) below which a crossing is taken to pass through a corner
+    static constexpr double kCornerSnap = 1e-9;
     VecF areas() const;
 };
 
