@@ -41,6 +41,11 @@ def algorithmic_bytes(p) -> int:
     return 8 * S + 16 * F * G + 16 * R * P * G
 
 
+def progress(msg: str):
+    """One line per finished section on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def read_pmc(path: str) -> dict:
     """Key: value lines of the committed rocprofv3 PMC summary of the sweep
     kernel, per inner sweep (HBM bytes from FETCH_SIZE + WRITE_SIZE calibrated
@@ -81,18 +86,21 @@ def oracle_k(factory, case):
     """FP64 eigenvalue of a branch case from the oracle (reference defaults)."""
     from oracle.binding import Oracle
     hp = factory.host_problem(case)
-    o = Oracle(hp, n_inner=factory.n_inner)
+    o = Oracle(hp, n_inner=factory.n_inner, use_cmfd=factory.use_cmfd)
     rc, st, _ = o.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
     return st.k
 
 
-def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
+def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use_cmfd: bool = True):
     """Batched branch cases (one graph-replayed batch) against the same
     problems solved one at a time with the single-problem device path; keff
-    of a sample checked against the FP64 oracle."""
+    of a sample checked against the FP64 oracle. With CMFD: 5 inner
+    iterations per outer, as mocc's CMFD decks use; without: 1."""
     from mocc_amd.branch import BranchFactory, c4_cases
     from mocc_amd.sweeper import GpuBatch
-    factory = BranchFactory(os.path.join(work_dir, "c4"))
+    n_inner = 5 if use_cmfd else 1
+    factory = BranchFactory(os.path.join(work_dir, "c4cmfd" if use_cmfd else "c4"), n_inner=n_inner,
+                            use_cmfd=use_cmfd)
     cases = c4_cases(n_cases)
     t0 = time.perf_counter()
     handles = [factory.make(c) for c in cases]
@@ -102,6 +110,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
     t0 = time.perf_counter()
     res = batch.solve_all(chunk=8)
     t_batch = time.perf_counter() - t0
+    progress(f"batched solve of {n_cases} cases: {t_batch:.2f} s")
     k_batch = {i: res[cid].k for i, cid in enumerate(ids)}
     iters = [res[cid].iterations for cid in ids]
     n_conv = sum(res[cid].converged for cid in ids)
@@ -118,11 +127,15 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
         rc, st = h.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
         k_serial[i] = st.k
     t_serial = time.perf_counter() - t0
+    progress(f"serial solve of {len(sub)} cases: {t_serial:.2f} s")
     for h in sh:
         h.close()
     # keff against the FP64 oracle on a sample of the smaller problems
     small = [i for i, c in enumerate(cases) if c.kind != "17x17"][:: max(1, n_cases // (2 * max(n_oracle, 1)))]
-    dk = [abs(k_batch[i] - oracle_k(factory, cases[i])) for i in small[:n_oracle]]
+    dk = []
+    for i in small[:n_oracle]:
+        dk.append(abs(k_batch[i] - oracle_k(factory, cases[i])))
+        progress(f"oracle keff of case {i} ({cases[i].kind}): |dk| = {dk[-1]:.2e}")
     factory.close()
     bat = n_cases / t_batch
     ser = len(sub) / t_serial
@@ -134,18 +147,21 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int):
         "batch_s": t_batch, "setup_s": t_setup,
         "keff_max_abs_err_vs_fp64": max(dk) if dk else None, "keff_checked": len(dk),
         "keff_max_abs_diff_batch_vs_serial": max(abs(k_batch[i] - k_serial[i]) for i in sub),
-        "energy": "Jacobi in energy, Jacobi boundary update, k_tol 1e-7, psi_tol 1e-6",
+        "solver": (f"{'CMFD, ' if use_cmfd else 'no CMFD, '}{n_inner} inner iteration(s) per outer, "
+                   "Jacobi in energy, Gauss-Seidel boundary update, k_tol 1e-7, psi_tol 1e-6"),
     }
 
 
-def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5):
+def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, use_cmfd: bool = True):
     """Poisson arrivals of branch-case requests served by the continuous-
     batching loop at the given fractions of its burst (saturation)
-    throughput, beside the same requests solved one at a time."""
+    throughput, beside the same requests solved one at a time (CMFD with 5
+    inner iterations per outer, or no CMFD with 1)."""
     import numpy as np
     from mocc_amd.branch import BranchFactory, c5_request
     from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
-    factory = BranchFactory(os.path.join(work_dir, "c5"))
+    factory = BranchFactory(os.path.join(work_dir, "c5cmfd" if use_cmfd else "c5"), n_inner=5 if use_cmfd else 1,
+                            use_cmfd=use_cmfd)
     rng = np.random.default_rng(seed)
     cases = [c5_request(rng, i) for i in range(n_req)]
     for c in cases:  # trace every geometry before timing
@@ -153,12 +169,15 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5):
     srv = Server(factory, chunk=4)
     burst = summarize(srv.serve(np.zeros(n_req), cases))
     out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
-                       "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]",
+                       "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]; "
+                       + ("CMFD, 5 inner iterations" if use_cmfd else "no CMFD, 1 inner iteration"),
            "burst": burst}
+    progress(f"serving burst: {burst['solves_per_s']:.3g} solves/s")
     for frac in rates:
         rate = frac * burst["solves_per_s"]
         out[f"poisson_{frac:g}x"] = {"rate_per_s": rate,
                                      **summarize(srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases))}
+        progress(f"serving at {frac:g}x burst done")
     srv.close()
     out["serial_burst"] = summarize(serve_serial(factory, np.zeros(n_req), cases))
     factory.close()
@@ -299,16 +318,24 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
     }
+    progress(f"C2 sweep: {value:.4g} updates/s")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hp, args.cpu_sweeps)
+        progress("CPU baseline done")
     dev.close()
     if not args.no_batch:
         line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 6)
+        progress(f"batched (CMFD): {line['batched']['batched_solves_per_s']:.4g} solves/s")
+        line["batched_no_cmfd"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases,
+                                                0 if args.no_cpu else 2, use_cmfd=False)
+        progress(f"batched (no CMFD): {line['batched_no_cmfd']['batched_solves_per_s']:.4g} solves/s")
     if not args.no_serve:
         rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
         line["serving"] = bench_serving(deck_dir, args.serve_requests, rates)
+        progress("serving done")
     if args.c3:
         line["c3"] = bench_c3(deck_dir, args.c3_oracle)
+        progress("C3 done")
     print(json.dumps(line), flush=True)
 
 

@@ -54,10 +54,14 @@ class BranchFactory:
 
     BASE_SEED = 4
 
-    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1):
+    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1, use_cmfd: bool = False):
+        """use_cmfd: CMFD-accelerated solves (mocc's CMFD decks use 5 inner
+        iterations per outer; with 1 the coarse system can lose its
+        diagonal dominance)."""
         self.work_dir = work_dir
         self.energy_mode = energy_mode
         self.n_inner = n_inner
+        self.use_cmfd = use_cmfd
         self._hp: dict = {}
         self._proto: dict = {}
         self._lock = threading.Lock()
@@ -75,26 +79,37 @@ class BranchFactory:
     def deck(self, c: Case) -> lg.Deck:
         return lattice_deck(c.kind, c.groups, c.spacing)
 
-    def host_problem(self, c: Case) -> HostProblem:
-        """The traced geometry of the case, loaded with the case's cross
-        sections (valid until the next call for the same geometry)."""
-        hp = self._host(c.kind, c.spacing)
+    def stage(self, c: Case) -> str:
+        """Host-only preparation of a case: trace its geometry if new and
+        write its cross-section library; returns the library's path."""
+        self._host(c.kind, c.spacing)
         d = self.deck(c)
         xsl = os.path.join(self.work_dir, f"{d.name}_b{c.seed}.xsl")
         lg.write_branch_xsl(d, self.BASE_SEED, c.seed, 0.1, xsl)
+        return xsl
+
+    def host_problem(self, c: Case, staged: str | None = None) -> HostProblem:
+        """The traced geometry of the case, loaded with the case's cross
+        sections (valid until the next call for the same geometry). staged:
+        the library written by stage(c), consumed here."""
+        xsl = staged if staged is not None else self.stage(c)
+        hp = self._host(c.kind, c.spacing)
         hp.reload_xs(xsl)
         os.remove(xsl)
         return hp
 
-    def make(self, c: Case) -> GpuMoCSweeper:
+    def make(self, c: Case, staged: str | None = None) -> GpuMoCSweeper:
+        """Device handle of a case (device allocations and copies on the
+        handle's own stream)."""
         with self._lock:
-            hp = self.host_problem(c)
+            hp = self.host_problem(c, staged)
             gkey = (c.kind, c.spacing)
             proto = self._proto.get(gkey)
             if proto is None:
-                proto = GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=False)
+                proto = GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode,
+                                      use_cmfd=self.use_cmfd)
                 self._proto[gkey] = proto
-            return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=False,
+            return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=self.use_cmfd,
                                  like=proto)
 
     def close(self):

@@ -29,6 +29,7 @@ namespace {
 
 constexpr int kCmfdThreads = 512;
 constexpr int kCmfdWaves = kCmfdThreads / 64;
+constexpr int kCmfdLdsVectors = 6;  // cell vectors of the solve in LDS (see cmfd_solve)
 enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5 };
 
 #define CMFD_CHECK(expr)                                                                          \
@@ -329,19 +330,53 @@ struct SolveArgs {
     int *info;
 };
 
+// One DPP lane move of a double (two 32-bit moves); lanes the control leaves
+// without a source, or whose row is masked off, read 0.
+template <int CTRL, int ROWS = 0xf> __device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave by DPP moves (VALU latency, no LDS trip):
+// pairs and quads (quad_perm), half rows and rows (mirrors), then rows 0+1
+// and 2+3 (row_bcast:15) and the two halves (row_bcast:31), which leaves the
+// total in lane 63. Every lane returns it.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f64<0xb1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4e>(v);        // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);       // row_half_mirror
+    v += dpp_f64<0x140>(v);       // row_mirror
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
+
+// Workgroup scratch of the coarse solve. Reductions and matrix-vector
+// products each use one of two LDS banks in turn, so each needs a single
+// barrier: a bank is rewritten only at the second use after, and every lane
+// has finished reading it before it reaches the barrier of the use between.
+// All lanes make the same sequence of calls, so the bank indices agree.
+constexpr int kRedBank = 2 * kCmfdWaves;  // doubles per reduction bank
+
+struct Scratch {
+    double *red;  // [2][kRedBank] wave partial sums
+    double *buf;  // [2][nc] operand of the matrix-vector products
+    int nc;
+    int rb, bb;   // bank of the next reduction / product
+};
 
 // Sum over the workgroup; every lane returns the same value (lane 0 of each
 // wave publishes, all lanes add the wave sums in wave order).
-__device__ __forceinline__ double block_sum(double v, double *red)
+__device__ __forceinline__ double block_sum(double v, Scratch &S)
 {
     v = wave_sum(v);
-    __syncthreads();
+    double *red = S.red + S.rb * kRedBank;
+    S.rb ^= 1;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = 0.0;
@@ -350,11 +385,12 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     return s;
 }
 
-__device__ __forceinline__ void block_sum2(double a, double b, double *red, double &sa, double &sb)
+__device__ __forceinline__ void block_sum2(double a, double b, Scratch &S, double &sa, double &sb)
 {
     a = wave_sum(a);
     b = wave_sum(b);
-    __syncthreads();
+    double *red = S.red + S.rb * kRedBank;
+    S.rb ^= 1;
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = a;
         red[kCmfdWaves + (threadIdx.x >> 6)] = b;
@@ -370,42 +406,96 @@ __device__ __forceinline__ void block_sum2(double a, double b, double *red, doub
 }
 
 template <int K> struct Lane {
-    int c[K];
+    int c[K];   // cells of this lane: threadIdx.x + k * kCmfdThreads
+    int ix[K];  // their x index
     bool on[K];
 };
 
-// v = A y for group g: y staged in LDS, rows summed in sorted column order
-// (south, west, diagonal, east, north) as a CSR product would. y and v may be
-// the same array; y stays readable in buf until the next product.
-template <int K>
-__device__ __forceinline__ void spmv(const SolveArgs &A, int g, const Lane<K> &L, const double (&y)[K],
-                                     double (&v)[K], double *buf)
+// The matrix of one group for this lane's cells. Up to 4 cells per lane the
+// five coefficients of each row stay in registers for the whole group solve;
+// with more, every product reloads them from cache (the registers would not
+// hold them beside the BiCGSTAB vectors).
+constexpr int kRegCoefMaxK = 4;
+
+template <int K> struct Rows {
+    double a[K <= kRegCoefMaxK ? 5 : 1][K <= kRegCoefMaxK ? K : 1];  // south, west, diagonal, east, north
+    const double *cf;  // coefficients of group g (reloaded when not held)
+};
+
+// Which of the five row entries (south, west, diagonal, east, north) of cell
+// c exist, and the cell each one multiplies (c itself where the neighbour is
+// missing, so that the product reads in bounds and adds an exact 0).
+__device__ __forceinline__ bool row_has(const SolveArgs &A, int c, int ix, int j)
 {
-    __syncthreads();
+    switch (j) {
+    case 0: return c >= A.nx;
+    case 1: return ix > 0;
+    case 3: return ix < A.nx - 1;
+    case 4: return c + A.nx < A.nc;
+    default: return true;
+    }
+}
+__device__ __forceinline__ int row_col(const SolveArgs &A, int c, int ix, int j)
+{
+    const int off[5] = {-A.nx, -1, 0, 1, A.nx};
+    return row_has(A, c, ix, j) ? c + off[j] : c;
+}
+
+template <int K> __device__ __forceinline__ void load_rows(const SolveArgs &A, int g, const Lane<K> &L, Rows<K> &R)
+{
+    R.cf = A.coef + (size_t)g * A.nc;
+    if constexpr (K <= kRegCoefMaxK) {
+        const size_t pl = (size_t)A.G * A.nc;
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                R.a[j][k] = (L.on[k] && row_has(A, L.c[k], L.ix[k], j)) ? R.cf[j * pl + L.c[k]] : 0.0;
+    }
+}
+
+// v = A y for group g: y staged in an LDS bank, rows summed in sorted column
+// order (south, west, diagonal, east, north) as a CSR product would. A
+// missing neighbour enters as 0 times the cell's own entry, which leaves the
+// sum unchanged, so the five reads of a row carry no branches and issue
+// together. y and v may be the same array. Returns the bank, where y stays
+// readable until the second product after this one.
+template <int K>
+__device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L,
+                                              const double (&y)[K], double (&v)[K], Scratch &S)
+{
+    double *buf = S.buf + S.bb * S.nc;
+    S.bb ^= 1;
 #pragma unroll
     for (int k = 0; k < K; k++)
         if (L.on[k]) buf[L.c[k]] = y[k];
     __syncthreads();
     const size_t pl = (size_t)A.G * A.nc;
-    // reload the coefficients from cache at every product rather than keep
-    // 5K of them live in registers across the iteration
-    const double *cf = A.coef;
-    __asm__ volatile("" : "+s"(cf));
+    const double *cf = R.cf;
+    if constexpr (K > kRegCoefMaxK) __asm__ volatile("" : "+s"(cf));
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double s = 0.0;
         if (L.on[k]) {
             const int c = L.c[k];
-            const int ix = c % A.nx;
-            const size_t i = (size_t)g * A.nc + c;
-            if (c >= A.nx) s += cf[0 * pl + i] * buf[c - A.nx];
-            if (ix > 0) s += cf[1 * pl + i] * buf[c - 1];
-            s += cf[2 * pl + i] * buf[c];
-            if (ix < A.nx - 1) s += cf[3 * pl + i] * buf[c + 1];
-            if (c + A.nx < A.nc) s += cf[4 * pl + i] * buf[c + A.nx];
+            const int ix = L.ix[k];
+            double a[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                if constexpr (K <= kRegCoefMaxK)
+                    a[j] = R.a[j][k];
+                else
+                    a[j] = row_has(A, c, ix, j) ? cf[j * pl + c] : 0.0;
+            }
+            double y5[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) y5[j] = buf[row_col(A, c, ix, j)];
+#pragma unroll
+            for (int j = 0; j < 5; j++) s += a[j] * y5[j];
         }
         v[k] = s;
     }
+    return buf;
 }
 
 // Jacobi-preconditioned BiCGSTAB with the relative test |r| <= tol |b| and a
@@ -416,19 +506,19 @@ __device__ __forceinline__ void spmv(const SolveArgs &A, int g, const Lane<K> &L
 // (the inverse diagonal, the shadow residual r0 and b) live in LDS slots that
 // only the lane owning the cell touches, which keeps the rest in registers.
 template <int K>
-__device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)[K], double tol, int max_iter,
-                        double *buf, double *red, double *sinv, double *sr0, const double *sb)
+__device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, double (&x)[K], double tol,
+                        int max_iter, Scratch &S, double *sinv, double *sr0, const double *sb)
 {
     const size_t pl = (size_t)A.G * A.nc;
     double r[K], p[K], v[K], t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (L.on[k]) {
-            const double d = A.coef[2 * pl + (size_t)g * A.nc + L.c[k]];
+            const double d = R.cf[2 * pl + L.c[k]];
             sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
         }
     }
-    spmv<K>(A, g, L, x, t, buf);
+    spmv<K>(A, R, L, x, t, S);
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -441,7 +531,7 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)
         lb += bk * bk;
     }
     double r0_sq, rhs_sq;
-    block_sum2(lr0, lb, red, r0_sq, rhs_sq);
+    block_sum2(lr0, lb, S, r0_sq, rhs_sq);
     if (rhs_sq == 0.0) {
 #pragma unroll
         for (int k = 0; k < K; k++) x[k] = 0.0;
@@ -460,12 +550,12 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)
             if (L.on[k]) lrho += sr0[L.c[k]] * r[k];
         }
         double rr, rho_new;
-        block_sum2(lrr, lrho, red, rr, rho_new);
+        block_sum2(lrr, lrho, S, rr, rho_new);
         if (!(rr > tol2 && i < max_iter)) break;
         const double rho_old = rho;
         rho = rho_new;
         if (fabs(rho) < eps2 * r0_sq) {
-            spmv<K>(A, g, L, x, t, buf);
+            spmv<K>(A, R, L, x, t, S);
             double l = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -473,7 +563,7 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)
                 if (L.on[k]) sr0[L.c[k]] = r[k];
                 l += r[k] * r[k];
             }
-            rho = r0_sq = block_sum(l, red);
+            rho = r0_sq = block_sum(l, S);
             if (restarts++ == 0) i = 0;
         }
         const double beta = (rho / rho_old) * (alpha / w);
@@ -482,19 +572,19 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)
             p[k] = r[k] + beta * (p[k] - w * v[k]);
             t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * p[k];  // y
         }
-        spmv<K>(A, g, L, t, v, buf);
+        const double *yb = spmv<K>(A, R, L, t, v, S);
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
             if (L.on[k]) l += sr0[L.c[k]] * v[k];
-        alpha = rho / block_sum(l, red);
+        alpha = rho / block_sum(l, S);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (L.on[k]) x[k] += alpha * buf[L.c[k]];
+            if (L.on[k]) x[k] += alpha * yb[L.c[k]];
             r[k] = r[k] - alpha * v[k];                     // s
             t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * r[k];  // z
         }
-        spmv<K>(A, g, L, t, t, buf);  // t = A z
+        const double *zb = spmv<K>(A, R, L, t, t, S);  // t = A z
         double ltt = 0.0, lts = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -502,11 +592,11 @@ __device__ int bicgstab(const SolveArgs &A, int g, const Lane<K> &L, double (&x)
             lts += t[k] * r[k];
         }
         double tt, ts;
-        block_sum2(ltt, lts, red, tt, ts);
+        block_sum2(ltt, lts, S, tt, ts);
         w = (tt > 0.0) ? ts / tt : 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (L.on[k]) x[k] += w * buf[L.c[k]];
+            if (L.on[k]) x[k] += w * zb[L.c[k]];
             r[k] = r[k] - w * t[k];
         }
         ++i;
@@ -554,12 +644,12 @@ __device__ __forceinline__ double fission_source(const SolveArgs &A, const Lane<
 
 // ||A x - b||^2 of group g with x the stored coarse flux (cmfd.cpp:640-666)
 template <int K>
-__device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Lane<K> &L, const double (&b)[K],
-                                            double (&x)[K], double (&t)[K], double *buf)
+__device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Rows<K> &R, const Lane<K> &L,
+                                            const double (&b)[K], double (&x)[K], double (&t)[K], Scratch &S)
 {
 #pragma unroll
     for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * A.G + g] : 0.0;
-    spmv<K>(A, g, L, x, t, buf);
+    spmv<K>(A, R, L, x, t, S);
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) s += (t[k] - b[k]) * (t[k] - b[k]);
@@ -567,28 +657,31 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Lan
 }
 
 // CMFD::solve (cmfd.cpp:171-279) by one workgroup of kCmfdThreads lanes;
-// lds holds 5 * nc doubles, red 2 * kCmfdWaves.
+// lds holds kCmfdLdsVectors * nc doubles, red 2 * kRedBank.
 template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
-    double *buf = lds;                // [nc] operand of the matrix-vector products
-    double *fs_old = lds + A.nc;      // [nc] previous coarse fission source
-    double *sinv = lds + 2 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
-    double *sr0 = lds + 3 * A.nc;     // [nc] BiCGSTAB: shadow residual
-    double *sb = lds + 4 * A.nc;      // [nc] BiCGSTAB: right-hand side
+    Scratch S{red, lds, A.nc, 0, 0};  // [2][nc] product operands
+    double *fs_old = lds + 2 * A.nc;  // [nc] previous coarse fission source
+    double *sinv = lds + 3 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
+    double *sr0 = lds + 4 * A.nc;     // [nc] BiCGSTAB: shadow residual
+    double *sb = lds + 5 * A.nc;      // [nc] BiCGSTAB: right-hand side
     Lane<K> L;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         L.c[k] = threadIdx.x + k * kCmfdThreads;
+        L.ix[k] = L.c[k] % A.nx;
         L.on[k] = L.c[k] < A.nc;
     }
     const int G = A.G;
     double k_eff = A.k[0];
     double fsc[K], b[K], x[K], t[K];
-    double tfis = block_sum(fission_source<K>(A, L, k_eff, fsc), red);
+    Rows<K> R;
+    double tfis = block_sum(fission_source<K>(A, L, k_eff, fsc), S);
     double r0 = 0.0;
     for (int g = 0; g < G; g++) {
         cmfd_source<K>(A, g, L, fsc, b);
-        r0 += block_sum(residual2<K>(A, g, L, b, x, t, buf), red);
+        load_rows<K>(A, g, L, R);
+        r0 += block_sum(residual2<K>(A, g, R, L, b, x, t, S), S);
     }
     r0 = sqrt(r0) / (G * A.nc);
     const double tol = A.rr * r0;
@@ -603,11 +696,12 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
         double ri = 0.0;
         for (int g = 0; g < G; g++) {
             cmfd_source<K>(A, g, L, fsc, b);
-            ri += block_sum(residual2<K>(A, g, L, b, x, t, buf), red);
+            load_rows<K>(A, g, L, R);
+            ri += block_sum(residual2<K>(A, g, R, L, b, x, t, S), S);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) sb[L.c[k]] = b[k];
-            n_bicg += bicgstab<K>(A, g, L, x, tol, 150, buf, red, sinv, sr0, sb);
+            n_bicg += bicgstab<K>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
@@ -618,14 +712,14 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
         for (int k = 0; k < K; k++)
             if (L.on[k])
                 for (int g = 0; g < G; g++) lf += A.hnf[(size_t)L.c[k] * G + g] * A.cflux[(size_t)L.c[k] * G + g];
-        tfis = block_sum(lf, red);
+        tfis = block_sum(lf, S);
         const double k_old = k_eff;
         k_eff = k_eff * tfis / tfis_old;
         double le = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
             if (L.on[k]) le += (fsc[k] - fs_old[L.c[k]]) * (fsc[k] - fs_old[L.c[k]]);
-        const double psi_err = sqrt(block_sum(le, red));
+        const double psi_err = sqrt(block_sum(le, S));
         if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
             break;
     }
@@ -646,7 +740,7 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
 template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(SolveArgs A)
 {
     extern __shared__ double lds[];
-    __shared__ double red[2 * kCmfdWaves];
+    __shared__ double red[2 * kRedBank];
     cmfd_solve<K>(A, lds, red);
 }
 
@@ -697,7 +791,7 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
         int dev = 0, lds_max = 0;
         CMFD_CHECK(hipGetDevice(&dev));
         CMFD_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = 5 * (size_t)nc_ * sizeof(double) + 4 * kCmfdWaves * sizeof(double);
+        const size_t need = kCmfdLdsVectors * (size_t)nc_ * sizeof(double) + 2 * kRedBank * sizeof(double);
         if (need > (size_t)lds_max)
             throw std::runtime_error("CMFD mesh of " + std::to_string(nc_) + " cells needs " + std::to_string(need) +
                                      " bytes of LDS per workgroup; the device allows " + std::to_string(lds_max));
@@ -806,7 +900,7 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     hipLaunchKernelGGL(k_assemble, dim3(nblk((size_t)nc_ * G_)), dim3(256), 0, s, ca.m, G_, d_hrm_, d_dtil_,
                        d_dhat_, d_coef_);
     const SolveArgs a = CmfdAccess::solve(*this, v);
-    const size_t lds = 5 * (size_t)nc_ * sizeof(double);  // buf, fs_old and the BiCGSTAB vectors
+    const size_t lds = kCmfdLdsVectors * (size_t)nc_ * sizeof(double);  // operands, fs_old, BiCGSTAB vectors
     const int kneed = (nc_ + kCmfdThreads - 1) / kCmfdThreads;
     if (kneed <= 1) hipLaunchKernelGGL(k_cmfd_solve<1>, dim3(1), dim3(kCmfdThreads), lds, s, a);
     else if (kneed <= 2) hipLaunchKernelGGL(k_cmfd_solve<2>, dim3(1), dim3(kCmfdThreads), lds, s, a);
@@ -915,7 +1009,7 @@ template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_b_solve(
                                                                               const int32_t *__restrict__ status)
 {
     extern __shared__ double lds[];
-    __shared__ double red[2 * kCmfdWaves];
+    __shared__ double red[2 * kRedBank];
     const CmfdCase &d = cs[blockIdx.x];
     if (case_done(d, status)) return;
     cmfd_solve<K>(d.a, lds, red);
@@ -995,7 +1089,7 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
                                   hipMemcpyHostToDevice, s));
     }
     impl_->kmax = (ncmax + kCmfdThreads - 1) / kCmfdThreads;
-    impl_->lds = 5 * (size_t)ncmax * sizeof(double);
+    impl_->lds = kCmfdLdsVectors * (size_t)ncmax * sizeof(double);
     CMFD_CHECK(hipStreamSynchronize(s));
 }
 

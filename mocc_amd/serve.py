@@ -6,9 +6,14 @@ device-memory budget has room for its state (FIFO); every admitted problem
 keeps its flux, sources and boundary fluxes on the device until it finishes.
 The device advances all admitted problems together a few outer iterations
 at a time; finished problems leave and free their memory, and waiting ones
-join at the next iteration boundary. While the device works on one chunk,
-the host prepares the handles of requests that have arrived (library reload,
-device allocation) so that admission itself is a small copy.
+join at the next iteration boundary.
+
+While the device works on one chunk, the host does only host work for the
+requests that have arrived (their cross-section libraries). Their device
+handles are made once the chunk has finished: with more streams than the
+device's hardware queues (GPU_MAX_HW_QUEUES, 4 by default), a handle's stream
+can share the batch stream's queue, and its set-up copies would wait behind
+the running chunk.
 """
 from __future__ import annotations
 
@@ -57,17 +62,18 @@ class Server:
     def serve(self, arrivals, cases: list[Case]) -> list[Record]:
         recs = [Record(i, c, float(t)) for i, (t, c) in enumerate(zip(arrivals, cases))]
         future = deque(range(len(recs)))  # not yet arrived
-        ready = deque()                   # arrived: (index, prepared handle)
+        staged = deque()                  # arrived: (index, cross-section library)
+        ready = deque()                   # (index, device handle) waiting for room
         active = {}                       # batch id -> (index, handle)
         b = self.batch
         t0 = time.perf_counter()
         busy = False
-        while future or ready or active:
+        while future or staged or ready or active:
             now = time.perf_counter() - t0
             # host work while the device runs the previous chunk
             while future and recs[future[0]].t_arrive <= now:
                 i = future.popleft()
-                ready.append((i, self.factory.make(recs[i].case)))
+                staged.append((i, self.factory.stage(recs[i].case)))
             if busy:
                 for cid, st in b.poll():
                     i, s = active.pop(cid)
@@ -79,6 +85,10 @@ class Server:
                     b.remove(cid)
                     s.close()
                 busy = False
+            # the device is idle: make the handles of the staged requests
+            while staged:
+                i, xsl = staged.popleft()
+                ready.append((i, self.factory.make(recs[i].case, xsl)))
             while ready:
                 i, s = ready[0]
                 try:
