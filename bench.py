@@ -83,10 +83,13 @@ K_TOL, PSI_TOL, MAX_IT = 1e-7, 1e-6, 3000
 
 
 def oracle_k(factory, case):
-    """FP64 eigenvalue of a branch case from the oracle (reference defaults)."""
+    """FP64 eigenvalue of a branch case from the oracle, run with the
+    device's iteration scheme (inner iterations, CMFD, energy mode): Jacobi
+    and Gauss-Seidel in energy stop at different distances from the fixed
+    point (up to 2e-6 in k on 7-group pins at k_tol 1e-7)."""
     from oracle.binding import Oracle
     hp = factory.host_problem(case)
-    o = Oracle(hp, n_inner=factory.n_inner, use_cmfd=factory.use_cmfd)
+    o = Oracle(hp, n_inner=factory.n_inner, use_cmfd=factory.use_cmfd, energy_mode=factory.energy_mode)
     rc, st, _ = o.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
     return st.k
 
@@ -152,7 +155,8 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     }
 
 
-def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, use_cmfd: bool = True):
+def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, use_cmfd: bool = True,
+                  n_serial: int = 64):
     """Poisson arrivals of branch-case requests served by the continuous-
     batching loop at the given fractions of its burst (saturation)
     throughput, beside the same requests solved one at a time (CMFD with 5
@@ -179,7 +183,8 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
                                      **summarize(srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases))}
         progress(f"serving at {frac:g}x burst done")
     srv.close()
-    out["serial_burst"] = summarize(serve_serial(factory, np.zeros(n_req), cases))
+    n_ser = min(n_req, n_serial)
+    out["serial_burst"] = {"sample": n_ser, **summarize(serve_serial(factory, np.zeros(n_ser), cases[:n_ser]))}
     factory.close()
     return out
 
@@ -247,8 +252,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--serial-cases", type=int, default=32)
-    ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--serve-rates", default="0.5,1.0",
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rates", default="0.25,0.5,0.75,1.0,1.25",
                     help="comma-separated arrival rates as fractions of the burst throughput")
     ap.add_argument("--c3", action="store_true", help="add the full C3 core solve with CMFD")
     ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
