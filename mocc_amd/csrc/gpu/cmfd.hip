@@ -361,10 +361,13 @@ __device__ __forceinline__ double wave_sum(double v)
 // barrier: a bank is rewritten only at the second use after, and every lane
 // has finished reading it before it reaches the barrier of the use between.
 // All lanes make the same sequence of calls, so the bank indices agree.
-constexpr int kRedBank = 2 * kCmfdWaves;  // doubles per reduction bank
+// The solve runs in workgroups of NT lanes (NT / 64 waves). NT is 512 for
+// every mesh: with more lanes the per-lane register budget (4 cells' rows and
+// BiCGSTAB vectors need about 240 VGPRs) is exceeded and the solve spills.
+template <int NT> constexpr int red_bank() { return 2 * (NT / 64); }  // doubles per reduction bank
 
-struct Scratch {
-    double *red;  // [2][kRedBank] wave partial sums
+template <int NT> struct Scratch {
+    double *red;  // [2][red_bank<NT>()] wave partial sums
     double *buf;  // [2][nc] operand of the matrix-vector products
     int nc;
     int rb, bb;   // bank of the next reduction / product
@@ -372,41 +375,43 @@ struct Scratch {
 
 // Sum over the workgroup; every lane returns the same value (lane 0 of each
 // wave publishes, all lanes add the wave sums in wave order).
-__device__ __forceinline__ double block_sum(double v, Scratch &S)
+template <int NT> __device__ __forceinline__ double block_sum(double v, Scratch<NT> &S)
 {
+    constexpr int NW = NT / 64;
     v = wave_sum(v);
-    double *red = S.red + S.rb * kRedBank;
+    double *red = S.red + S.rb * red_bank<NT>();
     S.rb ^= 1;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kCmfdWaves; w++) s += red[w];
+    for (int w = 0; w < NW; w++) s += red[w];
     return s;
 }
 
-__device__ __forceinline__ void block_sum2(double a, double b, Scratch &S, double &sa, double &sb)
+template <int NT> __device__ __forceinline__ void block_sum2(double a, double b, Scratch<NT> &S, double &sa, double &sb)
 {
+    constexpr int NW = NT / 64;
     a = wave_sum(a);
     b = wave_sum(b);
-    double *red = S.red + S.rb * kRedBank;
+    double *red = S.red + S.rb * red_bank<NT>();
     S.rb ^= 1;
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = a;
-        red[kCmfdWaves + (threadIdx.x >> 6)] = b;
+        red[NW + (threadIdx.x >> 6)] = b;
     }
     __syncthreads();
     sa = 0.0;
     sb = 0.0;
 #pragma unroll
-    for (int w = 0; w < kCmfdWaves; w++) {
+    for (int w = 0; w < NW; w++) {
         sa += red[w];
-        sb += red[kCmfdWaves + w];
+        sb += red[NW + w];
     }
 }
 
 template <int K> struct Lane {
-    int c[K];   // cells of this lane: threadIdx.x + k * kCmfdThreads
+    int c[K];   // cells of this lane: threadIdx.x + k * NT
     int ix[K];  // their x index
     bool on[K];
 };
@@ -460,9 +465,9 @@ template <int K> __device__ __forceinline__ void load_rows(const SolveArgs &A, i
 // sum unchanged, so the five reads of a row carry no branches and issue
 // together. y and v may be the same array. Returns the bank, where y stays
 // readable until the second product after this one.
-template <int K>
+template <int K, int NT>
 __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L,
-                                              const double (&y)[K], double (&v)[K], Scratch &S)
+                                              const double (&y)[K], double (&v)[K], Scratch<NT> &S)
 {
     double *buf = S.buf + S.bb * S.nc;
     S.bb ^= 1;
@@ -505,9 +510,9 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
 // operand, so x is updated as (x + alpha y) + w z. The loop-invariant vectors
 // (the inverse diagonal, the shadow residual r0 and b) live in LDS slots that
 // only the lane owning the cell touches, which keeps the rest in registers.
-template <int K>
+template <int K, int NT>
 __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, double (&x)[K], double tol,
-                        int max_iter, Scratch &S, double *sinv, double *sr0, const double *sb)
+                        int max_iter, Scratch<NT> &S, double *sinv, double *sr0, const double *sb)
 {
     const size_t pl = (size_t)A.G * A.nc;
     double r[K], p[K], v[K], t[K];
@@ -518,7 +523,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
             sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
         }
     }
-    spmv<K>(A, R, L, x, t, S);
+    spmv<K, NT>(A, R, L, x, t, S);
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -555,7 +560,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         const double rho_old = rho;
         rho = rho_new;
         if (fabs(rho) < eps2 * r0_sq) {
-            spmv<K>(A, R, L, x, t, S);
+            spmv<K, NT>(A, R, L, x, t, S);
             double l = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -572,7 +577,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
             p[k] = r[k] + beta * (p[k] - w * v[k]);
             t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * p[k];  // y
         }
-        const double *yb = spmv<K>(A, R, L, t, v, S);
+        const double *yb = spmv<K, NT>(A, R, L, t, v, S);
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -584,7 +589,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
             r[k] = r[k] - alpha * v[k];                     // s
             t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * r[k];  // z
         }
-        const double *zb = spmv<K>(A, R, L, t, t, S);  // t = A z
+        const double *zb = spmv<K, NT>(A, R, L, t, t, S);  // t = A z
         double ltt = 0.0, lts = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -643,24 +648,24 @@ __device__ __forceinline__ double fission_source(const SolveArgs &A, const Lane<
 }
 
 // ||A x - b||^2 of group g with x the stored coarse flux (cmfd.cpp:640-666)
-template <int K>
+template <int K, int NT>
 __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Rows<K> &R, const Lane<K> &L,
-                                            const double (&b)[K], double (&x)[K], double (&t)[K], Scratch &S)
+                                            const double (&b)[K], double (&x)[K], double (&t)[K], Scratch<NT> &S)
 {
 #pragma unroll
     for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * A.G + g] : 0.0;
-    spmv<K>(A, R, L, x, t, S);
+    spmv<K, NT>(A, R, L, x, t, S);
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) s += (t[k] - b[k]) * (t[k] - b[k]);
     return s;
 }
 
-// CMFD::solve (cmfd.cpp:171-279) by one workgroup of kCmfdThreads lanes;
-// lds holds kCmfdLdsVectors * nc doubles, red 2 * kRedBank.
-template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
+// CMFD::solve (cmfd.cpp:171-279) by one workgroup of NT lanes, K cells per
+// lane; lds holds kCmfdLdsVectors * nc doubles, red 2 * red_bank<NT>().
+template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
-    Scratch S{red, lds, A.nc, 0, 0};  // [2][nc] product operands
+    Scratch<NT> S{red, lds, A.nc, 0, 0};  // [2][nc] product operands
     double *fs_old = lds + 2 * A.nc;  // [nc] previous coarse fission source
     double *sinv = lds + 3 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
     double *sr0 = lds + 4 * A.nc;     // [nc] BiCGSTAB: shadow residual
@@ -668,7 +673,7 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
     Lane<K> L;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        L.c[k] = threadIdx.x + k * kCmfdThreads;
+        L.c[k] = threadIdx.x + k * NT;
         L.ix[k] = L.c[k] % A.nx;
         L.on[k] = L.c[k] < A.nc;
     }
@@ -681,7 +686,7 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
     for (int g = 0; g < G; g++) {
         cmfd_source<K>(A, g, L, fsc, b);
         load_rows<K>(A, g, L, R);
-        r0 += block_sum(residual2<K>(A, g, R, L, b, x, t, S), S);
+        r0 += block_sum(residual2<K, NT>(A, g, R, L, b, x, t, S), S);
     }
     r0 = sqrt(r0) / (G * A.nc);
     const double tol = A.rr * r0;
@@ -697,11 +702,11 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
         for (int g = 0; g < G; g++) {
             cmfd_source<K>(A, g, L, fsc, b);
             load_rows<K>(A, g, L, R);
-            ri += block_sum(residual2<K>(A, g, R, L, b, x, t, S), S);
+            ri += block_sum(residual2<K, NT>(A, g, R, L, b, x, t, S), S);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) sb[L.c[k]] = b[k];
-            n_bicg += bicgstab<K>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
+            n_bicg += bicgstab<K, NT>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
@@ -737,11 +742,11 @@ template <int K> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, 
     }
 }
 
-template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_solve(SolveArgs A)
+template <int K, int NT> __global__ __launch_bounds__(NT) void k_cmfd_solve(SolveArgs A)
 {
     extern __shared__ double lds[];
-    __shared__ double red[2 * kRedBank];
-    cmfd_solve<K>(A, lds, red);
+    __shared__ double red[2 * red_bank<NT>()];
+    cmfd_solve<K, NT>(A, lds, red);
 }
 
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -791,7 +796,7 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
         int dev = 0, lds_max = 0;
         CMFD_CHECK(hipGetDevice(&dev));
         CMFD_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = kCmfdLdsVectors * (size_t)nc_ * sizeof(double) + 2 * kRedBank * sizeof(double);
+        const size_t need = kCmfdLdsVectors * (size_t)nc_ * sizeof(double) + 2 * red_bank<512>() * sizeof(double);
         if (need > (size_t)lds_max)
             throw std::runtime_error("CMFD mesh of " + std::to_string(nc_) + " cells needs " + std::to_string(need) +
                                      " bytes of LDS per workgroup; the device allows " + std::to_string(lds_max));
@@ -902,10 +907,10 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     const SolveArgs a = CmfdAccess::solve(*this, v);
     const size_t lds = kCmfdLdsVectors * (size_t)nc_ * sizeof(double);  // operands, fs_old, BiCGSTAB vectors
     const int kneed = (nc_ + kCmfdThreads - 1) / kCmfdThreads;
-    if (kneed <= 1) hipLaunchKernelGGL(k_cmfd_solve<1>, dim3(1), dim3(kCmfdThreads), lds, s, a);
-    else if (kneed <= 2) hipLaunchKernelGGL(k_cmfd_solve<2>, dim3(1), dim3(kCmfdThreads), lds, s, a);
-    else if (kneed <= 4) hipLaunchKernelGGL(k_cmfd_solve<4>, dim3(1), dim3(kCmfdThreads), lds, s, a);
-    else hipLaunchKernelGGL(k_cmfd_solve<6>, dim3(1), dim3(kCmfdThreads), lds, s, a);
+    if (kneed <= 1) hipLaunchKernelGGL((k_cmfd_solve<1, 512>), dim3(1), dim3(512), lds, s, a);
+    else if (kneed <= 2) hipLaunchKernelGGL((k_cmfd_solve<2, 512>), dim3(1), dim3(512), lds, s, a);
+    else if (kneed <= 4) hipLaunchKernelGGL((k_cmfd_solve<4, 512>), dim3(1), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((k_cmfd_solve<6, 512>), dim3(1), dim3(512), lds, s, a);
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
     hipLaunchKernelGGL(k_set_pin_flux_all, dim3(nblk(npg)), dim3(256), 0, s, v, d_cflux_, d_info_ + 1);
     CMFD_CHECK(hipGetLastError());
@@ -1005,14 +1010,14 @@ __global__ void k_cmfd_b_elem(const CmfdCase *__restrict__ cs, const CaseBlock *
     if (STAGE == 5) set_pin_flux_item(d.v, d.a.cflux, d.info + 1, t);
 }
 
-template <int K> __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_b_solve(const CmfdCase *__restrict__ cs,
+template <int K, int NT> __global__ __launch_bounds__(NT) void k_cmfd_b_solve(const CmfdCase *__restrict__ cs,
                                                                               const int32_t *__restrict__ status)
 {
     extern __shared__ double lds[];
-    __shared__ double red[2 * kRedBank];
+    __shared__ double red[2 * red_bank<NT>()];
     const CmfdCase &d = cs[blockIdx.x];
     if (case_done(d, status)) return;
-    cmfd_solve<K>(d.a, lds, red);
+    cmfd_solve<K, NT>(d.a, lds, red);
 }
 
 std::vector<CaseBlock> case_blocks(const std::vector<int> &counts)
@@ -1102,10 +1107,10 @@ void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
     hipLaunchKernelGGL(k_cmfd_b_elem<1>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<2>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<3>, dim3(m.nbm[2]), dim3(256), 0, s, m.cases, m.bm[2], status);
-    if (m.kmax <= 1) hipLaunchKernelGGL(k_cmfd_b_solve<1>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
-    else if (m.kmax <= 2) hipLaunchKernelGGL(k_cmfd_b_solve<2>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
-    else if (m.kmax <= 4) hipLaunchKernelGGL(k_cmfd_b_solve<4>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
-    else hipLaunchKernelGGL(k_cmfd_b_solve<6>, dim3(n_), dim3(kCmfdThreads), m.lds, s, m.cases, status);
+    if (m.kmax <= 1) hipLaunchKernelGGL((k_cmfd_b_solve<1, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
+    else if (m.kmax <= 2) hipLaunchKernelGGL((k_cmfd_b_solve<2, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
+    else if (m.kmax <= 4) hipLaunchKernelGGL((k_cmfd_b_solve<4, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
+    else hipLaunchKernelGGL((k_cmfd_b_solve<6, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
     hipLaunchKernelGGL(k_cmfd_b_elem<4>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<5>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
 }

@@ -65,3 +65,32 @@ def test_serve_under_memory_budget(factory):
     recs = srv.serve(poisson_arrivals(len(cases), rate=1000.0, seed=5), cases)
     srv.close()
     assert all(r.converged for r in recs)
+
+
+@pytest.fixture(scope="module")
+def cmfd_factory(tmp_path_factory):
+    from mocc_amd.branch import BranchFactory
+    f = BranchFactory(str(tmp_path_factory.mktemp("branch_cmfd")), n_inner=5, use_cmfd=True)
+    yield f
+    f.close()
+
+
+def test_serve_cmfd_matches_alone(cmfd_factory):
+    """Serving with CMFD (5 inner iterations), including 2x2 assemblies of
+    17x17 pins: an even pin count, whose rays graze pin corners. Every request
+    converges in tens of outer iterations, and a request's answer is bitwise
+    the answer of the same request solved alone."""
+    from mocc_amd.branch import Case
+    from mocc_amd.serve import Server, poisson_arrivals, summarize
+    kinds = ["2x2x17x17", "pin", "3x3", "17x17", "2x2x17x17", "3x3"]
+    cases = [Case(k, 2 if i % 2 == 0 else 7, 0.1, 700 + i) for i, k in enumerate(kinds)]
+    srv = Server(cmfd_factory, chunk=2, keep_flux=True)
+    recs = srv.serve(poisson_arrivals(len(cases), rate=100.0, seed=7), cases)
+    srv.close()
+    print(summarize(recs), flush=True)
+    assert all(r.converged and r.iterations < 40 for r in recs), [(r.case.kind, r.iterations) for r in recs]
+    for r in recs[::2]:
+        print(f"request {r.index} {r.case}", flush=True)
+        st, flux = _solve_alone(cmfd_factory, r.case)
+        assert st.k == r.k and st.iterations == r.iterations
+        assert np.array_equal(flux, r.flux)
