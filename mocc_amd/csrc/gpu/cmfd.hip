@@ -30,6 +30,14 @@ namespace {
 constexpr int kCmfdThreads = 512;
 constexpr int kCmfdWaves = kCmfdThreads / 64;
 constexpr int kCmfdLdsVectors = 6;  // cell vectors of the solve in LDS (see cmfd_solve)
+
+// Cells per lane of a mesh of nc cells (1..6), and the LDS the solve uses:
+// each vector holds K * kCmfdThreads entries, the mesh cells and the dummies.
+inline int cmfd_cells_per_lane(int nc) { return std::max(1, (nc + kCmfdThreads - 1) / kCmfdThreads); }
+inline size_t cmfd_lds_bytes(int nc)
+{
+    return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * kCmfdThreads * sizeof(double);
+}
 enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5 };
 
 #define CMFD_CHECK(expr)                                                                          \
@@ -368,8 +376,8 @@ template <int NT> constexpr int red_bank() { return 2 * (NT / 64); }  // doubles
 
 template <int NT> struct Scratch {
     double *red;  // [2][red_bank<NT>()] wave partial sums
-    double *buf;  // [2][nc] operand of the matrix-vector products
-    int nc;
+    double *buf;  // [2][n] operand of the matrix-vector products
+    int n;        // entries per bank (K * NT)
     int rb, bb;   // bank of the next reduction / product
 };
 
@@ -410,11 +418,21 @@ template <int NT> __device__ __forceinline__ void block_sum2(double a, double b,
     }
 }
 
+// A lane's K cell slots: c = threadIdx.x + k * NT. Slots past the mesh (c >=
+// nc) are dummy cells of the LDS vectors (K * NT entries): their rows are
+// zero and reach no neighbour, and their vector entries stay exact zeros
+// (inverse diagonal 1), so the BiCGSTAB loop runs without per-slot branches
+// and the dummies add nothing to any sum.
 template <int K> struct Lane {
     int c[K];   // cells of this lane: threadIdx.x + k * NT
     int ix[K];  // their x index
-    bool on[K];
+    bool on[K]; // c < nc (a mesh cell)
 };
+
+// Up to 3 cells per lane the loop computes the dummy slots too (no branches);
+// with more, per-slot branches skip them: without those, the compiler keeps
+// enough values live to spill at 4 cells per lane and beyond.
+template <int K> constexpr bool branch_free() { return K <= 3; }
 
 // The matrix of one group for this lane's cells. Up to 4 cells per lane the
 // five coefficients of each row stay in registers for the whole group solve;
@@ -469,11 +487,11 @@ template <int K, int NT>
 __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L,
                                               const double (&y)[K], double (&v)[K], Scratch<NT> &S)
 {
-    double *buf = S.buf + S.bb * S.nc;
+    double *buf = S.buf + S.bb * (K * NT);
     S.bb ^= 1;
 #pragma unroll
     for (int k = 0; k < K; k++)
-        if (L.on[k]) buf[L.c[k]] = y[k];
+        if (branch_free<K>() || L.on[k]) buf[L.c[k]] = y[k];
     __syncthreads();
     const size_t pl = (size_t)A.G * A.nc;
     const double *cf = R.cf;
@@ -481,23 +499,25 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double s = 0.0;
-        if (L.on[k]) {
-            const int c = L.c[k];
-            const int ix = L.ix[k];
-            double a[5];
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-                if constexpr (K <= kRegCoefMaxK)
-                    a[j] = R.a[j][k];
-                else
-                    a[j] = row_has(A, c, ix, j) ? cf[j * pl + c] : 0.0;
-            }
-            double y5[5];
-#pragma unroll
-            for (int j = 0; j < 5; j++) y5[j] = buf[row_col(A, c, ix, j)];
-#pragma unroll
-            for (int j = 0; j < 5; j++) s += a[j] * y5[j];
+        if (!branch_free<K>() && !L.on[k]) {
+            v[k] = 0.0;
+            continue;
         }
+        const int c = L.c[k];
+        const int ix = L.ix[k];
+        double a[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            if constexpr (K <= kRegCoefMaxK)
+                a[j] = R.a[j][k];
+            else
+                a[j] = (L.on[k] && row_has(A, c, ix, j)) ? cf[j * pl + c] : 0.0;
+        }
+        double y5[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) y5[j] = buf[L.on[k] ? row_col(A, c, ix, j) : c];
+#pragma unroll
+        for (int j = 0; j < 5; j++) s += a[j] * y5[j];
         v[k] = s;
     }
     return buf;
@@ -518,18 +538,17 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
     double r[K], p[K], v[K], t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        if (L.on[k]) {
-            const double d = R.cf[2 * pl + L.c[k]];
-            sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
-        }
+        if (!branch_free<K>() && !L.on[k]) continue;
+        const double d = L.on[k] ? R.cf[2 * pl + L.c[k]] : 0.0;
+        sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
     }
     spmv<K, NT>(A, R, L, x, t, S);
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const double bk = L.on[k] ? sb[L.c[k]] : 0.0;
+        const double bk = (branch_free<K>() || L.on[k]) ? sb[L.c[k]] : 0.0;
         r[k] = bk - t[k];
-        if (L.on[k]) sr0[L.c[k]] = r[k];
+        if (branch_free<K>() || L.on[k]) sr0[L.c[k]] = r[k];
         p[k] = 0.0;
         v[k] = 0.0;
         lr0 += r[k] * r[k];
@@ -552,7 +571,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
 #pragma unroll
         for (int k = 0; k < K; k++) {
             lrr += r[k] * r[k];
-            if (L.on[k]) lrho += sr0[L.c[k]] * r[k];
+            if (branch_free<K>() || L.on[k]) lrho += sr0[L.c[k]] * r[k];
         }
         double rr, rho_new;
         block_sum2(lrr, lrho, S, rr, rho_new);
@@ -564,8 +583,8 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
             double l = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                r[k] = (L.on[k] ? sb[L.c[k]] : 0.0) - t[k];
-                if (L.on[k]) sr0[L.c[k]] = r[k];
+                r[k] = ((branch_free<K>() || L.on[k]) ? sb[L.c[k]] : 0.0) - t[k];
+                if (branch_free<K>() || L.on[k]) sr0[L.c[k]] = r[k];
                 l += r[k] * r[k];
             }
             rho = r0_sq = block_sum(l, S);
@@ -575,19 +594,19 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
 #pragma unroll
         for (int k = 0; k < K; k++) {
             p[k] = r[k] + beta * (p[k] - w * v[k]);
-            t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * p[k];  // y
+            t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * p[k];  // y
         }
         const double *yb = spmv<K, NT>(A, R, L, t, v, S);
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
-            if (L.on[k]) l += sr0[L.c[k]] * v[k];
+            if (branch_free<K>() || L.on[k]) l += sr0[L.c[k]] * v[k];
         alpha = rho / block_sum(l, S);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (L.on[k]) x[k] += alpha * yb[L.c[k]];
+            if (branch_free<K>() || L.on[k]) x[k] += alpha * yb[L.c[k]];
             r[k] = r[k] - alpha * v[k];                     // s
-            t[k] = (L.on[k] ? sinv[L.c[k]] : 1.0) * r[k];  // z
+            t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * r[k];  // z
         }
         const double *zb = spmv<K, NT>(A, R, L, t, t, S);  // t = A z
         double ltt = 0.0, lts = 0.0;
@@ -601,7 +620,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         w = (tt > 0.0) ? ts / tt : 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (L.on[k]) x[k] += w * zb[L.c[k]];
+            if (branch_free<K>() || L.on[k]) x[k] += w * zb[L.c[k]];
             r[k] = r[k] - w * t[k];
         }
         ++i;
@@ -662,14 +681,15 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Row
 }
 
 // CMFD::solve (cmfd.cpp:171-279) by one workgroup of NT lanes, K cells per
-// lane; lds holds kCmfdLdsVectors * nc doubles, red 2 * red_bank<NT>().
+// lane; lds holds kCmfdLdsVectors * K * NT doubles, red 2 * red_bank<NT>().
 template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
-    Scratch<NT> S{red, lds, A.nc, 0, 0};  // [2][nc] product operands
-    double *fs_old = lds + 2 * A.nc;  // [nc] previous coarse fission source
-    double *sinv = lds + 3 * A.nc;    // [nc] BiCGSTAB: inverse diagonal
-    double *sr0 = lds + 4 * A.nc;     // [nc] BiCGSTAB: shadow residual
-    double *sb = lds + 5 * A.nc;      // [nc] BiCGSTAB: right-hand side
+    constexpr int KN = K * NT;            // entries of each LDS vector (cells and dummies)
+    Scratch<NT> S{red, lds, KN, 0, 0};    // [2][KN] product operands
+    double *fs_old = lds + 2 * KN;        // [KN] previous coarse fission source
+    double *sinv = lds + 3 * KN;          // [KN] BiCGSTAB: inverse diagonal
+    double *sr0 = lds + 4 * KN;           // [KN] BiCGSTAB: shadow residual
+    double *sb = lds + 5 * KN;            // [KN] BiCGSTAB: right-hand side
     Lane<K> L;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -705,7 +725,7 @@ template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveA
             ri += block_sum(residual2<K, NT>(A, g, R, L, b, x, t, S), S);
 #pragma unroll
             for (int k = 0; k < K; k++)
-                if (L.on[k]) sb[L.c[k]] = b[k];
+                if (branch_free<K>() || L.on[k]) sb[L.c[k]] = b[k];  // 0 at dummies
             n_bicg += bicgstab<K, NT>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
@@ -796,7 +816,7 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
         int dev = 0, lds_max = 0;
         CMFD_CHECK(hipGetDevice(&dev));
         CMFD_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = kCmfdLdsVectors * (size_t)nc_ * sizeof(double) + 2 * red_bank<512>() * sizeof(double);
+        const size_t need = cmfd_lds_bytes(nc_) + 2 * red_bank<kCmfdThreads>() * sizeof(double);
         if (need > (size_t)lds_max)
             throw std::runtime_error("CMFD mesh of " + std::to_string(nc_) + " cells needs " + std::to_string(need) +
                                      " bytes of LDS per workgroup; the device allows " + std::to_string(lds_max));
@@ -905,12 +925,15 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     hipLaunchKernelGGL(k_assemble, dim3(nblk((size_t)nc_ * G_)), dim3(256), 0, s, ca.m, G_, d_hrm_, d_dtil_,
                        d_dhat_, d_coef_);
     const SolveArgs a = CmfdAccess::solve(*this, v);
-    const size_t lds = kCmfdLdsVectors * (size_t)nc_ * sizeof(double);  // operands, fs_old, BiCGSTAB vectors
-    const int kneed = (nc_ + kCmfdThreads - 1) / kCmfdThreads;
-    if (kneed <= 1) hipLaunchKernelGGL((k_cmfd_solve<1, 512>), dim3(1), dim3(512), lds, s, a);
-    else if (kneed <= 2) hipLaunchKernelGGL((k_cmfd_solve<2, 512>), dim3(1), dim3(512), lds, s, a);
-    else if (kneed <= 4) hipLaunchKernelGGL((k_cmfd_solve<4, 512>), dim3(1), dim3(512), lds, s, a);
-    else hipLaunchKernelGGL((k_cmfd_solve<6, 512>), dim3(1), dim3(512), lds, s, a);
+    const size_t lds = cmfd_lds_bytes(nc_);  // operands, fs_old, BiCGSTAB vectors
+    switch (cmfd_cells_per_lane(nc_)) {
+    case 1: hipLaunchKernelGGL((k_cmfd_solve<1, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 5: hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    default: hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    }
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
     hipLaunchKernelGGL(k_set_pin_flux_all, dim3(nblk(npg)), dim3(256), 0, s, v, d_cflux_, d_info_ + 1);
     CMFD_CHECK(hipGetLastError());
@@ -1093,8 +1116,8 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
         CMFD_CHECK(hipMemcpyAsync(impl_->bm[k], maps[k].data(), maps[k].size() * sizeof(CaseBlock),
                                   hipMemcpyHostToDevice, s));
     }
-    impl_->kmax = (ncmax + kCmfdThreads - 1) / kCmfdThreads;
-    impl_->lds = kCmfdLdsVectors * (size_t)ncmax * sizeof(double);
+    impl_->kmax = cmfd_cells_per_lane(ncmax);
+    impl_->lds = cmfd_lds_bytes(ncmax);
     CMFD_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1107,10 +1130,15 @@ void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
     hipLaunchKernelGGL(k_cmfd_b_elem<1>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<2>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<3>, dim3(m.nbm[2]), dim3(256), 0, s, m.cases, m.bm[2], status);
-    if (m.kmax <= 1) hipLaunchKernelGGL((k_cmfd_b_solve<1, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
-    else if (m.kmax <= 2) hipLaunchKernelGGL((k_cmfd_b_solve<2, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
-    else if (m.kmax <= 4) hipLaunchKernelGGL((k_cmfd_b_solve<4, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
-    else hipLaunchKernelGGL((k_cmfd_b_solve<6, 512>), dim3(n_), dim3(512), m.lds, s, m.cases, status);
+    const dim3 grid(n_), block(kCmfdThreads);
+    switch (m.kmax) {
+    case 1: hipLaunchKernelGGL((k_cmfd_b_solve<1, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 2: hipLaunchKernelGGL((k_cmfd_b_solve<2, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 3: hipLaunchKernelGGL((k_cmfd_b_solve<3, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 4: hipLaunchKernelGGL((k_cmfd_b_solve<4, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 5: hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    default: hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    }
     hipLaunchKernelGGL(k_cmfd_b_elem<4>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<5>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);
 }
