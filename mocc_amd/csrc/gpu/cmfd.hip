@@ -34,9 +34,16 @@ constexpr int kCmfdLdsVectors = 6;  // cell vectors of the solve in LDS (see cmf
 // Cells per lane of a mesh of nc cells (1..6), and the LDS the solve uses:
 // each vector holds K * kCmfdThreads entries, the mesh cells and the dummies.
 inline int cmfd_cells_per_lane(int nc) { return std::max(1, (nc + kCmfdThreads - 1) / kCmfdThreads); }
+// Lanes of the solve: meshes of one cell per lane use the fewest whole waves
+// that cover them (a barrier or a reduction then waits for no idle wave);
+// larger meshes use kCmfdThreads. The lane of cell c is the same either way.
+inline int cmfd_threads(int nc)
+{
+    return nc <= kCmfdThreads ? std::max(64, (nc + 63) / 64 * 64) : kCmfdThreads;
+}
 inline size_t cmfd_lds_bytes(int nc)
 {
-    return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * kCmfdThreads * sizeof(double);
+    return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * cmfd_threads(nc) * sizeof(double);
 }
 enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5 };
 
@@ -927,7 +934,18 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     const SolveArgs a = CmfdAccess::solve(*this, v);
     const size_t lds = cmfd_lds_bytes(nc_);  // operands, fs_old, BiCGSTAB vectors
     switch (cmfd_cells_per_lane(nc_)) {
-    case 1: hipLaunchKernelGGL((k_cmfd_solve<1, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 1:
+        switch (cmfd_threads(nc_)) {
+        case 64: hipLaunchKernelGGL((k_cmfd_solve<1, 64>), dim3(1), dim3(64), lds, s, a); break;
+        case 128: hipLaunchKernelGGL((k_cmfd_solve<1, 128>), dim3(1), dim3(128), lds, s, a); break;
+        case 192: hipLaunchKernelGGL((k_cmfd_solve<1, 192>), dim3(1), dim3(192), lds, s, a); break;
+        case 256: hipLaunchKernelGGL((k_cmfd_solve<1, 256>), dim3(1), dim3(256), lds, s, a); break;
+        case 320: hipLaunchKernelGGL((k_cmfd_solve<1, 320>), dim3(1), dim3(320), lds, s, a); break;
+        case 384: hipLaunchKernelGGL((k_cmfd_solve<1, 384>), dim3(1), dim3(384), lds, s, a); break;
+        case 448: hipLaunchKernelGGL((k_cmfd_solve<1, 448>), dim3(1), dim3(448), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_cmfd_solve<1, 512>), dim3(1), dim3(512), lds, s, a); break;
+        }
+        break;
     case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
     case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
     case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
@@ -1057,7 +1075,8 @@ struct CmfdBatch::Impl {
     CmfdCase *cases = nullptr;
     CaseBlock *bm[3] = {nullptr, nullptr, nullptr};  // pins, surfaces, cells
     int nbm[3] = {0, 0, 0};
-    int kmax = 1;
+    int kmax = 1;          // cells per lane of the largest mesh
+    int nt = kCmfdThreads; // lanes of the solve workgroups
     size_t lds = 0;
     void release()
     {
@@ -1117,6 +1136,7 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
                                   hipMemcpyHostToDevice, s));
     }
     impl_->kmax = cmfd_cells_per_lane(ncmax);
+    impl_->nt = cmfd_threads(ncmax);
     impl_->lds = cmfd_lds_bytes(ncmax);
     CMFD_CHECK(hipStreamSynchronize(s));
 }
@@ -1132,7 +1152,18 @@ void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
     hipLaunchKernelGGL(k_cmfd_b_elem<3>, dim3(m.nbm[2]), dim3(256), 0, s, m.cases, m.bm[2], status);
     const dim3 grid(n_), block(kCmfdThreads);
     switch (m.kmax) {
-    case 1: hipLaunchKernelGGL((k_cmfd_b_solve<1, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 1:
+        switch (m.nt) {
+        case 64: hipLaunchKernelGGL((k_cmfd_b_solve<1, 64>), grid, dim3(64), m.lds, s, m.cases, status); break;
+        case 128: hipLaunchKernelGGL((k_cmfd_b_solve<1, 128>), grid, dim3(128), m.lds, s, m.cases, status); break;
+        case 192: hipLaunchKernelGGL((k_cmfd_b_solve<1, 192>), grid, dim3(192), m.lds, s, m.cases, status); break;
+        case 256: hipLaunchKernelGGL((k_cmfd_b_solve<1, 256>), grid, dim3(256), m.lds, s, m.cases, status); break;
+        case 320: hipLaunchKernelGGL((k_cmfd_b_solve<1, 320>), grid, dim3(320), m.lds, s, m.cases, status); break;
+        case 384: hipLaunchKernelGGL((k_cmfd_b_solve<1, 384>), grid, dim3(384), m.lds, s, m.cases, status); break;
+        case 448: hipLaunchKernelGGL((k_cmfd_b_solve<1, 448>), grid, dim3(448), m.lds, s, m.cases, status); break;
+        default: hipLaunchKernelGGL((k_cmfd_b_solve<1, 512>), grid, dim3(512), m.lds, s, m.cases, status); break;
+        }
+        break;
     case 2: hipLaunchKernelGGL((k_cmfd_b_solve<2, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
     case 3: hipLaunchKernelGGL((k_cmfd_b_solve<3, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
     case 4: hipLaunchKernelGGL((k_cmfd_b_solve<4, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
