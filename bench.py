@@ -170,7 +170,7 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     cases = [c5_request(rng, i) for i in range(n_req)]
     for c in cases:  # trace every geometry before timing
         factory.host_problem(c)
-    srv = Server(factory, chunk=4)
+    srv = Server(factory, chunk=1)
     burst = summarize(srv.serve(np.zeros(n_req), cases))
     out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
                        "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]; "
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rates", default="0.25,0.5,0.75,1.0,1.25",
                     help="comma-separated arrival rates as fractions of the burst throughput")
-    ap.add_argument("--c3", action="store_true", help="add the full C3 core solve with CMFD")
+    ap.add_argument("--c3", action="store_true", help="add the full eigen solve of the 51x51-pin core (3x3 assemblies, 7 groups) with CMFD")
     ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -323,7 +323,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
     }
-    progress(f"C2 sweep: {value:.4g} updates/s")
+    progress(f"17x17 7-group sweep: {value:.4g} updates/s")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(hp, args.cpu_sweeps)
         progress("CPU baseline done")
@@ -340,7 +340,7 @@ def main():
         progress("serving done")
     if args.c3:
         line["c3"] = bench_c3(deck_dir, args.c3_oracle)
-        progress("C3 done")
+        progress("51x51-pin core solve done")
     print(json.dumps(line), flush=True)
 
 

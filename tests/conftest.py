@@ -34,10 +34,10 @@ def c1(c1_path):
 
 @pytest.fixture(scope="session")
 def cmfd_decks(deck_dir):
-    """Decks for the CMFD and eigenvalue parity tests: C1 itself, one 17x17
-    assembly (289 cells) and the 3x3-assembly core geometry of C3 (51x51
-    cells, vacuum east/south) with coarser rays so the FP64 oracle finishes
-    in seconds."""
+    """Decks for the CMFD and eigenvalue parity tests: the 3x3-pin deck
+    itself, one 17x17 assembly (289 cells) and the 3x3-assembly core geometry
+    (51x51 cells, vacuum east/south) with coarser rays so the FP64 oracle
+    finishes in seconds."""
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {
