@@ -188,9 +188,12 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const double sx = ldexp(1.0, kx);
     int kc = 0, ep = 0, ee = 0;
     double sc = 1.0;
-    // the lane's next coarse-surface event, held in registers so that steps
-    // without one do not wait on a load; nstep = 0x10000 is past every step
-    uint32_t nev = 0u;
+    // The lane's next two coarse-surface events, held in registers: neither a
+    // step without an event nor the test for a second event at the same step
+    // waits on a load. The event after them is loaded when one is consumed
+    // and is needed only at the next consumption. nstep = 0x10000 is past
+    // every step.
+    uint32_t nev = 0u, nev2 = 0u;
     int nstep = 0x10000;
     if (cur) {
         kc = fixed_exp(psi_bound * (double)cwsum * 1.0001, (double)B.max_surf_contrib);
@@ -202,8 +205,15 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
                 nev = A.ev[ep];
                 nstep = (int)(nev & 0xFFFFu);
             }
+            if (ep + 1 < ee) nev2 = A.ev[ep + 1];
         }
     }
+    auto next_event = [&]() {
+        ep++;
+        nev = nev2;
+        nstep = ep < ee ? (int)(nev & 0xFFFFu) : 0x10000;
+        if (ep + 1 < ee) nev2 = A.ev[ep + 1];
+    };
     const float csign = dir ? -1.0f : 1.0f;
     const int n = r.nseg;
     const int nr = B.n_rays;
@@ -282,12 +292,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
                 const int t = c * kChunk + u;
                 while (nstep == t) {
                     tally_current(nev);
-                    ep++;
-                    nstep = 0x10000;
-                    if (ep < ee) {
-                        nev = A.ev[ep];
-                        nstep = (int)(nev & 0xFFFFu);
-                    }
+                    next_event();
                 }
             }
             float cacc = 0.0f;
@@ -304,12 +309,7 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         // events at the exit step (after the last segment)
         while (nstep != 0x10000) {
             tally_current(nev);
-            ep++;
-            nstep = 0x10000;
-            if (ep < ee) {
-                nev = A.ev[ep];
-                nstep = (int)(nev & 0xFFFFu);
-            }
+            next_event();
         }
     }
     if (active) {
