@@ -170,12 +170,28 @@ struct mocgpu_handle {
         a.phi = d_phi.p;
         a.psi = d_psi.p;
         a.psum = d_psum.p;
+        psum_strides(g_fixed, a.psum_sp, a.psum_sg);
         a.slot_pos = d_slot_pos.p;
         a.ev_off = d_ev_off.p;
         a.ev = d_ev.p;
         a.csum = d_csum.p;
         a.ssum = d_ssum.p;
         return a;
+    }
+
+    // Layout of the band partials for a sweep of group g (g < 0: all groups)
+    // and its reduction. All groups: position-major, so the G workgroups of a
+    // band (one XCD) fill whole lines of a position's groups. One group:
+    // group-major, so its reduction reads only that group's partials.
+    void psum_strides(int g, int64_t &sp, int64_t &sg) const
+    {
+        if (g < 0) {
+            sp = G;
+            sg = 1;
+        } else {
+            sp = 1;
+            sg = (int64_t)plan->loc_fsr.size();
+        }
     }
 
     ReduceArgs reduce_args(int g_fixed) const
@@ -187,6 +203,7 @@ struct mocgpu_handle {
         r.n_slots = (int)plan->loc_fsr.size();
         r.slot_ptr = d_slot_ptr.p;
         r.psum = d_psum.p;
+        psum_strides(g_fixed, r.psum_sp, r.psum_sg);
         r.fsr_mat = d_fsr_mat.p;
         r.mat_tr = d_mat_tr.p;
         r.mat_ss = d_mat_ss.p;

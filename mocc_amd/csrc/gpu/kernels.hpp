@@ -39,7 +39,11 @@ struct SweepArgs {
     const double *mat_tr, *mat_ss;
     const double *src, *phi;
     float *psi;             // [G][2][bcpg]
-    double *psum;           // [n_slots][G]: slot s of group g at psum[slot_pos[s] * G + g]
+    // Band partials, slot s of group g at psum[slot_pos[s] * psum_sp + g * psum_sg]:
+    // position-major ([n_slots][G]: sp = G, sg = 1) when all groups are swept
+    // together, group-major ([G][n_slots]: sp = 1, sg = n_slots) when one is
+    double *psum;
+    int64_t psum_sp, psum_sg;
     const int32_t *slot_pos;
     const int32_t *ev_off;
     const uint32_t *ev;
@@ -48,8 +52,9 @@ struct SweepArgs {
 
 struct ReduceArgs {
     int32_t F, G, g_fixed, n_slots;
-    const int32_t *slot_ptr;  // partials of FSR f: psum[s * G + g], s in [slot_ptr[f], slot_ptr[f + 1])
-    const double *psum;
+    const int32_t *slot_ptr;  // partials of FSR f: positions s in [slot_ptr[f], slot_ptr[f + 1])
+    const double *psum;       // position s of group g at psum[s * psum_sp + g * psum_sg] (as written)
+    int64_t psum_sp, psum_sg;
     const int32_t *fsr_mat;
     const double *mat_tr, *mat_ss, *vol;
     const double *src;

@@ -323,10 +323,12 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     }
     __syncthreads();
     const double rscale = ldexp(1.0, -kx);
-    // each band slot goes to its FSR-major position; the G workgroups of a
-    // band run on one XCD, so its L2 joins their writes of a slot's groups
+    // each band slot goes to its FSR-major position. All-group sweeps store
+    // position-major: the G workgroups of a band run on one XCD, so its L2
+    // joins their writes of a slot's groups.
     const int32_t *__restrict__ pos = A.slot_pos + B.loc_base;
-    for (int l = tid; l < B.n_local; l += 2 * W) A.psum[(size_t)pos[l] * A.G + g] = (double)(long long)tal[l] * rscale;
+    double *__restrict__ pg = A.psum + g * A.psum_sg;
+    for (int l = tid; l < B.n_local; l += 2 * W) pg[pos[l] * A.psum_sp] = (double)(long long)tal[l] * rscale;
     if (cur) {
         const double rc = ldexp(1.0, -kc);
         double *__restrict__ cs = A.csum + (size_t)g * A.n_lslots + B.lsurf_base;
@@ -412,9 +414,9 @@ __device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, 
     const int G = A.G;
     double T = 0.0;
     if (valid) {
-        const double *__restrict__ ps = A.psum + g;
+        const double *__restrict__ ps = A.psum + g * A.psum_sg;
         const int s1 = A.slot_ptr[f + 1];
-        for (int s = A.slot_ptr[f] + lane; s < s1; s += kRedLanes) T += ps[(size_t)s * G];
+        for (int s = A.slot_ptr[f] + lane; s < s1; s += kRedLanes) T += ps[s * A.psum_sp];
     }
 #pragma unroll
     for (int o = 1; o < kRedLanes; o <<= 1) T += __shfl_xor(T, o, kRedLanes);
