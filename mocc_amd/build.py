@@ -17,6 +17,11 @@ PKG = os.path.join(ROOT, "mocc_amd")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
 GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip"]
+# The coarse solve is instantiated for 1-6 cells per lane, and a batch runs
+# every problem at the cells per lane of its largest mesh. Without
+# contraction each multiply and add rounds on its own, so a problem's
+# arithmetic does not depend on how the compiler fused a given instantiation.
+GPU_EXTRA_FLAGS = {"cmfd.hip": ["-ffp-contract=off"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # No packed FP32 instructions (v_pk_*_f32) anywhere in device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -49,7 +54,7 @@ def build_gpu():
         src = os.path.join(PKG, "csrc", "gpu", s)
         obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
         if s.endswith(".hip"):
-            _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+            _run([HIPCC] + HIP_FLAGS + GPU_EXTRA_FLAGS.get(s, []) + ["-c", src, "-o", obj])
         else:
             _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-c", src, "-o", obj])
         objs.append(obj)

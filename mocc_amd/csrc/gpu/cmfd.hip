@@ -524,12 +524,17 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
 #pragma unroll
         for (int j = 0; j < 5; j++) y5[j] = buf[L.on[k] ? row_col(A, c, ix, j) : c];
 #pragma unroll
-        for (int j = 0; j < 5; j++) s += a[j] * y5[j];
+        for (int j = 0; j < 5; j++) s = fma(a[j], y5[j], s);
         v[k] = s;
     }
     return buf;
 }
 
+// This file is compiled without floating-point contraction, so that a
+// problem's arithmetic is the same at any cells per lane (a batch runs every
+// problem at those of its largest mesh): the hot loops below write their
+// fused multiply-adds explicitly.
+//
 // Jacobi-preconditioned BiCGSTAB with the relative test |r| <= tol |b| and a
 // restart when rho vanishes (van der Vorst 1992, the iteration of Eigen's
 // BiCGSTAB<SparseMatrix<double>> that cmfd.cpp:291 calls). The preconditioned
@@ -558,8 +563,8 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         if (branch_free<K>() || L.on[k]) sr0[L.c[k]] = r[k];
         p[k] = 0.0;
         v[k] = 0.0;
-        lr0 += r[k] * r[k];
-        lb += bk * bk;
+        lr0 = fma(r[k], r[k], lr0);
+        lb = fma(bk, bk, lb);
     }
     double r0_sq, rhs_sq;
     block_sum2(lr0, lb, S, r0_sq, rhs_sq);
@@ -577,8 +582,8 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         double lrr = 0.0, lrho = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            lrr += r[k] * r[k];
-            if (branch_free<K>() || L.on[k]) lrho += sr0[L.c[k]] * r[k];
+            lrr = fma(r[k], r[k], lrr);
+            if (branch_free<K>() || L.on[k]) lrho = fma(sr0[L.c[k]], r[k], lrho);
         }
         double rr, rho_new;
         block_sum2(lrr, lrho, S, rr, rho_new);
@@ -592,7 +597,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
             for (int k = 0; k < K; k++) {
                 r[k] = ((branch_free<K>() || L.on[k]) ? sb[L.c[k]] : 0.0) - t[k];
                 if (branch_free<K>() || L.on[k]) sr0[L.c[k]] = r[k];
-                l += r[k] * r[k];
+                l = fma(r[k], r[k], l);
             }
             rho = r0_sq = block_sum(l, S);
             if (restarts++ == 0) i = 0;
@@ -600,35 +605,35 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         const double beta = (rho / rho_old) * (alpha / w);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            p[k] = r[k] + beta * (p[k] - w * v[k]);
+            p[k] = fma(beta, fma(-w, v[k], p[k]), r[k]);
             t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * p[k];  // y
         }
         const double *yb = spmv<K, NT>(A, R, L, t, v, S);
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
-            if (branch_free<K>() || L.on[k]) l += sr0[L.c[k]] * v[k];
+            if (branch_free<K>() || L.on[k]) l = fma(sr0[L.c[k]], v[k], l);
         alpha = rho / block_sum(l, S);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (branch_free<K>() || L.on[k]) x[k] += alpha * yb[L.c[k]];
-            r[k] = r[k] - alpha * v[k];                     // s
+            if (branch_free<K>() || L.on[k]) x[k] = fma(alpha, yb[L.c[k]], x[k]);
+            r[k] = fma(-alpha, v[k], r[k]);                 // s
             t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * r[k];  // z
         }
         const double *zb = spmv<K, NT>(A, R, L, t, t, S);  // t = A z
         double ltt = 0.0, lts = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            ltt += t[k] * t[k];
-            lts += t[k] * r[k];
+            ltt = fma(t[k], t[k], ltt);
+            lts = fma(t[k], r[k], lts);
         }
         double tt, ts;
         block_sum2(ltt, lts, S, tt, ts);
         w = (tt > 0.0) ? ts / tt : 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (branch_free<K>() || L.on[k]) x[k] += w * zb[L.c[k]];
-            r[k] = r[k] - w * t[k];
+            if (branch_free<K>() || L.on[k]) x[k] = fma(w, zb[L.c[k]], x[k]);
+            r[k] = fma(-w, t[k], r[k]);
         }
         ++i;
     }
@@ -683,7 +688,10 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Row
     spmv<K, NT>(A, R, L, x, t, S);
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < K; k++) s += (t[k] - b[k]) * (t[k] - b[k]);
+    for (int k = 0; k < K; k++) {
+        const double d = t[k] - b[k];
+        s = fma(d, d, s);
+    }
     return s;
 }
 

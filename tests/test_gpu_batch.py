@@ -189,3 +189,20 @@ def test_batch_cmfd_graph_matches_eager(batch_decks):
         be.poll()
         for n in names:
             assert np.array_equal(sg[n].flux(), se[n].flux()), (rep, n)
+
+
+def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks):
+    """The coarse solve of a batch runs at the width of its largest mesh (6
+    cells per lane for the 51x51 core, against one wave for 3x3 pins alone);
+    a small problem's k, iterations and flux are bitwise the same either way."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
+    print("cmfd 3x3 pins alone", flush=True)
+    alone = {"c1": mk("c1")}
+    r_alone = _solve_batch(alone, ["c1"])["c1"]
+    print("cmfd 3x3 pins beside the 51x51 core", flush=True)
+    mixed = {"core51": mk("core51"), "c1": mk("c1")}
+    r = _solve_batch(mixed, ["core51", "c1"])
+    assert r["c1"].converged and r["core51"].converged, (r["c1"].iterations, r["core51"].iterations)
+    assert r["c1"].k == r_alone.k and r["c1"].iterations == r_alone.iterations
+    assert np.array_equal(mixed["c1"].flux(), alone["c1"].flux())
