@@ -17,11 +17,14 @@ PKG = os.path.join(ROOT, "mocc_amd")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
 GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip"]
-# The coarse solve is instantiated for 1-6 cells per lane, and a batch runs
-# every problem at the cells per lane of its largest mesh. Without
-# contraction each multiply and add rounds on its own, so a problem's
-# arithmetic does not depend on how the compiler fused a given instantiation.
-GPU_EXTRA_FLAGS = {"cmfd.hip": ["-ffp-contract=off"]}
+# No floating-point contraction in device code: a kernel is instantiated
+# several ways (cells per lane of the coarse solve, with or without current
+# tallies, single-problem or batched), and a batch picks the instantiation for
+# all of its problems. Without contraction each multiply and add rounds on
+# its own, so a problem's arithmetic does not depend on how the compiler fused
+# a given instantiation; the hot loops write their fused multiply-adds
+# explicitly.
+GPU_EXTRA_FLAGS = {s: ["-ffp-contract=off"] for s in ("sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip")}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # No packed FP32 instructions (v_pk_*_f32) anywhere in device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -530,7 +530,7 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
     return buf;
 }
 
-// This file is compiled without floating-point contraction, so that a
+// Device code is compiled without floating-point contraction, so that a
 // problem's arithmetic is the same at any cells per lane (a batch runs every
 // problem at those of its largest mesh): the hot loops below write their
 // fused multiply-adds explicitly.

@@ -206,3 +206,21 @@ def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks):
     assert r["c1"].converged and r["core51"].converged, (r["c1"].iterations, r["core51"].iterations)
     assert r["c1"].k == r_alone.k and r["c1"].iterations == r_alone.iterations
     assert np.array_equal(mixed["c1"].flux(), alone["c1"].flux())
+
+
+def test_batch_plain_problem_beside_cmfd_bitwise(batch_decks):
+    """A problem without CMFD sweeps its last inner iteration in the
+    current-tallying queue kernel when a CMFD problem shares the batch, and
+    in the plain one alone: bitwise the same either way."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    target = "asm17"
+    alone = _sweepers(batch_decks, [target])
+    alone[target] = GpuMoCSweeper(batch_decks[target], energy_mode=1, n_inner=5)
+    print("17x17 without CMFD alone", flush=True)
+    r_alone = _solve_batch(alone, [target])[target]
+    mixed = {target: GpuMoCSweeper(batch_decks[target], energy_mode=1, n_inner=5),
+             "c1": GpuMoCSweeper(batch_decks["c1"], energy_mode=1, n_inner=5, use_cmfd=True)}
+    print("17x17 without CMFD beside a CMFD problem", flush=True)
+    r = _solve_batch(mixed, ["c1", target])[target]
+    assert r.k == r_alone.k and r.iterations == r_alone.iterations
+    assert np.array_equal(mixed[target].flux(), alone[target].flux())
