@@ -218,14 +218,14 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     const int n = r.nseg;
     const int nr = B.n_rays;
 
-    auto tally_current = [&](uint32_t evv) {
+    auto tally_current = [&](uint32_t evv, const float(&pv)[NP]) {
         const int ls = (int)((evv >> 16) & 0x7FFFu);
         const int nn = (int)(evv >> 31);
         float cv = 0.0f, sv = 0.0f;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-            cv = fmaf(psi[p], nn ? cw[p][1] : cw[p][0], cv);
-            sv = fmaf(psi[p], nn ? fw[p][1] : fw[p][0], sv);
+            cv = fmaf(pv[p], nn ? cw[p][1] : cw[p][0], cv);
+            sv = fmaf(pv[p], nn ? fw[p][1] : fw[p][0], sv);
         }
         atomicAdd(&tcur[ls], to_fixed(csign * cv, sc));
         atomicAdd(&tsfl[ls], to_fixed(sv, sc));
@@ -285,15 +285,16 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         ca = ap[0];
         cb = ap[1];
         __builtin_amdgcn_sched_barrier(0);
-        // Phase 2: the psi recurrence and the tallies.
+        // Phase 2: the psi recurrence and the tallies. With currents, psi at
+        // the start of every step is kept (ph) and the chunk's surface
+        // crossings are tallied after it: a wave then runs the tally block
+        // once per chunk, not at every step where one of its lanes crosses.
+        float ph[kChunk][NP];
 #pragma unroll
         for (int u = 0; u < kChunk; u++) {
             if (CUR) {
-                const int t = c * kChunk + u;
-                while (nstep == t) {
-                    tally_current(nev);
-                    next_event();
-                }
+#pragma unroll
+                for (int p = 0; p < NP; p++) ph[u][p] = psi[p];
             }
             float cacc = 0.0f;
 #pragma unroll
@@ -304,11 +305,35 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
             }
             atomicAdd((unsigned long long *)((char *)tal + loc[u]), to_fixed(cacc, sx));
         }
+        if (CUR) {
+            // crossings at steps t0 .. t0 + 7 use psi at the start of their
+            // step (a padding step leaves psi unchanged, so an exit inside
+            // the last chunk sees the outgoing flux)
+            static_assert(kChunk == 8, "the step select below covers 8 steps");
+            const int t0 = c * kChunk;
+            while (nstep < t0 + kChunk) {
+                const int uu = nstep - t0;
+                const bool b0 = uu & 1, b1 = uu & 2, b2 = uu & 4;
+                float pe[NP];
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const float a01 = b0 ? ph[1][p] : ph[0][p];
+                    const float a23 = b0 ? ph[3][p] : ph[2][p];
+                    const float a45 = b0 ? ph[5][p] : ph[4][p];
+                    const float a67 = b0 ? ph[7][p] : ph[6][p];
+                    const float a03 = b1 ? a23 : a01;
+                    const float a47 = b1 ? a67 : a45;
+                    pe[p] = b2 ? a47 : a03;
+                }
+                tally_current(nev, pe);
+                next_event();
+            }
+        }
     }
     if (CUR && active) {
         // events at the exit step (after the last segment)
         while (nstep != 0x10000) {
-            tally_current(nev);
+            tally_current(nev, psi);
             next_event();
         }
     }
