@@ -29,7 +29,9 @@ namespace {
 
 constexpr int kCmfdThreads = 512;
 constexpr int kCmfdWaves = kCmfdThreads / 64;
-constexpr int kCmfdLdsVectors = 6;  // cell vectors of the solve in LDS (see cmfd_solve)
+constexpr int kCmfdLdsVectors = 6;    // cell vectors of the solve in LDS (see cmfd_solve)
+constexpr int kCmfdLdsVectorsRL = 7;  // with the rows in LDS: nc-long vectors
+constexpr size_t kCmfdLdsMax = 160 * 1024;  // LDS of a gfx950 workgroup
 
 // Cells per lane of a mesh of nc cells (1..6), and the LDS the solve uses:
 // each vector holds K * kCmfdThreads entries, the mesh cells and the dummies.
@@ -41,8 +43,18 @@ inline int cmfd_threads(int nc)
 {
     return nc <= kCmfdThreads ? std::max(64, (nc + 63) / 64 * 64) : kCmfdThreads;
 }
+// Meshes of 5 or 6 cells per lane keep the group's rows in LDS when they fit
+// (up to about 2900 cells): without that, each product reloads them from
+// cache, since the registers hold no more than 4 cells' rows. The two
+// reduction banks take 2 doubles per wave each.
+inline bool cmfd_rows_in_lds(int nc)
+{
+    return cmfd_cells_per_lane(nc) >= 5 &&
+           (size_t)kCmfdLdsVectorsRL * nc * sizeof(double) + 4 * kCmfdWaves * sizeof(double) <= kCmfdLdsMax;
+}
 inline size_t cmfd_lds_bytes(int nc)
 {
+    if (cmfd_rows_in_lds(nc)) return (size_t)kCmfdLdsVectorsRL * nc * sizeof(double);
     return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * cmfd_threads(nc) * sizeof(double);
 }
 enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5 };
@@ -341,6 +353,7 @@ struct SolveArgs {
     const double *coef;  // [5][G][nc]
     const double *hnf, *hch, *hsc, *cvol;
     double *cflux;  // [nc][G]
+    double *gscr;   // [3][nc] global scratch of the solves that keep the rows in LDS
     double *k;
     int *info;
 };
@@ -450,6 +463,7 @@ constexpr int kRegCoefMaxK = 4;
 template <int K> struct Rows {
     double a[K <= kRegCoefMaxK ? 5 : 1][K <= kRegCoefMaxK ? K : 1];  // south, west, diagonal, east, north
     const double *cf;  // coefficients of group g (reloaded when not held)
+    double *rl;        // [5][nc] the rows in LDS (solves with RL), else null
 };
 
 // Which of the five row entries (south, west, diagonal, east, north) of cell
@@ -471,10 +485,28 @@ __device__ __forceinline__ int row_col(const SolveArgs &A, int c, int ix, int j)
     return row_has(A, c, ix, j) ? c + off[j] : c;
 }
 
-template <int K> __device__ __forceinline__ void load_rows(const SolveArgs &A, int g, const Lane<K> &L, Rows<K> &R)
+// p plus an opaque zero: the compiler cannot hoist loads through it out of a
+// loop (where the values would sit in registers that the loop's vectors need)
+template <class T> __device__ __forceinline__ T *opaque(T *p)
+{
+    int zero = 0;
+    __asm__ volatile("" : "+v"(zero));
+    return p + zero;
+}
+
+template <int K, bool RL> __device__ __forceinline__ void load_rows(const SolveArgs &A, int g, const Lane<K> &L, Rows<K> &R)
 {
     R.cf = A.coef + (size_t)g * A.nc;
-    if constexpr (K <= kRegCoefMaxK) {
+    if constexpr (RL) {
+        // each lane stages its own cells' rows; only it reads them
+        const size_t pl = (size_t)A.G * A.nc;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (L.on[k])
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    R.rl[j * A.nc + L.c[k]] = row_has(A, L.c[k], L.ix[k], j) ? R.cf[j * pl + L.c[k]] : 0.0;
+    } else if constexpr (K <= kRegCoefMaxK) {
         const size_t pl = (size_t)A.G * A.nc;
 #pragma unroll
         for (int j = 0; j < 5; j++)
@@ -489,20 +521,29 @@ template <int K> __device__ __forceinline__ void load_rows(const SolveArgs &A, i
 // missing neighbour enters as 0 times the cell's own entry, which leaves the
 // sum unchanged, so the five reads of a row carry no branches and issue
 // together. y and v may be the same array. Returns the bank, where y stays
-// readable until the second product after this one.
-template <int K, int NT>
+// readable until the second product after this one (with RL, whose single
+// bank each product overwrites after a barrier, until the next one).
+template <int K, int NT, bool RL>
 __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L,
                                               const double (&y)[K], double (&v)[K], Scratch<NT> &S)
 {
-    double *buf = S.buf + S.bb * (K * NT);
-    S.bb ^= 1;
+    double *buf;
+    if constexpr (RL) {
+        // one operand bank: every lane has read the previous operand first
+        buf = S.buf;
+        __syncthreads();
+    } else {
+        buf = S.buf + S.bb * (K * NT);
+        S.bb ^= 1;
+    }
 #pragma unroll
     for (int k = 0; k < K; k++)
         if (branch_free<K>() || L.on[k]) buf[L.c[k]] = y[k];
     __syncthreads();
     const size_t pl = (size_t)A.G * A.nc;
-    const double *cf = R.cf;
-    if constexpr (K > kRegCoefMaxK) __asm__ volatile("" : "+s"(cf));
+    // every product reloads the rows of K > kRegCoefMaxK cells
+    const double *cf = (K > kRegCoefMaxK && !RL) ? opaque(R.cf) : R.cf;
+    const double *rl = (K > kRegCoefMaxK && RL) ? opaque(R.rl) : R.rl;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double s = 0.0;
@@ -515,7 +556,9 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
         double a[5];
 #pragma unroll
         for (int j = 0; j < 5; j++) {
-            if constexpr (K <= kRegCoefMaxK)
+            if constexpr (RL)
+                a[j] = rl[j * A.nc + c];
+            else if constexpr (K <= kRegCoefMaxK)
                 a[j] = R.a[j][k];
             else
                 a[j] = (L.on[k] && row_has(A, c, ix, j)) ? cf[j * pl + c] : 0.0;
@@ -542,19 +585,23 @@ __device__ __forceinline__ const double *spmv(const SolveArgs &A, const Rows<K> 
 // operand, so x is updated as (x + alpha y) + w z. The loop-invariant vectors
 // (the inverse diagonal, the shadow residual r0 and b) live in LDS slots that
 // only the lane owning the cell touches, which keeps the rest in registers.
-template <int K, int NT>
+template <int K, int NT, bool RL = false>
 __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, double (&x)[K], double tol,
                         int max_iter, Scratch<NT> &S, double *sinv, double *sr0, const double *sb)
 {
     const size_t pl = (size_t)A.G * A.nc;
     double r[K], p[K], v[K], t[K];
+    // the Jacobi preconditioner: the inverse diagonal (in LDS, or with the
+    // rows in LDS in the global scratch, reloaded every iteration)
+    const double *si = sinv;
+    auto inv_diag = [&](int k) -> double { return si[L.c[k]]; };
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (!branch_free<K>() && !L.on[k]) continue;
         const double d = L.on[k] ? R.cf[2 * pl + L.c[k]] : 0.0;
         sinv[L.c[k]] = (d != 0.0) ? 1.0 / d : 1.0;
     }
-    spmv<K, NT>(A, R, L, x, t, S);
+    spmv<K, NT, RL>(A, R, L, x, t, S);
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -579,6 +626,10 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
     const double eps2 = eps * eps;
     int i = 0, restarts = 0;
     while (true) {
+        if constexpr (RL) {
+            si = opaque(sinv);
+            sb = opaque(sb);
+        }
         double lrr = 0.0, lrho = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -591,7 +642,7 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         const double rho_old = rho;
         rho = rho_new;
         if (fabs(rho) < eps2 * r0_sq) {
-            spmv<K, NT>(A, R, L, x, t, S);
+            spmv<K, NT, RL>(A, R, L, x, t, S);
             double l = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -606,9 +657,9 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
 #pragma unroll
         for (int k = 0; k < K; k++) {
             p[k] = fma(beta, fma(-w, v[k], p[k]), r[k]);
-            t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * p[k];  // y
+            t[k] = ((branch_free<K>() || L.on[k]) ? inv_diag(k) : 1.0) * p[k];  // y
         }
-        const double *yb = spmv<K, NT>(A, R, L, t, v, S);
+        const double *yb = spmv<K, NT, RL>(A, R, L, t, v, S);
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -618,9 +669,9 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
         for (int k = 0; k < K; k++) {
             if (branch_free<K>() || L.on[k]) x[k] = fma(alpha, yb[L.c[k]], x[k]);
             r[k] = fma(-alpha, v[k], r[k]);                 // s
-            t[k] = ((branch_free<K>() || L.on[k]) ? sinv[L.c[k]] : 1.0) * r[k];  // z
+            t[k] = ((branch_free<K>() || L.on[k]) ? inv_diag(k) : 1.0) * r[k];  // z
         }
-        const double *zb = spmv<K, NT>(A, R, L, t, t, S);  // t = A z
+        const double *zb = spmv<K, NT, RL>(A, R, L, t, t, S);  // t = A z
         double ltt = 0.0, lts = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -679,13 +730,13 @@ __device__ __forceinline__ double fission_source(const SolveArgs &A, const Lane<
 }
 
 // ||A x - b||^2 of group g with x the stored coarse flux (cmfd.cpp:640-666)
-template <int K, int NT>
+template <int K, int NT, bool RL = false>
 __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Rows<K> &R, const Lane<K> &L,
                                             const double (&b)[K], double (&x)[K], double (&t)[K], Scratch<NT> &S)
 {
 #pragma unroll
     for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * A.G + g] : 0.0;
-    spmv<K, NT>(A, R, L, x, t, S);
+    spmv<K, NT, RL>(A, R, L, x, t, S);
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -696,15 +747,33 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Row
 }
 
 // CMFD::solve (cmfd.cpp:171-279) by one workgroup of NT lanes, K cells per
-// lane; lds holds kCmfdLdsVectors * K * NT doubles, red 2 * red_bank<NT>().
-template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
+// lane; red holds 2 * red_bank<NT>() doubles. lds holds kCmfdLdsVectors *
+// K * NT doubles, or with RL (rows in LDS) kCmfdLdsVectorsRL * nc: one
+// product operand, the shadow residual and the group's five rows, while the
+// previous fission source, the right-hand side and the inverse diagonal go to
+// A.gscr.
+template <int K, int NT, bool RL = false>
+__device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
-    constexpr int KN = K * NT;            // entries of each LDS vector (cells and dummies)
-    Scratch<NT> S{red, lds, KN, 0, 0};    // [2][KN] product operands
-    double *fs_old = lds + 2 * KN;        // [KN] previous coarse fission source
-    double *sinv = lds + 3 * KN;          // [KN] BiCGSTAB: inverse diagonal
-    double *sr0 = lds + 4 * KN;           // [KN] BiCGSTAB: shadow residual
-    double *sb = lds + 5 * KN;            // [KN] BiCGSTAB: right-hand side
+    // the nc-long vectors hold no dummy slots: every access must skip them
+    static_assert(!RL || !branch_free<K>(), "rows in LDS need per-slot branches");
+    constexpr int KN = K * NT;  // entries of each LDS vector (cells and dummies)
+    Scratch<NT> S{red, lds, KN, 0, 0};
+    double *fs_old, *sinv, *sr0, *sb;
+    Rows<K> R;
+    if constexpr (RL) {
+        sr0 = lds + A.nc;         // [nc] BiCGSTAB: shadow residual
+        R.rl = lds + 2 * A.nc;    // [5][nc] the group's rows
+        fs_old = A.gscr;          // [nc] previous coarse fission source
+        sb = A.gscr + A.nc;       // [nc] BiCGSTAB: right-hand side
+        sinv = A.gscr + 2 * A.nc; // [nc] BiCGSTAB: inverse diagonal
+    } else {
+        fs_old = lds + 2 * KN;    // [KN] previous coarse fission source
+        sinv = lds + 3 * KN;      // [KN] BiCGSTAB: inverse diagonal
+        sr0 = lds + 4 * KN;       // [KN] BiCGSTAB: shadow residual
+        sb = lds + 5 * KN;        // [KN] BiCGSTAB: right-hand side
+        R.rl = nullptr;
+    }
     Lane<K> L;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -715,13 +784,12 @@ template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveA
     const int G = A.G;
     double k_eff = A.k[0];
     double fsc[K], b[K], x[K], t[K];
-    Rows<K> R;
     double tfis = block_sum(fission_source<K>(A, L, k_eff, fsc), S);
     double r0 = 0.0;
     for (int g = 0; g < G; g++) {
         cmfd_source<K>(A, g, L, fsc, b);
-        load_rows<K>(A, g, L, R);
-        r0 += block_sum(residual2<K, NT>(A, g, R, L, b, x, t, S), S);
+        load_rows<K, RL>(A, g, L, R);
+        r0 += block_sum(residual2<K, NT, RL>(A, g, R, L, b, x, t, S), S);
     }
     r0 = sqrt(r0) / (G * A.nc);
     const double tol = A.rr * r0;
@@ -736,12 +804,12 @@ template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveA
         double ri = 0.0;
         for (int g = 0; g < G; g++) {
             cmfd_source<K>(A, g, L, fsc, b);
-            load_rows<K>(A, g, L, R);
-            ri += block_sum(residual2<K, NT>(A, g, R, L, b, x, t, S), S);
+            load_rows<K, RL>(A, g, L, R);
+            ri += block_sum(residual2<K, NT, RL>(A, g, R, L, b, x, t, S), S);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (branch_free<K>() || L.on[k]) sb[L.c[k]] = b[k];  // 0 at dummies
-            n_bicg += bicgstab<K, NT>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
+            n_bicg += bicgstab<K, NT, RL>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
@@ -777,11 +845,11 @@ template <int K, int NT> __device__ __forceinline__ void cmfd_solve(const SolveA
     }
 }
 
-template <int K, int NT> __global__ __launch_bounds__(NT) void k_cmfd_solve(SolveArgs A)
+template <int K, int NT, bool RL = false> __global__ __launch_bounds__(NT) void k_cmfd_solve(SolveArgs A)
 {
     extern __shared__ double lds[];
     __shared__ double red[2 * red_bank<NT>()];
-    cmfd_solve<K, NT>(A, lds, red);
+    cmfd_solve<K, NT, RL>(A, lds, red);
 }
 
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -798,7 +866,7 @@ CmfdSolver::~CmfdSolver() { free_all(); }
 void CmfdSolver::free_all()
 {
     double **ds[] = {&d_dx_, &d_dy_, &d_cvol_, &d_cflux_, &d_htr_, &d_hnf_, &d_hf_, &d_hch_, &d_hrm_,
-                     &d_hsc_, &d_dtil_, &d_stil_, &d_dhat_, &d_shat_, &d_coef_};
+                     &d_hsc_, &d_dtil_, &d_stil_, &d_dhat_, &d_shat_, &d_coef_, &d_gscr_};
     for (double **p : ds) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -859,6 +927,7 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     dalloc(d_dhat_, (size_t)n_surf_ * G_);
     dalloc(d_shat_, (size_t)n_surf_ * G_);
     dalloc(d_coef_, 5 * ncg);
+    dalloc(d_gscr_, 3 * (size_t)nc_);
     dalloc(d_info_, 3);
     initialize(s);
 }
@@ -913,6 +982,7 @@ struct CmfdAccess {
         a.hsc = c.d_hsc_;
         a.cvol = c.d_cvol_;
         a.cflux = c.d_cflux_;
+        a.gscr = c.d_gscr_;
         a.k = v.k;
         a.info = c.d_info_;
         return a;
@@ -957,8 +1027,18 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
     case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
     case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-    case 5: hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-    default: hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+    case 5:
+        if (cmfd_rows_in_lds(nc_))
+            hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+        break;
+    default:
+        if (cmfd_rows_in_lds(nc_))
+            hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+        break;
     }
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
     hipLaunchKernelGGL(k_set_pin_flux_all, dim3(nblk(npg)), dim3(256), 0, s, v, d_cflux_, d_info_ + 1);
@@ -1059,14 +1139,14 @@ __global__ void k_cmfd_b_elem(const CmfdCase *__restrict__ cs, const CaseBlock *
     if (STAGE == 5) set_pin_flux_item(d.v, d.a.cflux, d.info + 1, t);
 }
 
-template <int K, int NT> __global__ __launch_bounds__(NT) void k_cmfd_b_solve(const CmfdCase *__restrict__ cs,
-                                                                              const int32_t *__restrict__ status)
+template <int K, int NT, bool RL = false>
+__global__ __launch_bounds__(NT) void k_cmfd_b_solve(const CmfdCase *__restrict__ cs, const int32_t *__restrict__ status)
 {
     extern __shared__ double lds[];
     __shared__ double red[2 * red_bank<NT>()];
     const CmfdCase &d = cs[blockIdx.x];
     if (case_done(d, status)) return;
-    cmfd_solve<K, NT>(d.a, lds, red);
+    cmfd_solve<K, NT, RL>(d.a, lds, red);
 }
 
 std::vector<CaseBlock> case_blocks(const std::vector<int> &counts)
@@ -1084,6 +1164,7 @@ struct CmfdBatch::Impl {
     CaseBlock *bm[3] = {nullptr, nullptr, nullptr};  // pins, surfaces, cells
     int nbm[3] = {0, 0, 0};
     int kmax = 1;          // cells per lane of the largest mesh
+    bool rl = false;       // its rows kept in LDS
     int nt = kCmfdThreads; // lanes of the solve workgroups
     size_t lds = 0;
     void release()
@@ -1145,6 +1226,7 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
     }
     impl_->kmax = cmfd_cells_per_lane(ncmax);
     impl_->nt = cmfd_threads(ncmax);
+    impl_->rl = cmfd_rows_in_lds(ncmax);
     impl_->lds = cmfd_lds_bytes(ncmax);
     CMFD_CHECK(hipStreamSynchronize(s));
 }
@@ -1175,8 +1257,18 @@ void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
     case 2: hipLaunchKernelGGL((k_cmfd_b_solve<2, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
     case 3: hipLaunchKernelGGL((k_cmfd_b_solve<3, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
     case 4: hipLaunchKernelGGL((k_cmfd_b_solve<4, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
-    case 5: hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
-    default: hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+    case 5:
+        if (m.rl)
+            hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads, true>), grid, block, m.lds, s, m.cases, status);
+        else
+            hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads>), grid, block, m.lds, s, m.cases, status);
+        break;
+    default:
+        if (m.rl)
+            hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads, true>), grid, block, m.lds, s, m.cases, status);
+        else
+            hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads>), grid, block, m.lds, s, m.cases, status);
+        break;
     }
     hipLaunchKernelGGL(k_cmfd_b_elem<4>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<5>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);

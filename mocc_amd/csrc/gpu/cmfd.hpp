@@ -64,6 +64,8 @@ private:
     double *d_hsc_ = nullptr;                                               // [nc][G][G]
     double *d_dtil_ = nullptr, *d_stil_ = nullptr, *d_dhat_ = nullptr, *d_shat_ = nullptr;  // [n_surf][G]
     double *d_coef_ = nullptr;  // [5][G][nc]: south, west, diagonal, east, north
+    double *d_gscr_ = nullptr;  // [3][nc]: previous coarse fission source, BiCGSTAB right-hand side and inverse diagonal (solves with the rows in LDS)
+                                // (solves that keep the rows in LDS)
     int *d_info_ = nullptr;     // [0] outer iterations, [1] non-positive pin flux seen
     int n_solve_ = 0;
 };
