@@ -3,7 +3,7 @@
 
 Workload: the 17x17 assembly (25 moderator guide tubes), 7 groups, Chebyshev-
 Yamamoto 32 azimuths x 3 polar per octant, ray spacing 0.03 cm, reflective
-(seeded synthetic cross sections, mocc_amd.lattice_gen.c2_deck).
+(seeded synthetic cross sections, mocc_amd.lattice_gen.assembly17_deck).
 
 A step is one inner sweep of all 7 groups (sweep kernel + flux reduction),
 i.e. 2 * P * S * G segment-group updates. Inputs are resident in HBM before
@@ -102,7 +102,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     from mocc_amd.branch import BranchFactory, c4_cases
     from mocc_amd.sweeper import GpuBatch
     n_inner = 5 if use_cmfd else 1
-    factory = BranchFactory(os.path.join(work_dir, "c4cmfd" if use_cmfd else "c4"), n_inner=n_inner,
+    factory = BranchFactory(os.path.join(work_dir, "batch_cmfd" if use_cmfd else "batch"), n_inner=n_inner,
                             use_cmfd=use_cmfd)
     cases = c4_cases(n_cases)
     t0 = time.perf_counter()
@@ -164,7 +164,7 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     import numpy as np
     from mocc_amd.branch import BranchFactory, c5_request
     from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
-    factory = BranchFactory(os.path.join(work_dir, "c5cmfd" if use_cmfd else "c5"), n_inner=5 if use_cmfd else 1,
+    factory = BranchFactory(os.path.join(work_dir, "serve_cmfd" if use_cmfd else "serve"), n_inner=5 if use_cmfd else 1,
                             use_cmfd=use_cmfd)
     rng = np.random.default_rng(seed)
     cases = [c5_request(rng, i) for i in range(n_req)]
@@ -189,7 +189,7 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     return out
 
 
-def bench_c3(work_dir: str, with_oracle: bool):
+def bench_core51(work_dir: str, with_oracle: bool):
     """The 3x3-assembly core (51x51 pins, vacuum east/south, CMFD, 5 inner
     iterations, CY 64x3 at 0.02 cm): full eigenvalue solve on the device with
     the FP64 CMFD; optionally the FP64 oracle's solve of the same deck."""
@@ -197,7 +197,7 @@ def bench_c3(work_dir: str, with_oracle: bool):
     from mocc_amd.native import HostProblem
     from mocc_amd.sweeper import GpuMoCSweeper
     t0 = time.perf_counter()
-    hp = HostProblem.load(lg.write_config("c3", work_dir))
+    hp = HostProblem.load(lg.write_config("core51", work_dir))
     t_trace = time.perf_counter() - t0
     par = hp.params()
     t0 = time.perf_counter()
@@ -255,8 +255,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rates", default="0.25,0.5,0.75,1.0,1.25",
                     help="comma-separated arrival rates as fractions of the burst throughput")
-    ap.add_argument("--c3", action="store_true", help="add the full eigen solve of the 51x51-pin core (3x3 assemblies, 7 groups) with CMFD")
-    ap.add_argument("--c3-oracle", action="store_true", help="with --c3: also the FP64 oracle solve")
+    ap.add_argument("--core51", action="store_true", help="add the full eigen solve of the 51x51-pin core (3x3 assemblies, 7 groups) with CMFD")
+    ap.add_argument("--core51-oracle", action="store_true", help="with --core51: also the FP64 oracle solve")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -273,7 +273,7 @@ def main():
     from mocc_amd.sweeper import GpuMoCSweeper
 
     deck_dir = tempfile.mkdtemp(prefix="mocc_bench_")
-    path = lg.write_config("c2", deck_dir)
+    path = lg.write_config("assembly17", deck_dir)
     hp = HostProblem.load(path)
     p = hp.p
     dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=1)
@@ -302,7 +302,7 @@ def main():
     _, kernel_ms = dev.time_sweeps(max(10, min(args.steps, 50)))
     B = algorithmic_bytes(p)
     achieved = B / (kernel_ms * 1e-3) / 1e9
-    pmc = read_pmc(os.path.join(ROOT, "profiles", "sweep_c2_pmc.txt"))
+    pmc = read_pmc(os.path.join(ROOT, "profiles", "sweep_assembly17_pmc.txt"))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": pmc.get("hbm_bytes_per_sweep"),
                 "kernel": "k_sweep<3,false>, both Gauss-Seidel phase launches of one sweep",
@@ -338,8 +338,8 @@ def main():
         rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
         line["serving"] = bench_serving(deck_dir, args.serve_requests, rates)
         progress("serving done")
-    if args.c3:
-        line["c3"] = bench_c3(deck_dir, args.c3_oracle)
+    if args.core51:
+        line["core51"] = bench_core51(deck_dir, args.core51_oracle)
         progress("51x51-pin core solve done")
     print(json.dumps(line), flush=True)
 

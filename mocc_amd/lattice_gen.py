@@ -216,30 +216,30 @@ def pin_lattice(n: int, groups: int, n_azimuthal: int, spacing: float, name: str
     return d
 
 
-def c1_deck() -> Deck:
+def pins3x3_deck() -> Deck:
     """3x3 pins, 2 groups, CY 16x3, spacing 0.05 cm, all reflective."""
-    return pin_lattice(3, 2, 16, 0.05, "c1_pin3x3_2g")
+    return pin_lattice(3, 2, 16, 0.05, "pin3x3_2g")
 
 
-def c2_deck() -> Deck:
+def assembly17_deck() -> Deck:
     """17x17 assembly with 25 moderator-filled guide tubes, 7 groups, CY 32x3, 0.03 cm."""
-    return pin_lattice(17, 7, 32, 0.03, "c2_asm17_7g", guide_tubes=True)
+    return pin_lattice(17, 7, 32, 0.03, "asm17_7g", guide_tubes=True)
 
 
-def c3_deck() -> Deck:
+def core51_deck() -> Deck:
     """3x3 assemblies of 17x17 (3 material sets), 7 groups, CY 64x3, 0.02 cm,
     vacuum east/south, reflective west/north, CMFD on, 5 inner iterations
     (as in mocc's example inputs: with a single inner iteration the D-hat of
     the first, unconverged sweep makes the coarse matrix lose its positive
     diagonal on this deck and mocc's CMFD iteration diverges)."""
-    d = Deck("c3_core3x3_7g", 7, 64, 0.02, n_sets=3, cmfd=True, n_inner=5,
+    d = Deck("core3x3_7g", 7, 64, 0.02, n_sets=3, cmfd=True, n_inner=5,
              bc={"east": "vacuum", "south": "vacuum"})
     d.lattices = {s + 1: (17, 17, lattice_rows(17, s, True)) for s in range(3)}
     d.core = [[1, 2, 3], [2, 3, 1], [3, 1, 2]]
     return d
 
 
-CONFIGS = {"c1": (c1_deck, 1), "c2": (c2_deck, 2), "c3": (c3_deck, 3)}
+CONFIGS = {"pins3x3": (pins3x3_deck, 1), "assembly17": (assembly17_deck, 2), "core51": (core51_deck, 3)}
 
 
 def write_config(key: str, out_dir: str, **overrides) -> str:

@@ -21,15 +21,15 @@ def deck_dir(tmp_path_factory):
 
 
 @pytest.fixture(scope="session")
-def c1_path(deck_dir):
+def pins3x3_path(deck_dir):
     from mocc_amd import lattice_gen as lg
-    return lg.write_config("c1", deck_dir)
+    return lg.write_config("pins3x3", deck_dir)
 
 
 @pytest.fixture(scope="session")
-def c1(c1_path):
+def pins3x3(pins3x3_path):
     from mocc_amd.native import HostProblem
-    return HostProblem.load(c1_path)
+    return HostProblem.load(pins3x3_path)
 
 
 @pytest.fixture(scope="session")
@@ -41,8 +41,8 @@ def cmfd_decks(deck_dir):
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {
-        "c1": lg.write_config("c1", deck_dir),
-        "asm17": lg.write_config("c2", deck_dir, name="asm17_cmfd", n_azimuthal=4, spacing=0.1, groups=2),
-        "core51": lg.write_config("c3", deck_dir, name="core51_cmfd", n_azimuthal=4, spacing=0.1),
+        "pins3x3": lg.write_config("pins3x3", deck_dir),
+        "asm17": lg.write_config("assembly17", deck_dir, name="asm17_cmfd", n_azimuthal=4, spacing=0.1, groups=2),
+        "core51": lg.write_config("core51", deck_dir, name="core51_cmfd", n_azimuthal=4, spacing=0.1),
     }
     return {k: HostProblem.load(v) for k, v in paths.items()}

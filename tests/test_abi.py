@@ -56,15 +56,15 @@ def test_bad_arguments_are_status_codes():
     assert b"null" in gpu().mocgpu_last_error(None)
 
 
-def test_no_device_is_a_status_code(c1):
+def test_no_device_is_a_status_code(pins3x3):
     """Without a usable device, creating a handle fails with a status code
     and a message (no crash and no CPU fallback)."""
     import torch
     if torch.cuda.is_available():
         pytest.skip("a GPU is present")
     from mocc_amd.native import MocgpuOptions, gpu
-    o = c1.options()
+    o = pins3x3.options()
     h = C.c_void_p()
-    rc = gpu().mocgpu_create(c1.view, C.byref(o), C.byref(h))
+    rc = gpu().mocgpu_create(pins3x3.view, C.byref(o), C.byref(h))
     assert rc == -2
     assert gpu().mocgpu_last_error(None)

@@ -26,8 +26,8 @@ def batch_decks(deck_dir):
     from mocc_amd.native import HostProblem
     decks = {
         "pin1": lg.write_deck(lg.pin_lattice(1, 2, 16, 0.05, "b_pin1_2g"), deck_dir, 21),
-        "c1": lg.write_config("c1", deck_dir),
-        "c1pert": lg.write_deck(dataclasses.replace(lg.c1_deck(), name="b_c1_pert"), deck_dir, 1,
+        "pins3x3": lg.write_config("pins3x3", deck_dir),
+        "pins3x3_pert": lg.write_deck(dataclasses.replace(lg.pins3x3_deck(), name="b_pins3x3_pert"), deck_dir, 1,
                                 rel_perturb=0.1),
         "asm17": lg.write_deck(lg.pin_lattice(17, 2, 4, 0.1, "b_asm17_2g", guide_tubes=True), deck_dir, 2),
     }
@@ -52,7 +52,7 @@ def _solve_batch(sweepers, order, use_graph=True):
 
 
 def test_batch_matches_single(batch_decks):
-    names = ["pin1", "c1", "c1pert", "asm17"]
+    names = ["pin1", "pins3x3", "pins3x3_pert", "asm17"]
     sw = _sweepers(batch_decks, names)
     res = _solve_batch(sw, names)
     assert set(res) == set(names)
@@ -71,11 +71,11 @@ def test_batch_matches_single(batch_decks):
 
 
 def test_alone_vs_mixed_bitwise(batch_decks):
-    target = "c1pert"
+    target = "pins3x3_pert"
     alone = _sweepers(batch_decks, [target])
     r_alone = _solve_batch(alone, [target])[target]
     f_alone = alone[target].flux()
-    for order in (["asm17", "pin1", target, "c1"], [target, "c1", "asm17"]):
+    for order in (["asm17", "pin1", target, "pins3x3"], [target, "pins3x3", "asm17"]):
         sw = _sweepers(batch_decks, order)
         r = _solve_batch(sw, order)[target]
         assert r.k == r_alone.k and r.iterations == r_alone.iterations
@@ -85,7 +85,7 @@ def test_alone_vs_mixed_bitwise(batch_decks):
 
 def test_graph_replays_match_eager(batch_decks):
     from mocc_amd.sweeper import GpuBatch
-    names = ["pin1", "c1", "asm17"]
+    names = ["pin1", "pins3x3", "asm17"]
     sg, se = _sweepers(batch_decks, names), _sweepers(batch_decks, names)
     bg, be = GpuBatch(use_graph=True), GpuBatch(use_graph=False)
     for n in names:
@@ -103,25 +103,25 @@ def test_graph_replays_match_eager(batch_decks):
 
 def test_admission_budget(batch_decks):
     from mocc_amd.sweeper import CapacityError, GpuBatch
-    sw = _sweepers(batch_decks, ["c1", "c1pert"])
-    need = sw["c1"].state_bytes()
-    assert need > 0 and sw["c1pert"].state_bytes() == need
+    sw = _sweepers(batch_decks, ["pins3x3", "pins3x3_pert"])
+    need = sw["pins3x3"].state_bytes()
+    assert need > 0 and sw["pins3x3_pert"].state_bytes() == need
     b = GpuBatch(budget_bytes=need + need // 2)
-    i1 = b.add(sw["c1"], K_TOL, PSI_TOL, MAX_IT)
+    i1 = b.add(sw["pins3x3"], K_TOL, PSI_TOL, MAX_IT)
     with pytest.raises(CapacityError):
-        b.add(sw["c1pert"], K_TOL, PSI_TOL, MAX_IT)
+        b.add(sw["pins3x3_pert"], K_TOL, PSI_TOL, MAX_IT)
     assert b.size() == 1 and b.bytes_used() == need
     b.remove(i1)
-    b.add(sw["c1pert"], K_TOL, PSI_TOL, MAX_IT)
+    b.add(sw["pins3x3_pert"], K_TOL, PSI_TOL, MAX_IT)
     res = b.solve_all()
     assert len(res) == 1 and list(res.values())[0].converged
 
 
 def test_create_like_shares_geometry(batch_decks):
     from mocc_amd.sweeper import DeviceError, GpuMoCSweeper
-    a = GpuMoCSweeper(batch_decks["c1"], energy_mode=1)
-    b = GpuMoCSweeper(batch_decks["c1pert"], energy_mode=1, like=a)
-    c = GpuMoCSweeper(batch_decks["c1pert"], energy_mode=1)
+    a = GpuMoCSweeper(batch_decks["pins3x3"], energy_mode=1)
+    b = GpuMoCSweeper(batch_decks["pins3x3_pert"], energy_mode=1, like=a)
+    c = GpuMoCSweeper(batch_decks["pins3x3_pert"], energy_mode=1)
     print("geometry bytes: shared", b.geometry_bytes(), "own", c.geometry_bytes(), "state", b.state_bytes(),
           flush=True)
     assert b.geometry_bytes() > 0 and c.geometry_bytes() > 0
@@ -143,7 +143,7 @@ def test_batch_cmfd_matches_single(batch_decks):
     the batch runs each one's CMFD step (one solve workgroup per problem)
     and its last inner sweep tallies currents; each converges like its
     one-at-a-time device solve."""
-    names = ["c1", "c1pert", "asm17"]
+    names = ["pins3x3", "pins3x3_pert", "asm17"]
     sw = _cmfd_sweepers(batch_decks, names)
     res = _solve_batch(sw, names)
     for n in names:
@@ -163,7 +163,7 @@ def test_batch_cmfd_matches_single(batch_decks):
 def test_batch_cmfd_alone_vs_mixed_bitwise(batch_decks):
     """A CMFD problem mixed with problems without CMFD: bitwise the same."""
     from mocc_amd.sweeper import GpuMoCSweeper
-    target = "c1pert"
+    target = "pins3x3_pert"
     alone = _cmfd_sweepers(batch_decks, [target])
     r_alone = _solve_batch(alone, [target])[target]
     f_alone = alone[target].flux()
@@ -176,7 +176,7 @@ def test_batch_cmfd_alone_vs_mixed_bitwise(batch_decks):
 
 def test_batch_cmfd_graph_matches_eager(batch_decks):
     from mocc_amd.sweeper import GpuBatch
-    names = ["c1", "asm17"]
+    names = ["pins3x3", "asm17"]
     sg, se = _cmfd_sweepers(batch_decks, names), _cmfd_sweepers(batch_decks, names)
     bg, be = GpuBatch(use_graph=True), GpuBatch(use_graph=False)
     for n in names:
@@ -198,14 +198,14 @@ def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks):
     from mocc_amd.sweeper import GpuMoCSweeper
     mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
     print("cmfd 3x3 pins alone", flush=True)
-    alone = {"c1": mk("c1")}
-    r_alone = _solve_batch(alone, ["c1"])["c1"]
+    alone = {"pins3x3": mk("pins3x3")}
+    r_alone = _solve_batch(alone, ["pins3x3"])["pins3x3"]
     print("cmfd 3x3 pins beside the 51x51 core", flush=True)
-    mixed = {"core51": mk("core51"), "c1": mk("c1")}
-    r = _solve_batch(mixed, ["core51", "c1"])
-    assert r["c1"].converged and r["core51"].converged, (r["c1"].iterations, r["core51"].iterations)
-    assert r["c1"].k == r_alone.k and r["c1"].iterations == r_alone.iterations
-    assert np.array_equal(mixed["c1"].flux(), alone["c1"].flux())
+    mixed = {"core51": mk("core51"), "pins3x3": mk("pins3x3")}
+    r = _solve_batch(mixed, ["core51", "pins3x3"])
+    assert r["pins3x3"].converged and r["core51"].converged, (r["pins3x3"].iterations, r["core51"].iterations)
+    assert r["pins3x3"].k == r_alone.k and r["pins3x3"].iterations == r_alone.iterations
+    assert np.array_equal(mixed["pins3x3"].flux(), alone["pins3x3"].flux())
 
 
 def test_batch_plain_problem_beside_cmfd_bitwise(batch_decks):
@@ -219,8 +219,8 @@ def test_batch_plain_problem_beside_cmfd_bitwise(batch_decks):
     print("17x17 without CMFD alone", flush=True)
     r_alone = _solve_batch(alone, [target])[target]
     mixed = {target: GpuMoCSweeper(batch_decks[target], energy_mode=1, n_inner=5),
-             "c1": GpuMoCSweeper(batch_decks["c1"], energy_mode=1, n_inner=5, use_cmfd=True)}
+             "pins3x3": GpuMoCSweeper(batch_decks["pins3x3"], energy_mode=1, n_inner=5, use_cmfd=True)}
     print("17x17 without CMFD beside a CMFD problem", flush=True)
-    r = _solve_batch(mixed, ["c1", target])[target]
+    r = _solve_batch(mixed, ["pins3x3", target])[target]
     assert r.k == r_alone.k and r.iterations == r_alone.iterations
     assert np.array_equal(mixed[target].flux(), alone[target].flux())

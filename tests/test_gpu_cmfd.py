@@ -47,7 +47,7 @@ def _check_pair(hp, tight=True):
     return dev, ora, kd, ko
 
 
-@pytest.mark.parametrize("deck", ["c1", "asm17", "core51"])
+@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "core51"])
 def test_cmfd_step_parity(deck, cmfd_decks):
     hp = cmfd_decks[deck]
     dev, ora, kd, ko = _check_pair(hp, tight=True)
@@ -67,7 +67,7 @@ def test_cmfd_step_parity(deck, cmfd_decks):
         np.testing.assert_allclose(s_d, s_o, rtol=1e-8, atol=1e-14 * np.abs(s_o).max())
 
 
-@pytest.mark.parametrize("deck", ["c1", "core51"])
+@pytest.mark.parametrize("deck", ["pins3x3", "core51"])
 def test_cmfd_step_default_tolerances(deck, cmfd_decks):
     hp = cmfd_decks[deck]
     dev, ora, kd, ko = _check_pair(hp, tight=False)
@@ -75,9 +75,9 @@ def test_cmfd_step_default_tolerances(deck, cmfd_decks):
     assert dev.cmfd_iterations() >= 1
 
 
-def test_cmfd_rejects_without_option(c1):
+def test_cmfd_rejects_without_option(pins3x3):
     from mocc_amd.sweeper import DeviceError, GpuMoCSweeper
-    dev = GpuMoCSweeper(c1, n_inner=1, use_cmfd=False)
+    dev = GpuMoCSweeper(pins3x3, n_inner=1, use_cmfd=False)
     dev.initialize()
     with pytest.raises(DeviceError):
         dev.cmfd_step(1.0)

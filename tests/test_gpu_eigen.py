@@ -39,10 +39,10 @@ def _solve_pair(hp, use_cmfd, max_iter):
 
 
 @pytest.mark.parametrize("use_cmfd", [False, True])
-def test_c1_keff(c1, use_cmfd):
-    dev, ora, st_d, st_o = _solve_pair(c1, use_cmfd, 3000)
+def test_pins3x3_keff(pins3x3, use_cmfd):
+    dev, ora, st_d, st_o = _solve_pair(pins3x3, use_cmfd, 3000)
     assert abs(st_d.k - st_o.k) < 1e-5, (st_d.k, st_o.k)
-    fd, fo = _fission_shape(c1, dev.flux()), _fission_shape(c1, ora.flux())
+    fd, fo = _fission_shape(pins3x3, dev.flux()), _fission_shape(pins3x3, ora.flux())
     m = fo > 0
     assert np.abs(fd[m] / fo[m] - 1.0).max() < 1e-4
 

@@ -17,7 +17,7 @@ def test_solve_history_and_output_file(tmp_path, deck_dir):
     from mocc_amd import output as h5
     from mocc_amd.native import HostProblem
     from mocc_amd.sweeper import GpuMoCSweeper
-    path = lg.write_config("c1", deck_dir)
+    path = lg.write_config("pins3x3", deck_dir)
     xml = open(path).read().replace('<sweeper type="moc"', '<sweeper type="moc" dump_fsr_flux="t"', 1)
     xml = xml.replace('cmfd="f"', 'cmfd="t"', 1)
     xml = xml.replace("</solver>", '  <cmfd dump_current="t" />\n</solver>', 1)
@@ -26,9 +26,9 @@ def test_solve_history_and_output_file(tmp_path, deck_dir):
         f.write(xml)
     hp = HostProblem.load(deck)
     assert h5.output_info(hp).cmfd == 1
-    print("c1 solve with CMFD and history", flush=True)
+    print("pins3x3 solve with CMFD and history", flush=True)
     dev = GpuMoCSweeper(hp, n_inner=1)
-    out = str(tmp_path / "c1.h5")
+    out = str(tmp_path / "pins3x3.h5")
     rc, st, hist = h5.solve_and_write(dev, out, K_TOL, PSI_TOL, MAX_IT)
     assert rc == 0 and st.converged
     # the same solve without history: same iterations and bitwise the same k

@@ -69,8 +69,8 @@ def test_attenuation_matches_reference_interpolant():
 
 @pytest.mark.parametrize("gs", [True, False])
 @pytest.mark.parametrize("energy_mode", [1, 0])
-def test_c1_fss_step_parity(c1, energy_mode, gs):
-    dev, ora = _setup_pair(c1, energy_mode, gs=gs)
+def test_pins3x3_fss_step_parity(pins3x3, energy_mode, gs):
+    dev, ora = _setup_pair(pins3x3, energy_mode, gs=gs)
     for s in (dev, ora):
         s.fss_step(with_current=True)
     phi_d, phi_o = dev.flux(), ora.flux()
@@ -78,9 +78,9 @@ def test_c1_fss_step_parity(c1, energy_mode, gs):
     bd, bo = dev.boundary(), ora.boundary()
     assert np.all((bd == 0) == (bo == 0))
     assert _rel(bd, bo) < 1e-5
-    p = c1.p
+    p = pins3x3.p
     n0, n1 = p.nx * p.ny, p.nx * p.ny + (p.nx + 1) * p.ny + (p.ny + 1) * p.nx
-    for g in range(c1.n_group):
+    for g in range(pins3x3.n_group):
         cd, sd = dev.currents(g)
         co, so = ora.currents(g)
         scale = np.abs(co[n0:n1]).max()
@@ -89,41 +89,41 @@ def test_c1_fss_step_parity(c1, energy_mode, gs):
 
 
 @pytest.mark.parametrize("gs", [True, False])
-def test_c1_one_wave_workgroup_parity(c1, gs):
+def test_pins3x3_one_wave_workgroup_parity(pins3x3, gs):
     """Bands of at most 32 rays run the one-wave workgroup (32 rays per
     direction, 64 lanes): the same parity against the oracle, currents
     included."""
-    dev, ora = _setup_pair(c1, 1, seed=5, gs=gs, band_rays=32)
+    dev, ora = _setup_pair(pins3x3, 1, seed=5, gs=gs, band_rays=32)
     assert dev.options().band_rays == 32
     for s in (dev, ora):
         s.fss_step(with_current=True)
     assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
     assert _rel(dev.boundary(), ora.boundary()) < 1e-5
-    p = c1.p
+    p = pins3x3.p
     n0, n1 = p.nx * p.ny, p.nx * p.ny + (p.nx + 1) * p.ny + (p.ny + 1) * p.nx
-    for g in range(c1.n_group):
+    for g in range(pins3x3.n_group):
         cd, _ = dev.currents(g)
         co, _ = ora.currents(g)
         assert np.abs(cd[n0:n1] - co[n0:n1]).max() <= 1e-5 * np.abs(co[n0:n1]).max()
 
 
-def test_c1_reproducible(c1):
+def test_pins3x3_reproducible(pins3x3):
     """Integer band tallies: two runs give bitwise identical fluxes."""
     outs = []
     for _ in range(2):
-        dev, _ora = _setup_pair(c1, 1, seed=3)
+        dev, _ora = _setup_pair(pins3x3, 1, seed=3)
         dev.fss_step(with_current=True)
         dev.fss_step(with_current=False)
         outs.append(dev.flux())
     assert np.array_equal(outs[0], outs[1])
 
 
-def test_c1_fission_and_pin_flux(c1):
-    dev, ora = _setup_pair(c1, 0)
+def test_pins3x3_fission_and_pin_flux(pins3x3):
+    dev, ora = _setup_pair(pins3x3, 0)
     fd, fo = dev.calc_fission_source(1.1), ora.calc_fission_source(1.1)
     assert np.allclose(fd, fo, rtol=1e-14, atol=0)
     assert dev.total_fission() == pytest.approx(ora.total_fission(), rel=1e-13)
-    for g in range(c1.n_group):
+    for g in range(pins3x3.n_group):
         pd, po = dev.get_pin_flux_1g(g), ora.pin_flux(g)
         assert np.allclose(pd, po, rtol=1e-13)
         rd = dev.set_pin_flux_1g(g, pd * 1.5)
@@ -133,35 +133,35 @@ def test_c1_fission_and_pin_flux(c1):
 
 
 @pytest.fixture(scope="module")
-def c2(deck_dir):
+def assembly17(deck_dir):
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
-    return HostProblem.load(lg.write_config("c2", deck_dir))
+    return HostProblem.load(lg.write_config("assembly17", deck_dir))
 
 
 @pytest.mark.parametrize("gs", [True, False])
-def test_c2_fss_step_parity(c2, gs):
+def test_assembly17_fss_step_parity(assembly17, gs):
     """The benchmark workload itself (17x17, 7 groups, 6.7M segments): one
     outer step with currents agrees with the FP64 oracle to 1e-4 (observed
     ~1e-6) in every FSR and group."""
-    print(f"c2 gs={gs}", flush=True)
-    dev, ora = _setup_pair(c2, 0, seed=1, gs=gs)
+    print(f"assembly17 gs={gs}", flush=True)
+    dev, ora = _setup_pair(assembly17, 0, seed=1, gs=gs)
     for s in (dev, ora):
         s.fss_step(with_current=True)
     assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
     assert _rel(dev.boundary(), ora.boundary()) < 1e-4
 
 
-def test_autotuned_lane_map(c1):
+def test_autotuned_lane_map(pins3x3):
     """lane_map = -1: the autotuner picks a band width and lane map by timing
     the candidates; the tuned handle sweeps as accurately as the default."""
     from mocc_amd.sweeper import GpuMoCSweeper
-    dev = GpuMoCSweeper(c1, n_inner=1, energy_mode=0, gs_boundary=True, lane_map=-1)
+    dev = GpuMoCSweeper(pins3x3, n_inner=1, energy_mode=0, gs_boundary=True, lane_map=-1)
     o = dev.options()
     assert o.band_rays in (32, 48, 64) and o.lane_map in (0, 1)
-    ora = Oracle(c1, n_inner=1, gs_boundary=True, energy_mode=0)
+    ora = Oracle(pins3x3, n_inner=1, gs_boundary=True, energy_mode=0)
     rng = np.random.default_rng(2)
-    phi0 = rng.uniform(0.5, 2.0, (c1.n_fsr, c1.n_group))
+    phi0 = rng.uniform(0.5, 2.0, (pins3x3.n_fsr, pins3x3.n_group))
     for s in (dev, ora):
         s.initialize()
         s.set_flux(phi0)

@@ -209,11 +209,11 @@ def test_raydata_volume_identity():
         hp.close()
 
 
-def test_raydata_volume_identity_cyl(c1):
+def test_raydata_volume_identity_cyl(pins3x3):
     # FLAT correction (ray_data.cpp:350-416): every FSR's ray-traced volume
     # equals its true area, per angle, for the cylindrical pins of a lattice
-    areas = np.zeros(c1.n_fsr)
-    n = host().mocc_host_fsr_areas(c1.h, _p(areas), c1.n_fsr)
-    assert n == c1.n_fsr
-    vol = _ray_volumes(c1)
+    areas = np.zeros(pins3x3.n_fsr)
+    n = host().mocc_host_fsr_areas(pins3x3.h, _p(areas), pins3x3.n_fsr)
+    assert n == pins3x3.n_fsr
+    vol = _ray_volumes(pins3x3)
     np.testing.assert_allclose(vol, areas * 4.0 * PI, rtol=1e-12)

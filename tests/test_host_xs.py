@@ -32,21 +32,21 @@ def _view_xs(hp):
 
 
 def test_deck_xs_match_materials(deck_dir):
-    path = lg.write_config("c1", deck_dir)
+    path = lg.write_config("pins3x3", deck_dir)
     hp = HostProblem.load(path)
     # materials of the deck, in material-id order (FUEL0 id 1, MOD0 id 2)
-    tr, nf, f, ch, sc, G = _expected(lg.deck_materials(lg.c1_deck(), 1))
+    tr, nf, f, ch, sc, G = _expected(lg.deck_materials(lg.pins3x3_deck(), 1))
     got = _view_xs(hp)
     for a, b in zip(got, (tr, nf, f, ch, sc)):
         np.testing.assert_allclose(a, b, rtol=1e-15, atol=0)
 
 
 def test_reload_branch_case(deck_dir):
-    path = lg.write_config("c1", deck_dir)
+    path = lg.write_config("pins3x3", deck_dir)
     hp = HostProblem.load(path)
     n_seg, fm = hp.p.n_seg, hp.array("fsr_mat", hp.n_fsr, np.int64)
     xsl = os.path.join(deck_dir, "c1_branch7.xsl")
-    mats = lg.write_branch_xsl(lg.c1_deck(), 1, 7, 0.1, xsl)
+    mats = lg.write_branch_xsl(lg.pins3x3_deck(), 1, 7, 0.1, xsl)
     base = _view_xs(hp)
     hp.reload_xs(xsl)
     tr, nf, f, ch, sc, G = _expected(mats)
@@ -61,10 +61,10 @@ def test_reload_branch_case(deck_dir):
 
 def test_reload_rejects_missing_material(deck_dir, tmp_path):
     import pytest
-    path = lg.write_config("c1", deck_dir)
+    path = lg.write_config("pins3x3", deck_dir)
     hp = HostProblem.load(path)
     bad = tmp_path / "bad.xsl"
-    mats = lg.deck_materials(lg.c1_deck(), 1)
+    mats = lg.deck_materials(lg.pins3x3_deck(), 1)
     mats[0].name = "OTHER"
     lg.write_xsl(str(bad), mats)
     with pytest.raises(RuntimeError, match="not found"):

@@ -99,19 +99,19 @@ def test_ihm_cmfd_k_infinity(ihm):
     np.testing.assert_allclose(ratio, ratio[0, 0], rtol=1e-8)
 
 
-def test_pin_flux_identity(c1):
-    o = Oracle(c1, n_inner=2)
+def test_pin_flux_identity(pins3x3):
+    o = Oracle(pins3x3, n_inner=2)
     o.initialize()
     o.fss_step()  # a non-flat flux
-    for g in range(c1.n_group):
+    for g in range(pins3x3.n_group):
         p1 = o.pin_flux(g)
         o.set_pin_flux(g, p1)
         p2 = o.pin_flux(g)
         np.testing.assert_allclose(p2, p1, rtol=4 * np.finfo(float).eps, atol=0)
 
 
-def test_set_pin_flux_rejects_nonpositive(c1):
-    o = Oracle(c1)
+def test_set_pin_flux_rejects_nonpositive(pins3x3):
+    o = Oracle(pins3x3)
     o.initialize()
     p = o.pin_flux(0)
     p[3] = 0.0
@@ -169,10 +169,10 @@ def _pin_balance(hp, n_inner=400):
     return worst
 
 
-def test_pin_balance_with_currents(c1):
+def test_pin_balance_with_currents(pins3x3):
     """At inner convergence every pin balances: sources - removal = net
     outflow from the coarse-surface currents."""
-    assert _pin_balance(c1) < 1e-9
+    assert _pin_balance(pins3x3) < 1e-9
 
 
 def _square_lattice(deck_dir, n, spacing, core=1):
@@ -294,21 +294,21 @@ def test_bicgstab_vs_dense():
     assert 0 < it[0] <= 150
 
 
-def test_cmfd_matrix_structure(c1):
+def test_cmfd_matrix_structure(pins3x3):
     """The assembled CMFD matrix (cmfd.cpp:506-544) with D-hat = 0 on the
     first solve is symmetric with row sums V * Sigma_r plus the boundary
     couplings (zero for reflective faces)."""
-    o = Oracle(c1, n_inner=1, use_cmfd=True)
+    o = Oracle(pins3x3, n_inner=1, use_cmfd=True)
     o.initialize()
     o.fss_step()
     o.cmfd_step(1.0)
-    p = c1.p
+    p = pins3x3.p
     nc = p.nx * p.ny
     rp = np.zeros(nc + 1, np.int32)
     col = np.zeros(5 * nc, np.int32)
     val = np.zeros(5 * nc)
     P = C.c_void_p
-    for g in range(c1.n_group):
+    for g in range(pins3x3.n_group):
         nnz = lib().oracle_cmfd_matrix(o.h, g, rp.ctypes.data_as(P), col.ctypes.data_as(P),
                                        val.ctypes.data_as(P), 5 * nc)
         assert nnz == rp[-1] and nnz == nc + 2 * (2 * p.nx * p.ny - p.nx - p.ny)

@@ -66,9 +66,9 @@ def _solve_oracle(hp, n_outer=4):
 
 
 def test_datasets_and_shapes(tmp_path):
-    hp, _ = _deck(tmp_path, "c1", fsr=True)
+    hp, _ = _deck(tmp_path, "pins3x3", fsr=True)
     o, hist = _solve_oracle(hp)
-    out = str(tmp_path / "c1.h5")
+    out = str(tmp_path / "pins3x3.h5")
     h5.write(hp, out, o.flux(), hist)
     G, F = hp.n_group, hp.n_fsr
     nx, ny = hp.p.nx, hp.p.ny
@@ -96,7 +96,7 @@ def test_datasets_and_shapes(tmp_path):
 
 
 def test_no_optional_groups_by_default(tmp_path):
-    hp, _ = _deck(tmp_path, "c1")
+    hp, _ = _deck(tmp_path, "pins3x3")
     o, hist = _solve_oracle(hp, 2)
     out = str(tmp_path / "plain.h5")
     h5.write(hp, out, o.flux(), hist)
@@ -105,9 +105,9 @@ def test_no_optional_groups_by_default(tmp_path):
 
 
 def test_pin_flux_is_normalized_oracle_pin_flux(tmp_path):
-    hp, _ = _deck(tmp_path, "c1")
+    hp, _ = _deck(tmp_path, "pins3x3")
     o, hist = _solve_oracle(hp)
-    out = str(tmp_path / "c1.h5")
+    out = str(tmp_path / "pins3x3.h5")
     h5.write(hp, out, o.flux(), hist)
     G, nx, ny = hp.n_group, hp.p.nx, hp.p.ny
     pins = np.stack([o.pin_flux(g) for g in range(G)])      # [G, cells], cell = x + nx*y
@@ -121,9 +121,9 @@ def test_pin_flux_is_normalized_oracle_pin_flux(tmp_path):
 
 
 def test_pin_powers_follow_the_fission_source(tmp_path):
-    hp, _ = _deck(tmp_path, "c1")
+    hp, _ = _deck(tmp_path, "pins3x3")
     o, hist = _solve_oracle(hp)
-    out = str(tmp_path / "c1.h5")
+    out = str(tmp_path / "pins3x3.h5")
     h5.write(hp, out, o.flux(), hist)
     fs = np.asarray(o.calc_fission_source(1.0))            # (1/k) sum_g nu Sigma_f phi
     vol = hp.array("fsr_vol", hp.n_fsr)
@@ -159,7 +159,7 @@ def test_flat_flux_closed_forms(tmp_path, deck_dir):
 
 
 def test_eubounds_and_quadrature(tmp_path):
-    hp, path = _deck(tmp_path, "c1")
+    hp, path = _deck(tmp_path, "pins3x3")
     o, hist = _solve_oracle(hp, 1)
     out = str(tmp_path / "q.h5")
     h5.write(hp, out, o.flux(), hist)
@@ -179,7 +179,7 @@ def test_eubounds_and_quadrature(tmp_path):
 
 
 def test_cmfd_currents_layout(tmp_path):
-    hp, _ = _deck(tmp_path, "c1", current=True)
+    hp, _ = _deck(tmp_path, "pins3x3", current=True)
     from oracle.binding import Oracle
     o = Oracle(hp, n_inner=1, gs_boundary=True, energy_mode=0)
     o.initialize()

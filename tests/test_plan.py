@@ -17,8 +17,8 @@ def _check(hp, width):
 
 
 @pytest.mark.parametrize("width", [64, 48, 17, 1])
-def test_plan_c1(c1, width):
-    rc, msg = _check(c1, width)
+def test_plan_pins3x3(pins3x3, width):
+    rc, msg = _check(pins3x3, width)
     assert rc == 0, msg
     assert msg.startswith("ok")
 
@@ -32,6 +32,6 @@ def test_plan_vacuum_and_cmfd_geometry(cmfd_decks):
 def test_plan_c2(deck_dir):
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
-    hp = HostProblem.load(lg.write_config("c2", deck_dir))
+    hp = HostProblem.load(lg.write_config("assembly17", deck_dir))
     rc, msg = _check(hp, 64)
     assert rc == 0, msg
