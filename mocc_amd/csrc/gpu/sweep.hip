@@ -478,9 +478,40 @@ __device__ __forceinline__ void reduce_current_item(const CurrentArgs &A, int t)
     A.sfl[(size_t)s * A.G + g] = f / A.area[s];
 }
 
+// The one-problem launch gives each surface kCurLanes lanes: they take the
+// slots k = first + lane, first + lane + kCurLanes, ... and an xor butterfly
+// adds their sums, an order fixed by the slot list. A 51x51-pin mesh has
+// only ~7900 surfaces per group, too few to fill the chip one lane each.
+constexpr int kCurLanes = 16;
+
 __global__ void k_reduce_current(CurrentArgs A)
 {
-    reduce_current_item(A, blockIdx.x * blockDim.x + threadIdx.x);
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gid / kCurLanes, lane = gid % kCurLanes;
+    int s, g;
+    if (A.g_fixed >= 0) {
+        s = t;
+        g = A.g_fixed;
+    } else {
+        s = t / A.G;
+        g = t % A.G;
+    }
+    const bool valid = s < A.n1 - A.n0;
+    s += A.n0;
+    double c = 0.0, f = 0.0;
+    if (valid)
+        for (int k = A.ptr[s] + lane; k < A.ptr[s + 1]; k += kCurLanes) {
+            c += A.csum[(size_t)g * A.n_lslots + A.idx[k]];
+            f += A.ssum[(size_t)g * A.n_lslots + A.idx[k]];
+        }
+#pragma unroll
+    for (int o = 1; o < kCurLanes; o <<= 1) {
+        c += __shfl_xor(c, o, kCurLanes);
+        f += __shfl_xor(f, o, kCurLanes);
+    }
+    if (!valid || lane != 0) return;
+    A.cur[(size_t)s * A.G + g] = c / A.area[s];
+    A.sfl[(size_t)s * A.G + g] = f / A.area[s];
 }
 
 // The same over the problems of a batch whose last inner iteration this is
@@ -614,7 +645,8 @@ int sweep_grid_items(int n_bands, int G, bool all_groups)
 
 void launch_reduce_current(const CurrentArgs &a, size_t n_items, hipStream_t s)
 {
-    if (n_items > 0) hipLaunchKernelGGL(k_reduce_current, dim3((n_items + 255) / 256), dim3(256), 0, s, a);
+    if (n_items > 0)
+        hipLaunchKernelGGL(k_reduce_current, dim3((n_items * kCurLanes + 255) / 256), dim3(256), 0, s, a);
 }
 
 void launch_reduce_current_batch(const CurrentArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
