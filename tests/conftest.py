@@ -37,12 +37,16 @@ def cmfd_decks(deck_dir):
     """Decks for the CMFD and eigenvalue parity tests: the 3x3-pin deck
     itself, one 17x17 assembly (289 cells) and the 3x3-assembly core geometry
     (51x51 cells, vacuum east/south) with coarser rays so the FP64 oracle
-    finishes in seconds."""
+    finishes in seconds; a 48x48-pin lattice (2304 cells, vacuum east/south)
+    is the size whose coarse solve runs 5 cells per lane with its rows in
+    LDS (the 51x51 core runs 6)."""
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {
         "pins3x3": lg.write_config("pins3x3", deck_dir),
         "asm17": lg.write_config("assembly17", deck_dir, name="asm17_cmfd", n_azimuthal=4, spacing=0.1, groups=2),
         "core51": lg.write_config("core51", deck_dir, name="core51_cmfd", n_azimuthal=4, spacing=0.1),
+        "lat48": lg.write_deck(lg.pin_lattice(48, 2, 4, 0.1, "lat48_cmfd", cmfd=True,
+                                              bc={"east": "vacuum", "south": "vacuum"}), deck_dir, 48),
     }
     return {k: HostProblem.load(v) for k, v in paths.items()}

@@ -47,7 +47,7 @@ def _check_pair(hp, tight=True):
     return dev, ora, kd, ko
 
 
-@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "core51"])
+@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "lat48", "core51"])
 def test_cmfd_step_parity(deck, cmfd_decks):
     hp = cmfd_decks[deck]
     dev, ora, kd, ko = _check_pair(hp, tight=True)

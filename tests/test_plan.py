@@ -24,12 +24,12 @@ def test_plan_pins3x3(pins3x3, width):
 
 
 def test_plan_vacuum_and_cmfd_geometry(cmfd_decks):
-    for name in ("asm17", "core51"):
+    for name in ("asm17", "lat48", "core51"):
         rc, msg = _check(cmfd_decks[name], 64)
         assert rc == 0, (name, msg)
 
 
-def test_plan_c2(deck_dir):
+def test_plan_assembly17(deck_dir):
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     hp = HostProblem.load(lg.write_config("assembly17", deck_dir))
