@@ -191,19 +191,21 @@ def test_batch_cmfd_graph_matches_eager(batch_decks):
             assert np.array_equal(sg[n].flux(), se[n].flux()), (rep, n)
 
 
-def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks):
-    """The coarse solve of a batch runs at the width of its largest mesh (6
-    cells per lane for the 51x51 core, against one wave for 3x3 pins alone);
-    a small problem's k, iterations and flux are bitwise the same either way."""
+@pytest.mark.parametrize("big", ["lat48", "core51"])
+def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks, big):
+    """The coarse solve of a batch runs at the width of its largest mesh (5
+    or 6 cells per lane with the rows in LDS for the 48x48 lattice or the
+    51x51 core, against one wave for 3x3 pins alone); a small problem's k,
+    iterations and flux are bitwise the same either way."""
     from mocc_amd.sweeper import GpuMoCSweeper
     mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
     print("cmfd 3x3 pins alone", flush=True)
     alone = {"pins3x3": mk("pins3x3")}
     r_alone = _solve_batch(alone, ["pins3x3"])["pins3x3"]
-    print("cmfd 3x3 pins beside the 51x51 core", flush=True)
-    mixed = {"core51": mk("core51"), "pins3x3": mk("pins3x3")}
-    r = _solve_batch(mixed, ["core51", "pins3x3"])
-    assert r["pins3x3"].converged and r["core51"].converged, (r["pins3x3"].iterations, r["core51"].iterations)
+    print(f"cmfd 3x3 pins beside {big}", flush=True)
+    mixed = {big: mk(big), "pins3x3": mk("pins3x3")}
+    r = _solve_batch(mixed, [big, "pins3x3"])
+    assert r["pins3x3"].converged and r[big].converged, (r["pins3x3"].iterations, r[big].iterations)
     assert r["pins3x3"].k == r_alone.k and r["pins3x3"].iterations == r_alone.iterations
     assert np.array_equal(mixed["pins3x3"].flux(), alone["pins3x3"].flux())
 
