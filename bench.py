@@ -9,9 +9,12 @@ A step is one inner sweep of all 7 groups (sweep kernel + flux reduction),
 i.e. 2 * P * S * G segment-group updates. Inputs are resident in HBM before
 the timed region. The JSON line also carries the roofline of the sweep
 kernel (HIP-event kernel time on the library's stream), the FP64 CPU oracle
-timed on this host, the batched branch-case solves/s against one-at-a-time
-device solves ("batched") and the serving loop's p50/p99 latency under
-Poisson arrivals ("serving").
+timed on this host (all threads and one), the batched branch-case solves/s
+against one-at-a-time device solves ("batched", autotuned band layouts,
+keff of a sample of every lattice kind against FP64), the serving loop's
+p50/p99 latency under Poisson arrivals ("serving", 2000 requests per rate,
+served keffs against FP64) and the full eigen solve of the 51x51-pin core
+("core51", keff against the FP64 oracle's solve of the same deck).
 """
 from __future__ import annotations
 
@@ -63,18 +66,39 @@ def read_pmc(path: str) -> dict:
     return vals
 
 
-def cpu_baseline(hp, n_sweeps: int):
-    """FP64 oracle (reference loop structure, OpenMP) timed on this host."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hp, n_sweeps: int, n_sweeps_1t: int):
+    """FP64 oracle (reference loop structure, OpenMP) timed on this host:
+    all host threads (the baseline value) and one thread."""
     from oracle.binding import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    o = Oracle(hp, n_inner=1, gs_boundary=True, energy_mode=0, n_threads=threads)
-    o.initialize()
-    o.calc_fission_source(1.0)
-    for g in range(hp.n_group):
-        o.build_source(g)
-    ms = o.time_sweeps(n_sweeps)
-    ups = hp.updates_per_sweep() / (ms * 1e-3)
-    return {"value": ups, "unit": "updates/s", "cores": threads, "kind": "port",
+
+    def median_ms(n_threads, n):
+        o = Oracle(hp, n_inner=1, gs_boundary=True, energy_mode=0, n_threads=n_threads)
+        o.initialize()
+        o.calc_fission_source(1.0)
+        for g in range(hp.n_group):
+            o.build_source(g)
+        return o.time_sweeps(n)
+
+    U = hp.updates_per_sweep()
+    ms = median_ms(threads, n_sweeps)
+    ms1 = median_ms(1, n_sweeps_1t)
+    return {"value": U / (ms * 1e-3), "unit": "updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "single_thread": {"value": U / (ms1 * 1e-3), "cores": 1, "ms_per_sweep": ms1,
+                              "sample": f"median of {n_sweeps_1t} sweeps after 1 warm-up"},
+            "ms_per_sweep": ms,
             "sample": f"median of {n_sweeps} all-group inner sweeps of the same 17x17/7-group "
                       f"deck (after 1 warm-up), FP64, sweep1g only, OMP threads={threads}"}
 
@@ -103,11 +127,12 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     from mocc_amd.sweeper import GpuBatch
     n_inner = 5 if use_cmfd else 1
     factory = BranchFactory(os.path.join(work_dir, "batch_cmfd" if use_cmfd else "batch"), n_inner=n_inner,
-                            use_cmfd=use_cmfd)
+                            use_cmfd=use_cmfd, autotune=True)
     cases = c4_cases(n_cases)
     t0 = time.perf_counter()
     handles = [factory.make(c) for c in cases]
     t_setup = time.perf_counter() - t0
+    tuned = {f"{k[0]}@{k[1]:g}cm": {"band_rays": v[0], "lane_map": v[1]} for k, v in factory.tuned().items()}
     batch = GpuBatch(use_graph=True)
     ids = [batch.add(h, K_TOL, PSI_TOL, MAX_IT) for h in handles]
     t0 = time.perf_counter()
@@ -133,22 +158,27 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     progress(f"serial solve of {len(sub)} cases: {t_serial:.2f} s")
     for h in sh:
         h.close()
-    # keff against the FP64 oracle on a sample of the smaller problems
-    small = [i for i, c in enumerate(cases) if c.kind != "17x17"][:: max(1, n_cases // (2 * max(n_oracle, 1)))]
+    # keff against the FP64 oracle on cases spread over the whole mix
+    # (single pins, 3x3 lattices and 17x17 assemblies)
+    sample = list(range(0, n_cases, max(1, n_cases // max(n_oracle, 1))))[:n_oracle]
     dk = []
-    for i in small[:n_oracle]:
+    for i in sample:
         dk.append(abs(k_batch[i] - oracle_k(factory, cases[i])))
-        progress(f"oracle keff of case {i} ({cases[i].kind}): |dk| = {dk[-1]:.2e}")
+        progress(f"oracle keff of case {i} ({cases[i].kind}, {cases[i].groups}g, {cases[i].spacing} cm): "
+                 f"|dk| = {dk[-1]:.2e}")
     factory.close()
     bat = n_cases / t_batch
     ser = len(sub) / t_serial
     return {
         "workload": f"{n_cases} branch cases: 1/4 single pin, 1/2 3x3, 1/4 17x17 (25 guide tubes); "
-                    "2/7 groups alternating; CY 16x3; spacing from {0.02, 0.035, 0.05} cm; XS x U[0.9,1.1]",
+                    "2/7 groups alternating; CY 16x3; spacing from {0.02, 0.035, 0.05} cm; XS x U[0.9,1.1]; "
+                    "band width and lane map autotuned per geometry",
         "batched_solves_per_s": bat, "serial_solves_per_s": ser, "speedup": bat / ser,
         "serial_sample": len(sub), "converged": int(n_conv), "outer_iterations_max": int(max(iters)),
         "batch_s": t_batch, "setup_s": t_setup,
         "keff_max_abs_err_vs_fp64": max(dk) if dk else None, "keff_checked": len(dk),
+        "keff_checked_kinds": sorted({cases[i].kind for i in sample}),
+        "autotuned": tuned,
         "keff_max_abs_diff_batch_vs_serial": max(abs(k_batch[i] - k_serial[i]) for i in sub),
         "solver": (f"{'CMFD, ' if use_cmfd else 'no CMFD, '}{n_inner} inner iteration(s) per outer, "
                    "Jacobi in energy, Gauss-Seidel boundary update, k_tol 1e-7, psi_tol 1e-6"),
@@ -156,7 +186,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
 
 
 def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, use_cmfd: bool = True,
-                  n_serial: int = 64):
+                  n_serial: int = 64, n_oracle: int = 8):
     """Poisson arrivals of branch-case requests served by the continuous-
     batching loop at the given fractions of its burst (saturation)
     throughput, beside the same requests solved one at a time (CMFD with 5
@@ -165,7 +195,7 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     from mocc_amd.branch import BranchFactory, c5_request
     from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
     factory = BranchFactory(os.path.join(work_dir, "serve_cmfd" if use_cmfd else "serve"), n_inner=5 if use_cmfd else 1,
-                            use_cmfd=use_cmfd)
+                            use_cmfd=use_cmfd, autotune=True)
     rng = np.random.default_rng(seed)
     cases = [c5_request(rng, i) for i in range(n_req)]
     for c in cases:  # trace every geometry before timing
@@ -177,12 +207,28 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
                        + ("CMFD, 5 inner iterations" if use_cmfd else "no CMFD, 1 inner iteration"),
            "burst": burst}
     progress(f"serving burst: {burst['solves_per_s']:.3g} solves/s")
+    last = None
     for frac in rates:
         rate = frac * burst["solves_per_s"]
-        out[f"poisson_{frac:g}x"] = {"rate_per_s": rate,
-                                     **summarize(srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases))}
+        last = srv.serve(poisson_arrivals(n_req, rate, seed + 1), cases)
+        out[f"poisson_{frac:g}x"] = {"rate_per_s": rate, **summarize(last)}
         progress(f"serving at {frac:g}x burst done")
     srv.close()
+    if n_oracle > 0 and last is not None:
+        # served keffs against the FP64 oracle (same iteration scheme), one
+        # request of each lattice kind and group count met
+        seen, dk = set(), []
+        for r in last:
+            key = (r.case.kind, r.case.groups)
+            if key in seen:
+                continue
+            seen.add(key)
+            dk.append(abs(r.k - oracle_k(factory, r.case)))
+            progress(f"served request {r.index} ({r.case.kind}, {r.case.groups}g): |dk| vs FP64 = {dk[-1]:.2e}")
+            if len(dk) >= n_oracle:
+                break
+        out["keff_max_abs_err_vs_fp64"] = max(dk)
+        out["keff_checked"] = len(dk)
     n_ser = min(n_req, n_serial)
     out["serial_burst"] = {"sample": n_ser, **summarize(serve_serial(factory, np.zeros(n_ser), cases[:n_ser]))}
     factory.close()
@@ -215,14 +261,23 @@ def bench_core51(work_dir: str, with_oracle: bool):
            "last_cmfd_bicgstab_iterations": dev.cmfd_bicgstab_iterations(),
            "k_tol": par.k_tol, "psi_tol": par.psi_tol}
     dev.close()
+    # the FP64 oracle's solve of the same deck, stored with the tests (it
+    # takes many minutes on host cores): tests/golden/make_core51_oracle.py
+    ref_path = os.path.join(ROOT, "tests", "golden", "core51_oracle.npz")
+    if os.path.exists(ref_path):
+        import numpy as np
+        ref = np.load(ref_path)
+        out["oracle"] = {"k": float(ref["k"]), "outer_iterations": int(ref["iterations"]),
+                         "keff_abs_err": abs(st.k - float(ref["k"])),
+                         "source": "tests/golden/core51_oracle.npz (FP64 oracle, same deck and iteration scheme)"}
     if with_oracle:
         from oracle.binding import Oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
         o = Oracle(hp, n_inner=par.n_inner, use_cmfd=True, n_threads=threads)
         t0 = time.perf_counter()
         rc_o, st_o, _ = o.eigen_solve(par.k_tol, par.psi_tol, par.max_iter)
-        out["oracle"] = {"k": st_o.k, "outer_iterations": st_o.iterations, "solve_s": time.perf_counter() - t0,
-                         "cores": threads, "keff_abs_err": abs(st.k - st_o.k)}
+        out["oracle_live"] = {"k": st_o.k, "outer_iterations": st_o.iterations, "solve_s": time.perf_counter() - t0,
+                              "cores": threads, "keff_abs_err": abs(st.k - st_o.k)}
     return out
 
 
@@ -246,17 +301,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-sweeps", type=int, default=5)
+    ap.add_argument("--cpu-sweeps", type=int, default=20)
+    ap.add_argument("--cpu-sweeps-1t", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched branch-case section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--serial-cases", type=int, default=32)
-    ap.add_argument("--serve-requests", type=int, default=256)
-    ap.add_argument("--serve-rates", default="0.25,0.5,0.75,1.0,1.25",
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rates", default="0.5,1.0",
                     help="comma-separated arrival rates as fractions of the burst throughput")
-    ap.add_argument("--core51", action="store_true", help="add the full eigen solve of the 51x51-pin core (3x3 assemblies, 7 groups) with CMFD")
-    ap.add_argument("--core51-oracle", action="store_true", help="with --core51: also the FP64 oracle solve")
+    ap.add_argument("--no-core51", action="store_true",
+                    help="skip the full eigen solve of the 51x51-pin core (3x3 assemblies, 7 groups, CMFD)")
+    ap.add_argument("--core51-oracle", action="store_true",
+                    help="also run the FP64 oracle's solve of the 51x51-pin core live (many minutes)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -325,20 +383,20 @@ def main():
     }
     progress(f"17x17 7-group sweep: {value:.4g} updates/s")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(hp, args.cpu_sweeps)
+        line["cpu_baseline"] = cpu_baseline(hp, args.cpu_sweeps, args.cpu_sweeps_1t)
         progress("CPU baseline done")
     dev.close()
     if not args.no_batch:
-        line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 6)
+        line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 8)
         progress(f"batched (CMFD): {line['batched']['batched_solves_per_s']:.4g} solves/s")
         line["batched_no_cmfd"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases,
                                                 0 if args.no_cpu else 2, use_cmfd=False)
         progress(f"batched (no CMFD): {line['batched_no_cmfd']['batched_solves_per_s']:.4g} solves/s")
     if not args.no_serve:
         rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
-        line["serving"] = bench_serving(deck_dir, args.serve_requests, rates)
+        line["serving"] = bench_serving(deck_dir, args.serve_requests, rates, n_oracle=0 if args.no_cpu else 8)
         progress("serving done")
-    if args.core51:
+    if not args.no_core51:
         line["core51"] = bench_core51(deck_dir, args.core51_oracle)
         progress("51x51-pin core solve done")
     print(json.dumps(line), flush=True)

@@ -53,6 +53,11 @@ const char *mocc_h5_last_error(void);
 /* Write the output file of the problem h (truncating an existing file). */
 int mocc_h5_write(const mocc_host_problem *h, const char *path, const mocc_output *out);
 
+/* Add a scalar variable-length string dataset to an existing file, as the
+ * driver adds "input_file" (the deck with its amendments) and "git_sha1"
+ * (src/driver.cpp:59-68). */
+int mocc_h5_write_string(const char *path, const char *name, const char *value);
+
 /* Read a numeric dataset (converted to double) into out[0 .. cap). dims
  * receives up to max_dims extents and ndim the rank. Returns the element
  * count. */

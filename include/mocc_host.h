@@ -103,6 +103,27 @@ int mocc_host_pin_fuel(const mocc_host_problem *h, int32_t *out, int32_t cap);
  * as Mesh::coarse_surf numbers them: out[6 * cell + s]. Returns the cell count. */
 int mocc_host_coarse_surfaces(const mocc_host_problem *h, int32_t *out, int32_t cap);
 
+/* --- mocc's command line (InputProcessor, src/input_proc.cpp:51-162) ------- */
+
+/* Apply n command-line amendments "path/to/name=value" (mocc -a,
+ * input_proc.cpp:127-146, 217-272) to the deck text xml and write the
+ * amended deck as XML text into out (NUL-terminated, truncated to cap).
+ * Returns the full text length, or < 0 with mocc's message in
+ * mocc_host_last_error() for a malformed amendment or a missing target. */
+int32_t mocc_host_amend_xml(const char *xml, int32_t n, const char *const *amendments, char *out,
+                            int32_t cap);
+
+/* The deck's <case_name> text into out: returns its length, 0 when the deck
+ * has none (mocc then names the case after the input file without its
+ * extension), < 0 when the element is present but empty
+ * (input_proc.cpp:149-155). */
+int32_t mocc_host_case_name(const char *xml, char *out, int32_t cap);
+
+/* <solver><dump_iterations> of the deck, sorted (eigen_solver.cpp:84-94):
+ * the outer iterations after which mocc writes <case>_iter_<n>.h5. Returns
+ * the count (entries beyond cap are not written). */
+int32_t mocc_host_dump_iterations(const mocc_host_problem *h, int32_t *out, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

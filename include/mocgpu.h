@@ -137,6 +137,9 @@ typedef struct mocgpu_options {
     int32_t cmfd_solver;    /* 0 BiCGSTAB (Eigen default), 1 GMRES         */
     int32_t gs_boundary;    /* <sweeper boundary_update>: 1 Gauss-Seidel in angle
                                (mocc's default, moc_sweeper.cpp:105-115), 0 Jacobi */
+    int32_t cmfd_negative_fixup; /* <cmfd negative_fixup>, default 0: clamp negative
+                               coarse fluxes to 0 before each CMFD solve (cmfd.cpp:152-154,
+                               175-184) */
 } mocgpu_options;
 
 typedef struct mocgpu_handle mocgpu_handle;
@@ -199,8 +202,10 @@ int mocgpu_build_source(mocgpu_handle *h, int32_t g);
 
 /* MoCSweeper::sweep(g) (moc_sweeper.cpp:189-225): n_inner x {self_scatter;
  * sweep1g}; on the last inner, if with_current, the moc::Current tallies
- * (moc_current_worker.hpp:137-330). The boundary update is Jacobi in angle
- * (the reference's boundary_update="jacobi", moc_sweeper_kernel.inc.hpp:146-150). */
+ * (moc_current_worker.hpp:137-330). The boundary update follows
+ * options.gs_boundary: Gauss-Seidel in angle by default (mocc's default,
+ * moc_sweeper_kernel.inc.hpp:137-142), Jacobi in angle with gs_boundary = 0
+ * (boundary_update="jacobi", moc_sweeper_kernel.inc.hpp:146-150). */
 int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current);
 
 /* One inner sweep of every group at once (sweep1g for all groups, Jacobi in
@@ -222,7 +227,12 @@ int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out);
 int mocgpu_flux_residual(mocgpu_handle *h, double *out);
 
 /* MoCSweeper::get_pin_flux_1g / set_pin_flux_1g with MeshTreatment::PIN_PLANE
- * (moc_sweeper.cpp:293-433). pin arrays are [nx*ny] in coarse-cell order. */
+ * (moc_sweeper.cpp:293-433). pin arrays are [nx*ny] in coarse-cell order.
+ * The problems here have one plane, so MeshTreatment::PIN (one value per pin
+ * and core plane, moc_sweeper.cpp:328-357, 382-392) is the same nx*ny array;
+ * the other treatments (TRUE, PLANE) are rejected by the reference too
+ * ("Unsupported mesh treatment", moc_sweeper.cpp:359-360, 428-430) and have no
+ * entry point. g outside [0, n_group) returns MOCGPU_ERR_ARG. */
 int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin);
 int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin,
                            double *resid);
@@ -283,6 +293,12 @@ int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol,
 int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol,
                                int32_t max_iter, int32_t min_iter,
                                mocgpu_eigen_state *state, double *history, int32_t cap);
+/* The start of EigenSolver::solve (eigen_solver.cpp:131-160): the sweeper's
+ * initial state (mocgpu_initialize), k = 1 on the device and in *state, the
+ * iteration counters cleared. mocgpu_eigen_iterate then advances the solve
+ * one outer iteration at a time (the caller applies solve's convergence
+ * test: err_k < k_tol, err_psi < psi_tol, iterations > min_iter). */
+int mocgpu_eigen_begin(mocgpu_handle *h, mocgpu_eigen_state *state);
 /* One outer iteration (EigenSolver::step + the convergence bookkeeping of solve). */
 int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
 
@@ -311,7 +327,8 @@ const char *mocgpu_batch_last_error(const mocgpu_batch *b);
 
 /* Admit a problem: resets it to the start of EigenSolver::solve (flat flux,
  * k = 1) and returns its id. MOCGPU_ERR_CAPACITY when the budget would be
- * exceeded (nothing changes). */
+ * exceeded (nothing changes); MOCGPU_ERR_UNSUPPORTED for a handle created
+ * with energy_mode != 1 (a batch sweeps Jacobi in energy only). */
 int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi_tol,
                      int32_t max_iter, int32_t min_iter, int32_t *id);
 
@@ -360,6 +377,7 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n_sweeps, double *ms_per_sweep,
  * table lookup of moc_sweeper_kernel.inc.hpp:94-99 for testing. Returns 0 or
  * MOCGPU_ERR_DEVICE. */
 int32_t mocgpu_attenuation(const float *tau, float *out, int32_t n);
+
 
 #ifdef __cplusplus
 }

@@ -54,14 +54,20 @@ class BranchFactory:
 
     BASE_SEED = 4
 
-    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1, use_cmfd: bool = False):
+    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1, use_cmfd: bool = False,
+                 autotune: bool = False):
         """use_cmfd: CMFD-accelerated solves (mocc's CMFD decks use 5 inner
         iterations per outer; with 1 the coarse system can lose its
-        diagonal dominance)."""
+        diagonal dominance). autotune: each geometry's prototype handle picks
+        its band width and lane map by timing the candidates on the device
+        (mocgpu_create with lane_map = -1); every case of that geometry then
+        shares the tuned plan, and a batch whose cases all tuned to bands of
+        at most 32 rays runs one-wave queue workgroups."""
         self.work_dir = work_dir
         self.energy_mode = energy_mode
         self.n_inner = n_inner
         self.use_cmfd = use_cmfd
+        self.autotune = autotune
         self._hp: dict = {}
         self._proto: dict = {}
         self._lock = threading.Lock()
@@ -107,10 +113,18 @@ class BranchFactory:
             proto = self._proto.get(gkey)
             if proto is None:
                 proto = GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode,
-                                      use_cmfd=self.use_cmfd)
+                                      use_cmfd=self.use_cmfd, lane_map=-1 if self.autotune else None)
                 self._proto[gkey] = proto
             return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=self.use_cmfd,
                                  like=proto)
+
+    def tuned(self) -> dict:
+        """(lattice, spacing) -> (band rays, lane map) of each geometry's plan."""
+        out = {}
+        for k, p in self._proto.items():
+            o = p.options()
+            out[k] = (int(o.band_rays), int(o.lane_map))
+        return out
 
     def close(self):
         for p in self._proto.values():

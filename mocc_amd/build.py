@@ -4,6 +4,7 @@
   * libmocc_host.so  -- C++17 host front-end (g++),
   * libmocgpu.so     -- HIP for gfx950 (hipcc),
   * libmocc_h5.so    -- mocc's HDF5 output file (g++, the HDF5 C library),
+  * mocc             -- the command-line program (g++),
   * oracle/_build/liboracle.so -- the FP64 checker (test infrastructure).
 """
 from __future__ import annotations
@@ -92,15 +93,37 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+def _git_sha1() -> str:
+    try:
+        r = subprocess.run(["git", "-C", ROOT, "rev-parse", "HEAD"], capture_output=True, text=True)
+        return r.stdout.strip() if r.returncode == 0 and r.stdout.strip() else "unknown"
+    except OSError:
+        return "unknown"
+
+
+def build_cli():
+    """The `mocc` command-line program (csrc/cli/mocc.cpp), next to the
+    libraries it links (libmocgpu, libmocc_host; libmocc_h5 is loaded at run
+    time after libhdf5)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "mocc")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f'-DMOCC_GIT_SHA1="{_git_sha1()}"',
+          os.path.join(PKG, "csrc", "cli", "mocc.cpp"), "-o", out + ".tmp",
+          "-L" + LIB, "-lmocgpu", "-lmocc_host", "-ldl", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+
+
 def build_all():
     build_host()
     build_gpu()
     build_h5()
+    build_cli()
     build_oracle()
 
 
 if __name__ == "__main__":
     which = sys.argv[1:] or ["all"]
     for w in which:
-        {"host": build_host, "gpu": build_gpu, "h5": build_h5, "oracle": build_oracle, "all": build_all}[w]()
+        {"host": build_host, "gpu": build_gpu, "h5": build_h5, "cli": build_cli, "oracle": build_oracle,
+         "all": build_all}[w]()
     print("built:", ", ".join(which))

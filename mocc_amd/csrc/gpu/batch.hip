@@ -213,6 +213,7 @@ struct mocgpu_batch {
     DBuf<CurrentArgs> d_cura;
     DBuf<uint32_t> d_counter;
     int n_bmF = 0, n_bmR = 0, n_bmJ = 0, n_items = 0, max_inner = 1, q_blocks = 1, q_blocks_cur = 1;
+    int wg_rays = kRays;  // rays per direction of a queue workgroup (32 or 64)
     int phase_first[2] = {0, 0}, phase_items[2] = {0, 0};
     size_t lds = 0, lds_cur = 0;
     // CMFD problems: their CMFD step runs first in every outer iteration, and
@@ -299,7 +300,7 @@ struct mocgpu_batch {
                 lds_cur = std::max(lds_cur, sweep_lds_bytes(a));
                 ja[i] = h->current_args(-1);
                 ja[i].radial_flag = h->d_radial.p;
-                cJ[i] = (size_t)(ja[i].n1 - ja[i].n0) * h->G;
+                cJ[i] = (size_t)(ja[i].n1 - ja[i].n0) * h->G * reduce_current_lanes();
                 cm.push_back(CmfdBatchEntry{i, h->cmfd.get(), h->cmfd_view(), h->d_radial.p});
             } else {
                 a.lds_lsurf = 0;
@@ -363,8 +364,14 @@ struct mocgpu_batch {
         int dev = 0, ncu = 0;
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        q_blocks = std::max(1, ncu * queue_blocks_per_cu(NP, lds, false));
-        q_blocks_cur = cm.empty() ? q_blocks : std::max(1, ncu * queue_blocks_per_cu(NP, lds_cur, true));
+        // one-wave workgroups when every problem's (possibly autotuned) band
+        // plan holds at most 32 rays per band; a band's arithmetic is the
+        // same in either width
+        wg_rays = 32;
+        for (int i = 0; i < n; i++)
+            if (cases[i].h->plan->max_band_rays > 32) wg_rays = kRays;
+        q_blocks = std::max(1, ncu * queue_blocks_per_cu(NP, lds, false, wg_rays));
+        q_blocks_cur = cm.empty() ? q_blocks : std::max(1, ncu * queue_blocks_per_cu(NP, lds_cur, true, wg_rays));
         sync();
         dirty = false;
     }
@@ -392,7 +399,7 @@ struct mocgpu_batch {
                 q.n_inner = d_ninner.p;
                 q.want_cur = d_wantcur.p;
                 launch_sweep_queue(q, NP, cur ? lds_cur : lds, std::min(cur ? q_blocks_cur : q_blocks, phase_items[ph]),
-                                   s, cur);
+                                   s, cur, wg_rays);
             }
             launch_reduce_flux_batch(d_red.p, d_bmR.p, n_bmR, d_status.p, d_ninner.p, inner, s);
             if (cur) launch_reduce_current_batch(d_cura.p, d_bmJ.p, n_bmJ, d_status.p, d_ninner.p, inner, s);
@@ -475,6 +482,14 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
 {
     if (!b || !h || !id) return MOCGPU_ERR_ARG;
     return bguard(b, [&] {
+        // The batch sweeps every group of a problem in one launch from the
+        // previous outer iteration's flux: Jacobi in energy only. A handle in
+        // mocc's Gauss-Seidel energy mode (energy_mode 0) would follow another
+        // iteration path than its own mocgpu_eigen_solve.
+        if (h->opt.energy_mode != 1) {
+            b->err = "batched problems sweep Jacobi in energy: create the handle with energy_mode = 1";
+            return (int)MOCGPU_ERR_UNSUPPORTED;
+        }
         if (!b->cases.empty() && h->NP != b->NP) {
             b->err = "all problems of a batch must have the same number of polar angles";
             return (int)MOCGPU_ERR_UNSUPPORTED;

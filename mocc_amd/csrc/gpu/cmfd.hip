@@ -138,7 +138,8 @@ __device__ __forceinline__ double thickness(const Grid &m, int cell, int normal)
 // --- set-up kernels --------------------------------------------------------------
 
 // MoCSweeper::get_pin_flux_1g (PIN_PLANE, moc_sweeper.cpp:293-326) for all
-// groups: item t = (pin, group).
+// groups: item t = (pin, group). With negative_fixup the coarse flux is
+// clamped at 0, as CMFD::solve does before homogenizing (cmfd.cpp:175-184).
 __device__ __forceinline__ void pin_flux_item(const CmfdView &v, double *cflux, int t)
 {
     if (t >= v.npin * v.G) return;
@@ -148,7 +149,8 @@ __device__ __forceinline__ void pin_flux_item(const CmfdView &v, double *cflux, 
         vs += v.vol[r];
         pf += v.phi[(size_t)r * v.G + g] * v.vol[r];
     }
-    cflux[(size_t)v.pin_cell[ip] * v.G + g] = pf / vs;
+    const double c = pf / vs;
+    cflux[(size_t)v.pin_cell[ip] * v.G + g] = v.neg_fixup ? fmax(c, 0.0) : c;
 }
 
 __global__ void k_pin_flux_all(CmfdView v, double *cflux)

@@ -21,6 +21,7 @@ struct CmfdView {
     double *phi, *cur, *sfl;
     double *k;  // device scalar, in/out
     int has_radial;
+    int neg_fixup;  // clamp negative coarse fluxes to 0 before the solve (<cmfd negative_fixup>)
 };
 
 class CmfdSolver {

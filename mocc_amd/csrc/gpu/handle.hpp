@@ -314,6 +314,7 @@ struct mocgpu_handle {
         v.sfl = d_sfl.p;
         v.k = d_scal.p + S_K;
         v.has_radial = has_radial ? 1 : 0;
+        v.neg_fixup = opt.cmfd_negative_fixup ? 1 : 0;
         return v;
     }
 

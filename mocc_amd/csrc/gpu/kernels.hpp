@@ -94,9 +94,11 @@ struct CurrentArgs {
 };
 
 void launch_sweep(const SweepArgs &a, int n_polar, bool with_current, int n_items, hipStream_t s);
-// Workgroups resident per CU for the queue kernel with lds bytes of LDS.
-int queue_blocks_per_cu(int n_polar, size_t lds, bool cur);
-void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur);
+// Workgroups resident per CU for the queue kernel with lds bytes of LDS and
+// wg_rays rays per direction (32: one-wave workgroups, else 64).
+int queue_blocks_per_cu(int n_polar, size_t lds, bool cur, int wg_rays);
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur,
+                        int wg_rays);
 void launch_reduce_current(const CurrentArgs &a, size_t n_items, hipStream_t s);
 void launch_reduce_current_batch(const CurrentArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
                                  const int32_t *n_inner, int inner, hipStream_t s);
@@ -110,6 +112,8 @@ void launch_attenuation(const float *tau, float *out, int n, hipStream_t s);
 
 // Lanes per (FSR, group) item of the flux reduction.
 int reduce_flux_lanes();
+// Lanes per (surface, group) item of the current reduction.
+int reduce_current_lanes();
 void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n_blocks, const int32_t *status,
                               const int32_t *n_inner, int inner, hipStream_t s);
 

@@ -257,6 +257,7 @@ void mocgpu_handle::copy_flux_to_old()
 namespace {
 template <typename F> int guard(mocgpu_handle *h, F &&f)
 {
+    if (!h) return MOCGPU_ERR_ARG;
     try {
         int rc = f();
         return rc;
@@ -270,6 +271,7 @@ template <typename F> int guard(mocgpu_handle *h, F &&f)
 }
 
 thread_local std::string g_create_err;
+
 }  // namespace
 
 extern "C" {
@@ -288,6 +290,7 @@ void mocgpu_default_options(mocgpu_options *o)
     o->cmfd_resid_reduction = 0.001;
     o->cmfd_solver = 0;
     o->gs_boundary = 1;
+    o->cmfd_negative_fixup = 0;
 }
 
 const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
@@ -612,6 +615,14 @@ int mocgpu_initialize(mocgpu_handle *h)
                 HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + b) * h->plan->bcpg, h->bc_template.data(),
                                          h->bc_template.size() * 4, hipMemcpyHostToDevice, h->stream));
         h->parity = 0;
+        // no coarse-surface currents until a sweep tallies them: the first
+        // CMFD solve after initialize runs with D-hat = 0, as after
+        // mocgpu_batch_add
+        h->has_radial = false;
+        if (h->d_cur.p) HIP_CHECK(hipMemsetAsync(h->d_cur.p, 0, h->d_cur.bytes(), h->stream));
+        if (h->d_sfl.p) HIP_CHECK(hipMemsetAsync(h->d_sfl.p, 0, h->d_sfl.bytes(), h->stream));
+        const int32_t no_radial = 0;
+        if (h->d_radial.p) HIP_CHECK(hipMemcpyAsync(h->d_radial.p, &no_radial, 4, hipMemcpyHostToDevice, h->stream));
         h->cmfd->initialize(h->stream);
         h->sync();
         return 0;
@@ -774,6 +785,8 @@ int mocgpu_flux_residual(mocgpu_handle *h, double *out)
 int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
 {
     return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        if (!pin) throw std::runtime_error("null pin flux array");
         hipLaunchKernelGGL(k_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
                            h->d_pin_off.p, h->d_pin_cell.p, h->d_vol.p, h->d_phi.p, h->d_pin.p);
         HIP_CHECK(hipMemcpyAsync(pin, h->d_pin.p, sizeof(double) * h->npin, hipMemcpyDeviceToHost, h->stream));
@@ -784,12 +797,15 @@ int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
 
 int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin, double *resid)
 {
-    for (int i = 0; i < h->npin; i++)
-        if (!(pin[i] > 0.0)) {
-            h->err = "Negative or zero input flux: " + std::to_string(pin[i]);
-            return MOCGPU_ERR_NONPOS_FLUX;
-        }
     return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        if (!pin) throw std::runtime_error("null pin flux array");
+        // moc_sweeper.cpp:367-375 throws on a non-positive input flux
+        for (int i = 0; i < h->npin; i++)
+            if (!(pin[i] > 0.0)) {
+                h->err = "Negative or zero input flux: " + std::to_string(pin[i]);
+                return (int)MOCGPU_ERR_NONPOS_FLUX;
+            }
         HIP_CHECK(hipMemcpyAsync(h->d_pin.p, pin, sizeof(double) * h->npin, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_set_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
                            h->d_pin_off.p, h->d_pin_cell.p, h->d_area.p, h->d_pin_area.p, h->d_pin.p, h->d_phi.p,
@@ -809,6 +825,7 @@ int32_t mocgpu_n_surf(const mocgpu_handle *h) { return h->n_surf; }
 int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *surface_flux)
 {
     return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         std::vector<double> c((size_t)h->n_surf * h->G), s((size_t)h->n_surf * h->G);
         HIP_CHECK(hipMemcpyAsync(c.data(), h->d_cur.p, c.size() * 8, hipMemcpyDeviceToHost, h->stream));
         HIP_CHECK(hipMemcpyAsync(s.data(), h->d_sfl.p, s.size() * 8, hipMemcpyDeviceToHost, h->stream));
@@ -939,17 +956,17 @@ int mocgpu_eigen_solve(mocgpu_handle *h, double k_tol, double psi_tol, int32_t m
     return mocgpu_eigen_solve_history(h, k_tol, psi_tol, max_iter, min_iter, st, nullptr, 0);
 }
 
-int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter,
-                               int32_t min_iter, mocgpu_eigen_state *st, double *history, int32_t cap)
+// The start of EigenSolver::solve (eigen_solver.cpp:131-160): the sweeper's
+// initial state, k = 1, the iteration counters cleared.
+int mocgpu_eigen_begin(mocgpu_handle *h, mocgpu_eigen_state *st)
 {
-    if (!st || (cap > 0 && !history)) return MOCGPU_ERR_ARG;
-    const auto t0 = std::chrono::steady_clock::now();
+    if (!st) return MOCGPU_ERR_ARG;
     int rc = mocgpu_initialize(h);
     if (rc) return rc;
     *st = mocgpu_eigen_state{};
     st->k = 1.0;
     st->k_prev = 1.0;
-    rc = guard(h, [&] {
+    return guard(h, [&] {
         std::vector<double> sc(S_COUNT, 0.0);
         sc[S_K] = 1.0;
         sc[S_KPREV] = 1.0;
@@ -958,6 +975,14 @@ int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol, i
         h->sync();
         return 0;
     });
+}
+
+int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol, int32_t max_iter,
+                               int32_t min_iter, mocgpu_eigen_state *st, double *history, int32_t cap)
+{
+    if (!st || (cap > 0 && !history)) return MOCGPU_ERR_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = mocgpu_eigen_begin(h, st);
     if (rc) return rc;
     for (int n = 0; n < max_iter; n++) {
         rc = mocgpu_eigen_iterate(h, st);

@@ -392,9 +392,11 @@ __global__ __launch_bounds__(2 * W) void k_sweep(SweepArgs A)
 // item index with a (vector) global atomic and shares it through LDS; every
 // workgroup leaves once the counter passes n_items, so the grid always drains.
 // The CUR instantiation tallies coarse-surface currents for the problems
-// whose want_cur is set (CMFD problems on their last inner iteration).
-template <int NP, bool CUR>
-__global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArgs *__restrict__ cases)
+// whose want_cur is set (CMFD problems on their last inner iteration). W rays
+// per direction: 2W lanes per workgroup, one wave when every problem's bands
+// hold at most 32 rays.
+template <int NP, bool CUR, int W>
+__global__ __launch_bounds__(2 * W) void k_sweep_queue(QueueArgs Q, const SweepArgs *__restrict__ cases)
 {
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned int s_bound;
@@ -410,8 +412,8 @@ __global__ __launch_bounds__(kWG) void k_sweep_queue(QueueArgs Q, const SweepArg
         if (Q.status[2 * c + 1] != 0 || Q.n_inner[c] <= Q.inner) continue;
         SweepArgs A = cases[c];
         if (!CUR || !(Q.want_cur && Q.want_cur[c] && Q.inner == Q.n_inner[c] - 1)) A.lds_lsurf = 0;
-        sweep_band<NP, CUR, kRays>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
-                                   smem, s_bound);
+        sweep_band<NP, CUR, W>(A, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z), smem,
+                               s_bound);
         __syncthreads();
     }
 }
@@ -457,37 +459,16 @@ __device__ __forceinline__ void reduce_flux_item(const ReduceArgs &A, int item, 
 // moc_current_worker.hpp:272-318). Surfaces no ray crosses end up zero, as
 // after CoarseData::zero_data_radial. Item t covers (surface n0 + t / G,
 // group t % G), or surface n0 + t of group g_fixed.
-__device__ __forceinline__ void reduce_current_item(const CurrentArgs &A, int t)
-{
-    int s, g;
-    if (A.g_fixed >= 0) {
-        s = t;
-        g = A.g_fixed;
-    } else {
-        s = t / A.G;
-        g = t % A.G;
-    }
-    if (s >= A.n1 - A.n0) return;
-    s += A.n0;
-    double c = 0.0, f = 0.0;
-    for (int k = A.ptr[s]; k < A.ptr[s + 1]; k++) {
-        c += A.csum[(size_t)g * A.n_lslots + A.idx[k]];
-        f += A.ssum[(size_t)g * A.n_lslots + A.idx[k]];
-    }
-    A.cur[(size_t)s * A.G + g] = c / A.area[s];
-    A.sfl[(size_t)s * A.G + g] = f / A.area[s];
-}
-
-// The one-problem launch gives each surface kCurLanes lanes: they take the
-// slots k = first + lane, first + lane + kCurLanes, ... and an xor butterfly
-// adds their sums, an order fixed by the slot list. A 51x51-pin mesh has
-// only ~7900 surfaces per group, too few to fill the chip one lane each.
+//
+// Each surface gets kCurLanes lanes: they take the slots k = first + lane,
+// first + lane + kCurLanes, ... and an xor butterfly adds their sums, an
+// order fixed by the slot list, so the one-problem and the batched launches
+// add in the same order. A 51x51-pin mesh has only ~7900 surfaces per group,
+// too few to fill the chip one lane each.
 constexpr int kCurLanes = 16;
 
-__global__ void k_reduce_current(CurrentArgs A)
+__device__ __forceinline__ void reduce_current_item(const CurrentArgs &A, int t, int lane)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = gid / kCurLanes, lane = gid % kCurLanes;
     int s, g;
     if (A.g_fixed >= 0) {
         s = t;
@@ -514,9 +495,16 @@ __global__ void k_reduce_current(CurrentArgs A)
     A.sfl[(size_t)s * A.G + g] = f / A.area[s];
 }
 
+__global__ void k_reduce_current(CurrentArgs A)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    reduce_current_item(A, gid / kCurLanes, gid % kCurLanes);
+}
+
 // The same over the problems of a batch whose last inner iteration this is
-// and that tally currents (CMFD); the first block of a problem raises its
-// flag that radial currents exist.
+// and that tally currents (CMFD): block b covers elements [first, first + 256)
+// of problem bm[b].c, kCurLanes elements per (surface, group) item. The first
+// block of a problem raises its flag that radial currents exist.
 __global__ void k_reduce_current_batch(const CurrentArgs *__restrict__ cases, const BlockMap *__restrict__ bm,
                                        const int32_t *__restrict__ status, const int32_t *__restrict__ n_inner,
                                        int inner)
@@ -527,7 +515,8 @@ __global__ void k_reduce_current_batch(const CurrentArgs *__restrict__ cases, co
     const CurrentArgs A = cases[c];
     const int first = __builtin_amdgcn_readfirstlane(b.first);
     if (first == 0 && threadIdx.x == 0 && A.radial_flag) *A.radial_flag = 1;
-    reduce_current_item(A, first + threadIdx.x);
+    const int gid = first + threadIdx.x;
+    reduce_current_item(A, gid / kCurLanes, gid % kCurLanes);
 }
 
 __global__ void k_reduce_flux(ReduceArgs A)
@@ -598,14 +587,20 @@ void launch_sweep(const SweepArgs &a, int n_polar, bool cur, int n_items, hipStr
 #undef MOCGPU_SWEEP_CASE
 }
 
-int queue_blocks_per_cu(int n_polar, size_t lds, bool cur)
+int queue_blocks_per_cu(int n_polar, size_t lds, bool cur, int wg_rays)
 {
     int nb = 0;
     hipError_t e = hipSuccess;
-#define MOCGPU_QOCC(NPV)                                                                                     \
-    case NPV:                                                                                                \
-        e = cur ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, true>, kWG, lds)      \
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, false>, kWG, lds);    \
+    const bool narrow = wg_rays == 32;
+#define MOCGPU_QOCC(NPV)                                                                                         \
+    case NPV:                                                                                                    \
+        if (narrow)                                                                                              \
+            e = cur ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, true, 32>, 64, lds)   \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, false, 32>, 64, lds); \
+        else                                                                                                     \
+            e = cur ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, true, kRays>, kWG, lds) \
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_queue<NPV, false, kRays>, kWG,   \
+                                                                   lds);                                         \
         break;
     switch (n_polar) {
         MOCGPU_QOCC(1)
@@ -618,13 +613,20 @@ int queue_blocks_per_cu(int n_polar, size_t lds, bool cur)
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur)
+void launch_sweep_queue(const QueueArgs &q, int n_polar, size_t lds, int n_blocks, hipStream_t s, bool cur,
+                        int wg_rays)
 {
-    dim3 grid(n_blocks), block(kWG);
-#define MOCGPU_QLAUNCH(NPV)                                                                             \
-    case NPV:                                                                                           \
-        if (cur) hipLaunchKernelGGL((k_sweep_queue<NPV, true>), grid, block, lds, s, q, q.cases);        \
-        else hipLaunchKernelGGL((k_sweep_queue<NPV, false>), grid, block, lds, s, q, q.cases);           \
+    const bool narrow = wg_rays == 32;
+    dim3 grid(n_blocks), block(narrow ? 64 : kWG);
+#define MOCGPU_QLAUNCH(NPV)                                                                                  \
+    case NPV:                                                                                                \
+        if (narrow) {                                                                                        \
+            if (cur) hipLaunchKernelGGL((k_sweep_queue<NPV, true, 32>), grid, block, lds, s, q, q.cases);    \
+            else hipLaunchKernelGGL((k_sweep_queue<NPV, false, 32>), grid, block, lds, s, q, q.cases);       \
+        } else {                                                                                             \
+            if (cur) hipLaunchKernelGGL((k_sweep_queue<NPV, true, kRays>), grid, block, lds, s, q, q.cases); \
+            else hipLaunchKernelGGL((k_sweep_queue<NPV, false, kRays>), grid, block, lds, s, q, q.cases);    \
+        }                                                                                                    \
         break;
     switch (n_polar) {
         MOCGPU_QLAUNCH(1)
@@ -664,6 +666,7 @@ void launch_reduce_flux_batch(const ReduceArgs *cases, const BlockMap *bm, int n
 }
 
 int reduce_flux_lanes() { return kRedLanes; }
+int reduce_current_lanes() { return kCurLanes; }
 
 void launch_reduce_flux(const ReduceArgs &a, hipStream_t s)
 {

@@ -283,6 +283,17 @@ int mocc_h5_write(const mocc_host_problem *h, const char *path, const mocc_outpu
     });
 }
 
+int mocc_h5_write_string(const char *path, const char *name, const char *value)
+{
+    return guard([&] {
+        if (!path || !name || !value) throw H5Error("null argument");
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        Id file(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), H5Fclose, std::string("cannot open ") + path);
+        write_string(file, name, value);
+        return 0;
+    });
+}
+
 int64_t mocc_h5_read(const char *path, const char *name, double *out, int64_t cap, int64_t *dims, int32_t max_dims,
                      int32_t *ndim)
 {

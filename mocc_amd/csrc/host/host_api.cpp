@@ -1,6 +1,7 @@
 // C ABI of the host front-end (include/mocc_host.h).
 #include "../../../include/mocc_host.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -254,6 +255,51 @@ int mocc_host_coarse_surfaces(const mocc_host_problem *h, int32_t *out, int32_t 
         for (int c = 0; c < n; c++)
             for (int s = 0; s < 6; s++) out[6 * c + s] = m.coarse_surf(c, order[s]);
         return n;
+    });
+}
+
+// --- the mocc command line (src/input_proc.cpp:51-162) -----------------------
+
+int32_t mocc_host_amend_xml(const char *xml, int32_t n, const char *const *amendments, char *out, int32_t cap)
+{
+    return guard([&] {
+        if (!xml || (n > 0 && !amendments)) throw std::runtime_error("null argument");
+        auto doc = xml_parse_string(xml);
+        for (int32_t k = 0; k < n; k++) xml_apply_amendment(*doc, amendments[k]);
+        const std::string t = xml_serialize(*doc);
+        if (out && cap > 0) {
+            const size_t m = std::min<size_t>(t.size(), (size_t)cap - 1);
+            std::memcpy(out, t.data(), m);
+            out[m] = '\0';
+        }
+        return (int)t.size();
+    });
+}
+
+int32_t mocc_host_case_name(const char *xml, char *out, int32_t cap)
+{
+    return guard([&] {
+        if (!xml) throw std::runtime_error("null argument");
+        auto doc = xml_parse_string(xml);
+        const XmlNode *c = doc->child("case_name");
+        if (!c) return 0;
+        if (c->text.empty()) throw std::runtime_error("<case_name> was provided, yet empty");
+        if (out && cap > 0) {
+            const size_t m = std::min<size_t>(c->text.size(), (size_t)cap - 1);
+            std::memcpy(out, c->text.data(), m);
+            out[m] = '\0';
+        }
+        return (int)c->text.size();
+    });
+}
+
+int32_t mocc_host_dump_iterations(const mocc_host_problem *h, int32_t *out, int32_t cap)
+{
+    return guard([&] {
+        if (!h || !h->prob) throw std::runtime_error("no solver deck loaded");
+        const auto &d = h->prob->params().dump_iterations;
+        for (size_t k = 0; k < d.size() && (int32_t)k < cap; k++) out[k] = d[k];
+        return (int)d.size();
     });
 }
 

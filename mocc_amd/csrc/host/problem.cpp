@@ -1,5 +1,7 @@
 #include "problem.hpp"
 
+#include <sstream>
+
 #include <algorithm>
 #include <map>
 
@@ -19,6 +21,7 @@ void host_default_options(mocgpu_options *o)
     o->cmfd_resid_reduction = 0.001;
     o->cmfd_solver = 0;
     o->gs_boundary = 1;
+    o->cmfd_negative_fixup = 0;
 }
 
 namespace {
@@ -72,11 +75,19 @@ void Problem::build(const XmlNode &doc, const std::string &base_dir)
         params_.max_iter = solver->attr_int("max_iter", 100);
         params_.min_iter = solver->attr_int("min_iter", 0);
         params_.cmfd = solver->attr_bool("cmfd", false);
+        if (const XmlNode *d = solver->child("dump_iterations")) {
+            // explode_string<int> of the element's text, then sorted
+            std::istringstream in(d->text);
+            int v;
+            while (in >> v) params_.dump_iterations.push_back(v);
+            std::sort(params_.dump_iterations.begin(), params_.dump_iterations.end());
+        }
         if (const XmlNode *c = solver->child("cmfd")) {
             opts_.cmfd_k_tol = c->attr_double("k_tol", opts_.cmfd_k_tol);
             opts_.cmfd_psi_tol = c->attr_double("psi_tol", opts_.cmfd_psi_tol);
             opts_.cmfd_resid_reduction = c->attr_double("residual_reduction", opts_.cmfd_resid_reduction);
             opts_.cmfd_max_iter = c->attr_int("max_iter", opts_.cmfd_max_iter);
+            opts_.cmfd_negative_fixup = c->attr_bool("negative_fixup", false) ? 1 : 0;
             if (c->has_attr("enabled") && !c->attr_bool("enabled", true)) params_.cmfd = false;
             params_.dump_current = c->attr_bool("dump_current", false);
         }

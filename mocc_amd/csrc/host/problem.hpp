@@ -20,6 +20,7 @@ struct SolverParams {
     bool dump_fsr_flux = false;         // <sweeper dump_fsr_flux> (moc_sweeper.cpp:103)
     bool dump_current = false;          // <cmfd dump_current> (cmfd.cpp:161-163)
     int n_polar = 0, n_azimuthal = 0;   // <ang_quad> attributes, as AngularQuadrature::output writes them
+    std::vector<int> dump_iterations;   // <solver><dump_iterations>, sorted (eigen_solver.cpp:84-94)
 };
 
 // Reference defaults for mocgpu_options (cmfd.cpp:62-102).

@@ -63,6 +63,20 @@ bool XmlNode::attr_bool(const std::string &n, bool def) const
     return c == '1' || c == 't' || c == 'T' || c == 'y' || c == 'Y';
 }
 
+XmlNode *XmlNode::child_mut(const std::string &n)
+{
+    for (auto &c : children)
+        if (c->name == n) return c.get();
+    return nullptr;
+}
+
+std::string *XmlNode::attr_mut(const std::string &n)
+{
+    for (auto &a : attrs)
+        if (a.first == n) return &a.second;
+    return nullptr;
+}
+
 namespace {
 
 struct Parser {
@@ -162,6 +176,7 @@ struct Parser {
         }
         if (all_ws || have_text) return;
         n.text = t;
+        n.has_text = true;
         have_text = true;
     }
 
@@ -251,6 +266,84 @@ std::unique_ptr<XmlNode> xml_parse_file(const std::string &path)
     std::stringstream ss;
     ss << f.rdbuf();
     return xml_parse_string(ss.str());
+}
+
+namespace {
+std::string escape(const std::string &v, bool attr)
+{
+    std::string o;
+    o.reserve(v.size());
+    for (char c : v) {
+        if (c == '&') o += "&amp;";
+        else if (c == '<') o += "&lt;";
+        else if (c == '>') o += "&gt;";
+        else if (attr && c == '"') o += "&quot;";
+        else o += c;
+    }
+    return o;
+}
+
+void serialize(const XmlNode &n, int depth, std::string &o)
+{
+    const std::string ind(2 * depth, ' ');
+    o += ind + "<" + n.name;
+    for (const auto &a : n.attrs) o += " " + a.first + "=\"" + escape(a.second, true) + "\"";
+    if (n.children.empty() && !n.has_text) {
+        o += " />\n";
+        return;
+    }
+    o += ">";
+    if (n.children.empty()) {
+        o += escape(n.text, false) + "</" + n.name + ">\n";
+        return;
+    }
+    o += "\n";
+    if (n.has_text) o += ind + "  " + escape(n.text, false) + "\n";
+    for (const auto &c : n.children) serialize(*c, depth + 1, o);
+    o += ind + "</" + n.name + ">\n";
+}
+}  // namespace
+
+std::string xml_serialize(const XmlNode &doc)
+{
+    std::string o = "<?xml version=\"1.0\"?>\n";
+    for (const auto &c : doc.children) serialize(*c, 0, o);
+    return o;
+}
+
+void xml_apply_amendment(XmlNode &doc, const std::string &amendment)
+{
+    const size_t loc = amendment.find('=');
+    if (loc == std::string::npos)
+        throw std::runtime_error("Malformed replacement in command line. Proper syntax:\n"
+                                 "-a path/to/attribute:new_value");
+    if (amendment.rfind('=') != loc) throw std::runtime_error("Malformed replacement in command line");
+    std::string path = amendment.substr(0, loc);
+    const std::string value = amendment.substr(loc + 1);
+    std::vector<std::string> comp;
+    while (true) {
+        const size_t pos = path.find('/');
+        comp.push_back(path.substr(0, pos));
+        if (pos == std::string::npos) break;
+        path = path.substr(pos + 1);
+    }
+    XmlNode *node = &doc;
+    for (size_t k = 0; k + 1 < comp.size(); k++) {
+        node = node->child_mut(comp[k]);
+        if (!node) throw std::runtime_error("Failed to apply command-line amendment");
+    }
+    const std::string &last = comp.back();
+    std::string *a = node->attr_mut(last);
+    XmlNode *c = node->child_mut(last);
+    if ((a != nullptr) == (c != nullptr))
+        throw std::runtime_error("Could not find the requested attribute or child to modify, or could not "
+                                 "disambiguate.");
+    if (a) {
+        *a = value;
+    } else {
+        if (!c->has_text) throw std::runtime_error("No text data found at requested location.");
+        c->text = value;
+    }
 }
 
 }  // namespace mocc_amd

@@ -51,7 +51,7 @@ class MocgpuOptions(C.Structure):
         ("n_inner", i32), ("energy_mode", i32), ("lane_map", i32), ("band_rays", i32),
         ("use_cmfd", i32), ("cmfd_max_iter", i32), ("cmfd_k_tol", f64),
         ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("cmfd_solver", i32),
-        ("gs_boundary", i32),
+        ("gs_boundary", i32), ("cmfd_negative_fixup", i32),
     ]
 
 
@@ -108,6 +108,12 @@ def host() -> C.CDLL:
         L.mocc_host_group_bounds.argtypes = [P, P, i32]
         L.mocc_host_pin_fuel.argtypes = [P, P, i32]
         L.mocc_host_coarse_surfaces.argtypes = [P, P, i32]
+        L.mocc_host_amend_xml.restype = i32
+        L.mocc_host_amend_xml.argtypes = [C.c_char_p, i32, C.POINTER(C.c_char_p), C.c_char_p, i32]
+        L.mocc_host_case_name.restype = i32
+        L.mocc_host_case_name.argtypes = [C.c_char_p, C.c_char_p, i32]
+        L.mocc_host_dump_iterations.restype = i32
+        L.mocc_host_dump_iterations.argtypes = [P, P, i32]
         _host = L
     return _host
 
@@ -155,6 +161,7 @@ def gpu() -> C.CDLL:
         L.mocgpu_eigen_solve.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState)]
         L.mocgpu_eigen_solve_history.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState), P, i32]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
+        L.mocgpu_eigen_begin.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_plan_check.argtypes = [C.POINTER(MocgpuProblem), i32, C.c_char_p, i32]
         L.mocgpu_attenuation.restype = i32
         L.mocgpu_attenuation.argtypes = [P, P, i32]
@@ -212,6 +219,13 @@ class HostProblem:
         if host().mocc_host_reload_xs(self.h, xsl_path.encode()) != 0:
             raise RuntimeError(host().mocc_host_last_error().decode())
 
+    def dump_iterations(self) -> list[int]:
+        """<solver><dump_iterations>, sorted."""
+        n = host().mocc_host_dump_iterations(self.h, None, 0)
+        out = np.zeros(max(n, 1), np.int32)
+        host().mocc_host_dump_iterations(self.h, out.ctypes.data_as(P), n)
+        return [int(v) for v in out[:n]]
+
     def params(self) -> HostParams:
         out = HostParams()
         host().mocc_host_get_params(self.h, C.byref(out))
@@ -248,3 +262,29 @@ class HostProblem:
             self.close()
         except Exception:
             pass
+
+
+def amend_xml(xml: str, amendments: list[str]) -> str:
+    """The deck text with mocc's command-line amendments applied
+    (mocc_host_amend_xml); raises ValueError with mocc's message."""
+    L = host()
+    arr = (C.c_char_p * max(1, len(amendments)))(*[a.encode() for a in amendments])
+    n = L.mocc_host_amend_xml(xml.encode(), len(amendments), arr, None, 0)
+    if n < 0:
+        raise ValueError(L.mocc_host_last_error().decode())
+    buf = C.create_string_buffer(n + 1)
+    L.mocc_host_amend_xml(xml.encode(), len(amendments), arr, buf, n + 1)
+    return buf.value.decode()
+
+
+def case_name(xml: str) -> str | None:
+    """The deck's <case_name> (None when absent); ValueError when empty."""
+    L = host()
+    n = L.mocc_host_case_name(xml.encode(), None, 0)
+    if n < 0:
+        raise ValueError(L.mocc_host_last_error().decode())
+    if n == 0:
+        return None
+    buf = C.create_string_buffer(n + 1)
+    L.mocc_host_case_name(xml.encode(), buf, n + 1)
+    return buf.value.decode()

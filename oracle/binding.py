@@ -21,7 +21,8 @@ i32, i64, f64, P = C.c_int32, C.c_int64, C.c_double, C.c_void_p
 class OracleOptions(C.Structure):
     _fields_ = [("n_inner", i32), ("gs_boundary", i32), ("energy_mode", i32), ("exact_exp", i32),
                 ("use_cmfd", i32), ("cmfd_max_iter", i32), ("cmfd_k_tol", f64),
-                ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("n_threads", i32)]
+                ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("n_threads", i32),
+                ("cmfd_negative_fixup", i32)]
 
 
 _L = None
@@ -158,6 +159,11 @@ class Oracle:
     def total_fission(self, old=False):
         out = np.zeros(1)
         lib().oracle_total_fission(self.h, 1 if old else 0, _p(out))
+        return float(out[0])
+
+    def flux_residual(self):
+        out = np.zeros(1)
+        lib().oracle_flux_residual(self.h, _p(out))
         return float(out[0])
 
     def currents(self, g):

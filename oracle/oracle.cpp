@@ -795,6 +795,9 @@ void cmfd_store_currents(Oracle &o)
 int cmfd_solve(Oracle &o, double &k)
 {
     const int G = o.G, nc = o.n_cell;
+    // <cmfd negative_fixup>: no negative coarse flux into the solve (cmfd.cpp:175-184)
+    if (o.o.cmfd_negative_fixup)
+        for (auto &v : o.cflux) v = std::max(v, 0.0);
     homogenize(o);
     cmfd_setup(o);
     double k_old = k;
@@ -942,6 +945,7 @@ void oracle_default_options(oracle_options *o)
     o->cmfd_psi_tol = 1e-5;
     o->cmfd_resid_reduction = 1e-3;
     o->n_threads = 0;
+    o->cmfd_negative_fixup = 0;
 }
 
 const char *oracle_last_error(void) { return g_err.c_str(); }

@@ -28,6 +28,7 @@ typedef struct oracle_options {
     int32_t cmfd_max_iter;
     double cmfd_k_tol, cmfd_psi_tol, cmfd_resid_reduction;
     int32_t n_threads;     /* 0: OpenMP default */
+    int32_t cmfd_negative_fixup; /* <cmfd negative_fixup> (cmfd.cpp:152-154, 175-184) */
 } oracle_options;
 
 void oracle_default_options(oracle_options *o);

@@ -39,7 +39,10 @@ def cmfd_decks(deck_dir):
     (51x51 cells, vacuum east/south) with coarser rays so the FP64 oracle
     finishes in seconds; a 48x48-pin lattice (2304 cells, vacuum east/south)
     is the size whose coarse solve runs 5 cells per lane with its rows in
-    LDS (the 51x51 core runs 6)."""
+    LDS (the 51x51 core runs 6). Square lattices of 24, 34 and 40 pins (576,
+    1156 and 1600 cells) run 2, 3 and 4 cells per lane with the rows in
+    registers, and 55x55 (3025 cells) 6 per lane re-reading the rows from
+    cache; all with vacuum east/south faces."""
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {
@@ -49,4 +52,7 @@ def cmfd_decks(deck_dir):
         "lat48": lg.write_deck(lg.pin_lattice(48, 2, 4, 0.1, "lat48_cmfd", cmfd=True,
                                               bc={"east": "vacuum", "south": "vacuum"}), deck_dir, 48),
     }
+    for n in (24, 34, 40, 55):
+        paths[f"lat{n}"] = lg.write_deck(lg.pin_lattice(n, 2, 4, 0.1, f"lat{n}_cmfd", cmfd=True,
+                                                        bc={"east": "vacuum", "south": "vacuum"}), deck_dir, n)
     return {k: HostProblem.load(v) for k, v in paths.items()}

@@ -41,7 +41,7 @@ def test_h5_exports():
     except RuntimeError as e:
         pytest.skip(str(e))
     names = declared("mocc_h5.h", "mocc_h5_")
-    assert len(names) == 5
+    assert len(names) == 6
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 

@@ -15,6 +15,8 @@ import dataclasses
 import numpy as np
 import pytest
 
+from oracle.binding import Oracle
+
 pytestmark = pytest.mark.gpu
 
 K_TOL, PSI_TOL, MAX_IT = 1e-7, 1e-6, 3000
@@ -226,3 +228,123 @@ def test_batch_plain_problem_beside_cmfd_bitwise(batch_decks):
     r = _solve_batch(mixed, ["pins3x3", target])[target]
     assert r.k == r_alone.k and r.iterations == r_alone.iterations
     assert np.array_equal(mixed[target].flux(), alone[target].flux())
+
+
+def test_batch_rejects_gauss_seidel_energy(batch_decks):
+    """A batch sweeps Jacobi in energy: a handle in mocc's Gauss-Seidel energy
+    mode (energy_mode 0) would follow another iteration path than its own
+    solve, so batch_add refuses it (MOCGPU_ERR_UNSUPPORTED) and the batch is
+    unchanged."""
+    from mocc_amd.sweeper import DeviceError, GpuBatch, GpuMoCSweeper
+    gs = GpuMoCSweeper(batch_decks["pins3x3"], energy_mode=0)
+    b = GpuBatch()
+    with pytest.raises(DeviceError, match="-6"):
+        b.add(gs, K_TOL, PSI_TOL, MAX_IT)
+    assert b.size() == 0 and b.bytes_used() == 0
+    ok = _sweepers(batch_decks, ["pins3x3"])["pins3x3"]
+    b.add(ok, K_TOL, PSI_TOL, MAX_IT)
+    assert b.size() == 1
+    b.close()
+
+
+@pytest.mark.parametrize("big", ["lat24", "lat40"])
+def test_batch_cmfd_register_rows_mixed_bitwise(cmfd_decks, big):
+    """Beside a mesh solved at 2 (24x24) or 4 (40x40) cells per lane with the
+    rows in registers, a 3x3-pin problem's k, iterations and flux are bitwise
+    those of its solve alone (one wave, one cell per lane)."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
+    alone = {"pins3x3": mk("pins3x3")}
+    r_alone = _solve_batch(alone, ["pins3x3"])["pins3x3"]
+    print(f"cmfd 3x3 pins beside {big}", flush=True)
+    mixed = {big: mk(big), "pins3x3": mk("pins3x3")}
+    r = _solve_batch(mixed, [big, "pins3x3"])
+    assert r["pins3x3"].converged and r[big].converged, (r["pins3x3"].iterations, r[big].iterations)
+    assert r["pins3x3"].k == r_alone.k and r["pins3x3"].iterations == r_alone.iterations
+    assert np.array_equal(mixed["pins3x3"].flux(), alone["pins3x3"].flux())
+
+
+def test_branch_batch_vs_oracle(tmp_path):
+    """Sixteen cross-section branch cases of the batched workload in one
+    batch (4 single pins, 8 3x3 lattices, 4 17x17 assemblies with guide
+    tubes; 2 and 7 groups; CY 16x3 at 0.02, 0.035 and 0.05 cm, the 17x17
+    cases at full density; cross sections x U[0.9, 1.1]; CMFD with 5 inner
+    iterations): every case's keff within 1e-5 of the FP64 oracle run in the
+    same iteration scheme (Jacobi energy sweeps, Gauss-Seidel boundary
+    update, CMFD)."""
+    from mocc_amd.branch import BranchFactory, c4_cases
+    from mocc_amd.sweeper import GpuBatch
+    cases = c4_cases(16)
+    big = [i for i, c in enumerate(cases) if c.kind == "17x17"]
+    for i, sp in zip(big, (0.02, 0.02, 0.035, 0.05)):
+        cases[i] = dataclasses.replace(cases[i], spacing=sp)
+    assert {cases[i].groups for i in big} == {2, 7}
+    factory = BranchFactory(str(tmp_path), energy_mode=1, n_inner=5, use_cmfd=True)
+    handles = [factory.make(c) for c in cases]
+    b = GpuBatch()
+    ids = [b.add(h, K_TOL, PSI_TOL, MAX_IT) for h in handles]
+    res = b.solve_all(chunk=4)
+    b.close()
+    worst = 0.0
+    for i, c in enumerate(cases):
+        st = res[ids[i]]
+        assert st.converged, (i, c, st.iterations)
+        hp = factory.host_problem(c)
+        o = Oracle(hp, n_inner=5, use_cmfd=True, energy_mode=1, gs_boundary=True)
+        rc, so, _ = o.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+        assert rc == 0
+        dk = abs(st.k - so.k)
+        worst = max(worst, dk)
+        print(f"case {i} {c.kind} {c.groups}g {c.spacing} cm: k {st.k:.8f} oracle {so.k:.8f} |dk| {dk:.1e} "
+              f"outer {st.iterations}/{so.iterations}", flush=True)
+        assert dk < 1e-5, (i, c, st.k, so.k)
+    for h in handles:
+        h.close()
+    factory.close()
+    print(f"worst |dk| over 16 branch cases: {worst:.2e}", flush=True)
+
+
+def test_batch_one_wave_queue_bitwise(batch_decks):
+    """A batch whose problems all run bands of at most 32 rays sweeps with
+    one-wave queue workgroups; a problem's result is bitwise the same as in a
+    batch that also holds 64-ray bands (two-wave workgroups)."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    mk = lambda n, br: GpuMoCSweeper(batch_decks[n], energy_mode=1, band_rays=br)
+    narrow = {"pins3x3": mk("pins3x3", 32)}
+    assert narrow["pins3x3"].options().band_rays == 32
+    print("3x3 pins, 32-ray bands, one-wave queue", flush=True)
+    r1 = _solve_batch(narrow, ["pins3x3"])["pins3x3"]
+    mixed = {"pins3x3": mk("pins3x3", 32), "asm17": mk("asm17", 64)}
+    print("3x3 pins, 32-ray bands, beside 64-ray bands", flush=True)
+    r2 = _solve_batch(mixed, ["asm17", "pins3x3"])["pins3x3"]
+    assert r1.converged and r1.k == r2.k and r1.iterations == r2.iterations
+    assert np.array_equal(narrow["pins3x3"].flux(), mixed["pins3x3"].flux())
+
+
+def test_autotuned_branch_batch(tmp_path):
+    """Branch cases whose geometries were autotuned (band width x lane map
+    timed on the device, shared by every case of the geometry) solve in one
+    batch to the eigenvalues of the same cases without tuning (within the
+    rounding of the fixed-point tally scale, which follows the band layout)."""
+    from mocc_amd.branch import BranchFactory, c4_cases
+    from mocc_amd.sweeper import GpuBatch
+    cases = c4_cases(8)
+    res = {}
+    for tune in (True, False):
+        f = BranchFactory(str(tmp_path / f"t{int(tune)}"), energy_mode=1, n_inner=5, use_cmfd=True, autotune=tune)
+        hs = [f.make(c) for c in cases]
+        if tune:
+            tuned = f.tuned()
+            print("tuned (band rays, lane map):", tuned, flush=True)
+            assert all(br in (32, 48, 64) and lm in (0, 1) for br, lm in tuned.values())
+        b = GpuBatch()
+        ids = [b.add(h, K_TOL, PSI_TOL, MAX_IT) for h in hs]
+        out = b.solve_all(chunk=4)
+        b.close()
+        res[tune] = [out[i] for i in ids]
+        for h in hs:
+            h.close()
+        f.close()
+    for a, b in zip(res[True], res[False]):
+        assert a.converged and b.converged
+        assert abs(a.k - b.k) < 1e-9, (a.k, b.k)

@@ -47,7 +47,11 @@ def _check_pair(hp, tight=True):
     return dev, ora, kd, ko
 
 
-@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "lat48", "core51"])
+# coarse-solve instantiations: 1 cell per lane (pins3x3 on one wave, asm17 on
+# 320 lanes), 2, 3 and 4 per lane with the rows in registers (lat24, lat34,
+# lat40), 5 and 6 with the rows in LDS (lat48, core51), 6 re-reading the rows
+# from cache (lat55)
+@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "lat24", "lat34", "lat40", "lat48", "core51", "lat55"])
 def test_cmfd_step_parity(deck, cmfd_decks):
     hp = cmfd_decks[deck]
     dev, ora, kd, ko = _check_pair(hp, tight=True)
@@ -81,3 +85,44 @@ def test_cmfd_rejects_without_option(pins3x3):
     dev.initialize()
     with pytest.raises(DeviceError):
         dev.cmfd_step(1.0)
+
+
+def test_cmfd_negative_fixup_parity(cmfd_decks):
+    """<cmfd negative_fixup> (cmfd.cpp:152-154, 175-184): a fine flux with one
+    pin made negative, identical on both sides; with the fixup the device
+    clamps that coarse flux to zero before the solve as the oracle does, and
+    the result differs from the solve without it."""
+    hp = cmfd_decks["lat24"]
+    out = {}
+    for fixup in (1, 0):
+        cm = dict(TIGHT, cmfd_negative_fixup=fixup)
+        print(f"negative fixup={fixup}", flush=True)
+        ora = _oracle_state(hp, **cm)
+        phi = ora.flux()
+        off = hp.array("pin_fsr_off", hp.p.nx * hp.p.ny + 1, np.int64)
+        ip = hp.p.nx * hp.p.ny // 2 + hp.p.nx // 2  # a pin inside the lattice
+        phi[off[ip]:off[ip + 1], 0] *= -0.5
+        ora.set_flux(phi)
+        dev = _device_from(hp, ora, **cm)
+        kd = dev.cmfd_step(1.0)
+        ko, _ = ora.cmfd_step(1.0)
+        assert abs(kd - ko) < 1e-10, (fixup, kd, ko)
+        np.testing.assert_allclose(dev.cmfd_flux(), ora.cmfd_flux(), rtol=1e-8, atol=0)
+        np.testing.assert_allclose(dev.flux(), ora.flux(), rtol=1e-8, atol=0)
+        out[fixup] = kd
+    assert abs(out[1] - out[0]) > 1e-9, out
+
+
+def test_cmfd_second_solve_bitwise(cmfd_decks):
+    """mocgpu_initialize drops the currents of an earlier solve: solving one
+    CMFD handle twice gives bitwise the same k, iteration count and flux
+    (each solve's first CMFD step runs with D-hat = 0)."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    hp = cmfd_decks["pins3x3"]
+    dev = GpuMoCSweeper(hp, n_inner=5, use_cmfd=True)
+    r1, s1 = dev.eigen_solve(1e-7, 1e-6, 200)
+    f1 = dev.flux()
+    r2, s2 = dev.eigen_solve(1e-7, 1e-6, 200)
+    assert r1 == r2 == 0
+    assert s1.k == s2.k and s1.iterations == s2.iterations and s1.err_psi == s2.err_psi
+    assert np.array_equal(dev.flux(), f1)

@@ -1,14 +1,29 @@
 """k-eigenvalue solves on the device against the FP64 oracle
 (EigenSolver::solve, src/solvers/eigen_solver.cpp:131-253).
 
-The device sweeps in FP32 with Jacobi boundary updates; the oracle runs the
-reference's defaults (Gauss-Seidel boundary update, FP64). Both iterate to the
-same discrete fixed point, so converged eigenvalues agree far inside the
-|dk| <= 1e-5 bar and the normalized FSR fission sources within 1e-4.
+Both sides run mocc's defaults: Gauss-Seidel boundary update in angle
+(moc_sweeper_kernel.inc.hpp:137-142) and Gauss-Seidel sweeps over groups
+(fixed_source_solver.cpp:102-117). The device sweeps in FP32 with FP64
+reductions, the oracle in FP64; both iterate the same scheme to the same
+discrete fixed point, so
+  * converged eigenvalues agree far inside the |dk| <= 1e-5 bar and the
+    normalized FSR fission sources within 1e-4;
+  * the solves stop at the same outer iteration (within one: the last
+    iteration's err_psi can sit on either side of psi_tol);
+  * on the infinite homogeneous medium both reach the closed-form
+    k = nu Sigma_f . M^-1 chi (test_MoC_IHM.cpp:152-185).
+
+The full-density 51x51-pin core (CY 64x3 at 0.02 cm, 178M segments) is
+checked against the oracle's solve stored in tests/golden/core51_oracle.npz
+(written by tests/golden/make_core51_oracle.py; the FP64 solve takes many
+minutes on host cores).
 """
+import os
+
 import numpy as np
 import pytest
 
+from conftest import DECKS, GOLDEN
 from oracle.binding import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -25,23 +40,38 @@ def _fission_shape(hp, phi):
     return fs / fs.sum()
 
 
-def _solve_pair(hp, use_cmfd, max_iter):
+def _solve_pair(hp, use_cmfd, max_iter, n_inner=None):
     from mocc_amd.sweeper import GpuMoCSweeper
-    n_inner = hp.params().n_inner
+    n_inner = n_inner or hp.params().n_inner
     print(f"eigen F={hp.n_fsr} G={hp.n_group} cmfd={use_cmfd} n_inner={n_inner}", flush=True)
-    dev = GpuMoCSweeper(hp, n_inner=n_inner, use_cmfd=use_cmfd)
+    dev = GpuMoCSweeper(hp, n_inner=n_inner, use_cmfd=use_cmfd, energy_mode=0, gs_boundary=True)
     rc_d, st_d = dev.eigen_solve(K_TOL, PSI_TOL, max_iter)
-    ora = Oracle(hp, n_inner=n_inner, use_cmfd=use_cmfd)
+    ora = Oracle(hp, n_inner=n_inner, use_cmfd=use_cmfd, energy_mode=0, gs_boundary=True)
     rc_o, st_o, _ = ora.eigen_solve(K_TOL, PSI_TOL, max_iter)
     assert rc_d == 0 and st_d.converged, (rc_d, st_d.iterations)
     assert rc_o == 0
+    print(f"  k device {st_d.k:.10f} oracle {st_o.k:.10f}; outer iterations {st_d.iterations} / {st_o.iterations}",
+          flush=True)
     return dev, ora, st_d, st_o
 
 
-@pytest.mark.parametrize("use_cmfd", [False, True])
-def test_pins3x3_keff(pins3x3, use_cmfd):
-    dev, ora, st_d, st_o = _solve_pair(pins3x3, use_cmfd, 3000)
+# (use_cmfd, n_inner): the deck's single inner iteration without CMFD, and
+# 5 inner iterations with CMFD, as mocc's CMFD decks run; there the device
+# and the oracle agree at every outer iteration (to ~3e-9 in k) and stop
+# together. With CMFD and one inner iteration the outer iteration count is
+# not a stable quantity on this deck: the coarse solves built on the first
+# unconverged sweeps drive k negative (-1.85 at the 7th outer iteration, on
+# both sides alike), and the transient that follows amplifies rounding
+# differences until the trajectories part; both settle ~150-175 iterations
+# later (the oracle alone stops anywhere from 171 to 174 depending on the
+# order its threads add their tallies). That case checks k and the fission
+# source only.
+@pytest.mark.parametrize("use_cmfd,n_inner,count", [(False, 1, True), (True, 5, True), (True, 1, False)])
+def test_pins3x3_keff(pins3x3, use_cmfd, n_inner, count):
+    dev, ora, st_d, st_o = _solve_pair(pins3x3, use_cmfd, 3000, n_inner)
     assert abs(st_d.k - st_o.k) < 1e-5, (st_d.k, st_o.k)
+    if count:
+        assert abs(st_d.iterations - st_o.iterations) <= 1, (st_d.iterations, st_o.iterations)
     fd, fo = _fission_shape(pins3x3, dev.flux()), _fission_shape(pins3x3, ora.flux())
     m = fo > 0
     assert np.abs(fd[m] / fo[m] - 1.0).max() < 1e-4
@@ -51,8 +81,77 @@ def test_core51_keff_cmfd(cmfd_decks):
     hp = cmfd_decks["core51"]
     dev, ora, st_d, st_o = _solve_pair(hp, True, 200)
     assert abs(st_d.k - st_o.k) < 1e-5, (st_d.k, st_o.k)
+    assert abs(st_d.iterations - st_o.iterations) <= 1, (st_d.iterations, st_o.iterations)
     # CMFD cuts the outer iterations by an order of magnitude on this deck
     assert st_d.iterations < 40
     fd, fo = _fission_shape(hp, dev.flux()), _fission_shape(hp, ora.flux())
     m = fo > 1e-6 * fo.max()
     assert np.abs(fd[m] / fo[m] - 1.0).max() < 1e-4
+
+
+IHM_XML = """<mesh id="1" type="rect" pitch="1.26"><sub_x>3</sub_x><sub_y>3</sub_y></mesh>
+<pin id="1" mesh="1">1 1 1 1 1 1 1 1 1</pin>
+<lattice id="1" nx="3" ny="2">1 1 1 1 1 1</lattice>
+<assembly id="1" np="1" hz="0.5"><lattices>1</lattices></assembly>
+<core nx="1" ny="1" north="reflect" south="reflect" east="reflect" west="reflect" top="reflect" bottom="reflect">1</core>
+<material_lib path="c5g7.xsl"><material id="1" name="UO2-3.3" /></material_lib>
+<source scattering="P0" />
+<sweeper type="moc" n_inner="1"><ang_quad type="ls" order="2" /><rays spacing="0.01" /></sweeper>"""
+
+
+def test_ihm_k_infinity():
+    """The infinite homogeneous medium of mocc's test_MoC_IHM (C5G7 UO2, 7
+    groups with up-scatter, level-symmetric quadrature): the device power
+    iteration converges to k = nu Sigma_f . M^-1 chi and the flux spectrum
+    M^-1 chi in every FSR."""
+    from mocc_amd.native import HostProblem
+    from mocc_amd.sweeper import GpuMoCSweeper
+    hp = HostProblem.from_string(IHM_XML, DECKS)
+    G = hp.n_group
+    tr = hp.array("xs_tr", G)
+    nf = hp.array("xs_nf", G)
+    ch = hp.array("xs_ch", G)
+    sc = hp.array("xs_sc", G * G).reshape(G, G)
+    phi = np.linalg.solve(np.diag(tr) - sc, ch)
+    kinf = float(nf @ phi)
+    for energy_mode in (0, 1):
+        print(f"IHM energy_mode={energy_mode}", flush=True)
+        dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=energy_mode)
+        rc, st = dev.eigen_solve(1e-10, 1e-8, 5000)
+        assert rc == 0 and st.converged, (rc, st.iterations)
+        assert abs(st.k - kinf) < 1e-6, (energy_mode, st.k, kinf)
+        f = dev.flux()
+        np.testing.assert_allclose(f / f[:, :1], np.tile(phi / phi[0], (hp.n_fsr, 1)), rtol=1e-5, atol=0)
+
+
+def test_core51_full_density_vs_oracle(deck_dir):
+    """The 51x51-pin core at full density (3x3 assemblies of 17x17, 7 groups,
+    CY 64x3 at 0.02 cm, vacuum east/south, CMFD, 5 inner iterations, mocc's
+    default Gauss-Seidel boundary and energy updates): keff within 1e-5 of
+    the FP64 oracle, the FSR fission-source shape within 1e-4 relative, and
+    the same outer iteration count within one."""
+    import time
+
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    from mocc_amd.sweeper import GpuMoCSweeper
+    ref = np.load(os.path.join(GOLDEN, "core51_oracle.npz"))
+    t0 = time.time()
+    hp = HostProblem.load(lg.write_config("core51", deck_dir))
+    par = hp.params()
+    print(f"core51 full density traced: F={hp.n_fsr} S={hp.p.n_seg} ({time.time() - t0:.0f} s)", flush=True)
+    dev = GpuMoCSweeper(hp, energy_mode=0, gs_boundary=True)
+    print("core51 device plan uploaded", flush=True)
+    t0 = time.time()
+    rc, st = dev.eigen_solve(par.k_tol, par.psi_tol, par.max_iter)
+    print(f"core51 device k={st.k:.10f} in {st.iterations} outer iterations ({time.time() - t0:.2f} s); "
+          f"oracle k={float(ref['k']):.10f} in {int(ref['iterations'])}", flush=True)
+    assert rc == 0 and st.converged
+    assert abs(st.k - float(ref["k"])) < 1e-5
+    assert abs(st.iterations - int(ref["iterations"])) <= 1
+    fd = _fission_shape(hp, dev.flux())
+    fo = ref["fission_shape"].astype(np.float64)
+    m = fo > 1e-6 * fo.max()
+    err = float(np.abs(fd[m] / fo[m] - 1.0).max())
+    print(f"core51 max relative fission-source-shape error {err:.2e}", flush=True)
+    assert err < 1e-4

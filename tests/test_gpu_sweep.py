@@ -168,3 +168,45 @@ def test_autotuned_lane_map(pins3x3):
         s.calc_fission_source(1.0)
         s.fss_step(with_current=False)
     assert _rel(dev.flux(), ora.flux()) < FLUX_RTOL
+
+
+@pytest.mark.parametrize("gs", [True, False])
+def test_flux_residual_parity(pins3x3, gs):
+    """TransportSweeper::flux_residual (transport_sweeper.cpp:210-223): the
+    2-norm of phi - phi_old after an outer step, against the oracle's."""
+    dev, ora = _setup_pair(pins3x3, 0, seed=11, gs=gs)
+    for s in (dev, ora):
+        s.fss_step()
+    rd, ro = dev.flux_residual(), ora.flux_residual()
+    print(f"flux residual device {rd:.10e} oracle {ro:.10e}", flush=True)
+    assert ro > 0 and abs(rd / ro - 1.0) < 1e-5
+    # a second step, closer to the fixed point: the residual shrinks on both
+    for s in (dev, ora):
+        s.fss_step()
+    rd2, ro2 = dev.flux_residual(), ora.flux_residual()
+    assert rd2 < rd and abs(rd2 / ro2 - 1.0) < 1e-4
+
+
+def test_group_index_checked(pins3x3):
+    """Entry points taking a group index reject g < 0 and g >= n_group with
+    MOCGPU_ERR_ARG before touching device memory."""
+    from mocc_amd.native import gpu, ptr
+    from mocc_amd.sweeper import GpuMoCSweeper
+    dev = GpuMoCSweeper(pins3x3, n_inner=1)
+    dev.initialize()
+    L = gpu()
+    G, npin = pins3x3.n_group, dev.npin
+    pin = np.ones(npin)
+    cur, sfl = np.zeros(L.mocgpu_n_surf(dev.h)), np.zeros(L.mocgpu_n_surf(dev.h))
+    r = np.zeros(1)
+    for g in (-1, G):
+        assert L.mocgpu_get_pin_flux_1g(dev.h, g, ptr(pin)) == -1
+        assert L.mocgpu_set_pin_flux_1g(dev.h, g, ptr(pin), ptr(r)) == -1
+        assert L.mocgpu_get_currents(dev.h, g, ptr(cur), ptr(sfl)) == -1
+        assert L.mocgpu_set_currents(dev.h, g, ptr(cur), ptr(sfl)) == -1
+        assert L.mocgpu_sweep(dev.h, g, 0) == -1
+        assert b"group out of range" in L.mocgpu_last_error(dev.h)
+    # the handle still works, and a non-positive pin flux is still reported
+    assert L.mocgpu_get_pin_flux_1g(dev.h, 0, ptr(pin)) == 0
+    pin[0] = 0.0
+    assert L.mocgpu_set_pin_flux_1g(dev.h, 0, ptr(pin), ptr(r)) == -4
