@@ -378,6 +378,17 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n_sweeps, double *ms_per_sweep,
  * MOCGPU_ERR_DEVICE. */
 int32_t mocgpu_attenuation(const float *tau, float *out, int32_t n);
 
+/* mocgpu_attenuation for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference. It calls no hipMalloc or hipFree: the
+ * values are staged in a fixed per-device array of
+ * 2 * MOCGPU_ATTENUATION_NOALLOC_MAX floats (2 MiB), held by each device the
+ * library runs on until the process ends. n above
+ * MOCGPU_ATTENUATION_NOALLOC_MAX returns MOCGPU_ERR_CAPACITY. Calls on one
+ * device share that array: calls that overlap on one device race. Results
+ * are bitwise those of mocgpu_attenuation. */
+#define MOCGPU_ATTENUATION_NOALLOC_MAX 262144
+int32_t mocgpu_attenuation_noalloc(const float *tau, float *out, int32_t n);
 
 #ifdef __cplusplus
 }

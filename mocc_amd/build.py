@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "mocc_amd")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
-GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip"]
+GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip", "noalloc.hip"]
 # No floating-point contraction in device code: a kernel is instantiated
 # several ways (cells per lane of the coarse solve, with or without current
 # tallies, single-problem or batched), and a batch picks the instantiation for
