@@ -140,6 +140,9 @@ typedef struct mocgpu_options {
     int32_t cmfd_negative_fixup; /* <cmfd negative_fixup>, default 0: clamp negative
                                coarse fluxes to 0 before each CMFD solve (cmfd.cpp:152-154,
                                175-184) */
+    int32_t correction_worker; /* 1: the handle can run the 2-D/3-D sweeper's correction
+                               sweep (mocgpu_sweep_2d3d, cmdo::MoCSweeper_2D3D); it keeps
+                               a ray-major copy of the segments. Default 0 */
 } mocgpu_options;
 
 typedef struct mocgpu_handle mocgpu_handle;
@@ -255,6 +258,33 @@ int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current,
  * mesh may hold at most 3072 pin cells. A non-positive coarse flux returns
  * MOCGPU_ERR_NONPOS_FLUX (MoCSweeper::set_pin_flux_1g throws there). */
 int mocgpu_cmfd_step(mocgpu_handle *h, double *k);
+
+/* MoCSweeper::update_incoming_flux (moc_sweeper.cpp:255-291): the incoming
+ * angular flux at every ray's entry, for every group and angle, set from the
+ * partial currents the last CMFD solve stored (2/(4 pi) (J+ + J-)) after the
+ * first solve, scaled by their ratio to the previous solve's after later ones.
+ * Indexed, as the reference does (moc_sweeper.hpp:217-222), by the ray's first
+ * coarse cell. Needs options.use_cmfd. The reference's caller has this call
+ * commented out (eigen_solver.cpp:289); the device solve loops do not make it. */
+int mocgpu_update_incoming_flux(mocgpu_handle *h);
+
+/* cmdo::MoCSweeper_2D3D::sweep(g) (src/sweepers/cmdo/moc_sweeper_2d3d.cpp:48-99)
+ * for a handle created with options.correction_worker = 1 and use_cmfd = 1:
+ * n_inner inner sweeps of group g (source built beforehand), the last with the
+ * cmdo::CurrentCorrections worker (correction_worker.hpp:36-292,
+ * correction_worker.cpp:33-160). The pin-cell Sigma_t is homogenized from the
+ * flux before that sweep; the worker computes, per coarse cell, angle and
+ * direction, alpha_x, alpha_y (cell-average over summed face-average angular
+ * flux) and beta (flux-weighted Sigma_t over the homogenized one), and the
+ * coarse currents and surface fluxes of group g with the worker's signs. */
+int mocgpu_sweep_2d3d(mocgpu_handle *h, int32_t g);
+/* CorrectionData (correction_data.hpp:45-100): alpha [G][4*ndir_oct][nx*ny][2]
+ * (x, y normal), beta [G][4*ndir_oct][nx*ny]; 0.5 and 1 until a correction
+ * sweep of the group sets them. Either pointer may be NULL. */
+int mocgpu_get_corrections(mocgpu_handle *h, double *alpha, double *beta);
+/* The residual triple (alpha_x, alpha_y, beta) of group g's last correction
+ * sweep (CurrentCorrections::residual, correction_worker.hpp:83-91). */
+int mocgpu_correction_residual(mocgpu_handle *h, int32_t g, double *out);
 
 /* Outer (coarse power) iterations of the last CMFD solve. */
 int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h);

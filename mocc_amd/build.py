@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "mocc_amd")
 LIB = os.path.join(PKG, "lib")
 HOST_SRC = ["xml.cpp", "model.cpp", "problem.cpp", "host_api.cpp"]
-GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip", "noalloc.hip"]
+GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "corr.hip", "mocgpu.hip", "batch.hip", "noalloc.hip"]
 # No floating-point contraction in device code: a kernel is instantiated
 # several ways (cells per lane of the coarse solve, with or without current
 # tallies, single-problem or batched), and a batch picks the instantiation for
@@ -25,7 +25,7 @@ GPU_SRC = ["plan.cpp", "sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip", "noal
 # its own, so a problem's arithmetic does not depend on how the compiler fused
 # a given instantiation; the hot loops write their fused multiply-adds
 # explicitly.
-GPU_EXTRA_FLAGS = {s: ["-ffp-contract=off"] for s in ("sweep.hip", "cmfd.hip", "mocgpu.hip", "batch.hip")}
+GPU_EXTRA_FLAGS = {s: ["-ffp-contract=off"] for s in ("sweep.hip", "cmfd.hip", "corr.hip", "mocgpu.hip", "batch.hip")}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # No packed FP32 instructions (v_pk_*_f32) anywhere in device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

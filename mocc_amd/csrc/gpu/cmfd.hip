@@ -221,6 +221,7 @@ struct CoupleArgs {
     int bc_lo[3], bc_hi[3];  // [normal]: low-side and high-side boundary kinds
     const double *htr, *cflux, *cur, *sfl;
     double *dtil, *stil, *dhat, *shat;
+    double *part, *part_old;  // partial currents [n_surf][G][2] (null: not kept)
 };
 
 // CMFD::setup_solve, surface couplings (cmfd.cpp:423-490): item t = (surface, group)
@@ -310,6 +311,15 @@ __device__ __forceinline__ void store_currents_item(const CoupleArgs &a, double 
     const double sf = (c1 >= 0) ? st * fl + (1.0 - st) * fr + sh * (fl + fr) : st * fr + sh * (fl + fr);
     sfl[(size_t)is * G + g] = sf;
     cur[(size_t)is * G + g] = j;
+    if (a.part) {
+        // CMFD::store_currents (cmfd.cpp:560-599): the previous partial
+        // currents become the old ones, then (sf/4 + J/2, sf/4 - J/2)
+        const size_t k = ((size_t)is * G + g) * 2;
+        a.part_old[k] = a.part[k];
+        a.part_old[k + 1] = a.part[k + 1];
+        a.part[k] = 0.25 * sf + 0.5 * j;
+        a.part[k + 1] = 0.25 * sf - 0.5 * j;
+    }
 }
 
 __global__ void k_store_currents(CoupleArgs a, double *cur, double *sfl)
@@ -868,7 +878,8 @@ CmfdSolver::~CmfdSolver() { free_all(); }
 void CmfdSolver::free_all()
 {
     double **ds[] = {&d_dx_, &d_dy_, &d_cvol_, &d_cflux_, &d_htr_, &d_hnf_, &d_hf_, &d_hch_, &d_hrm_,
-                     &d_hsc_, &d_dtil_, &d_stil_, &d_dhat_, &d_shat_, &d_coef_, &d_gscr_};
+                     &d_hsc_, &d_dtil_, &d_stil_, &d_dhat_, &d_shat_, &d_coef_, &d_gscr_, &d_part_,
+                     &d_part_old_};
     for (double **p : ds) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -930,6 +941,8 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     dalloc(d_shat_, (size_t)n_surf_ * G_);
     dalloc(d_coef_, 5 * ncg);
     dalloc(d_gscr_, 3 * (size_t)nc_);
+    dalloc(d_part_, 2 * (size_t)n_surf_ * G_);
+    dalloc(d_part_old_, 2 * (size_t)n_surf_ * G_);
     dalloc(d_info_, 3);
     initialize(s);
 }
@@ -937,9 +950,14 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
 void CmfdSolver::initialize(hipStream_t s)
 {
     n_solve_ = 0;
+    has_old_partial_ = false;
     if (!d_info_) return;
     hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(64), 0, s, d_info_);
+    const size_t np = (size_t)n_surf_ * G_ * 2;
+    CMFD_CHECK(hipMemsetAsync(d_part_, 0, np * sizeof(double), s));
+    CMFD_CHECK(hipMemsetAsync(d_part_old_, 0, np * sizeof(double), s));
 }
+
 
 // The arguments of one problem's CMFD kernels (shared by enqueue_step and the
 // batch launches over many problems).
@@ -966,6 +984,8 @@ struct CmfdAccess {
         ca.stil = c.d_stil_;
         ca.dhat = c.d_dhat_;
         ca.shat = c.d_shat_;
+        ca.part = c.d_part_;
+        ca.part_old = c.d_part_old_;
         return ca;
     }
     static SolveArgs solve(const CmfdSolver &c, const CmfdView &v)
@@ -999,6 +1019,14 @@ struct CmfdAccess {
     static double *coef(const CmfdSolver &c) { return c.d_coef_; }
     static bool enabled(const CmfdSolver &c) { return c.d_cflux_ != nullptr; }
 };
+
+void CmfdSolver::homogenize(const CmfdView &v, hipStream_t s)
+{
+    if (!d_cflux_) throw std::runtime_error("CMFD is not enabled for this handle (use_cmfd = 0)");
+    const int npg = v.npin * v.G;
+    hipLaunchKernelGGL(k_homogenize, dim3(nblk(npg)), dim3(256), 0, s, v, CmfdAccess::homog(*this));
+    CMFD_CHECK(hipGetLastError());
+}
 
 void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
 {
@@ -1043,6 +1071,7 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
         break;
     }
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
+    has_old_partial_ = n_solve_ > 0;  // cmfd.cpp:566, before n_solve_++ (:274)
     hipLaunchKernelGGL(k_set_pin_flux_all, dim3(nblk(npg)), dim3(256), 0, s, v, d_cflux_, d_info_ + 1);
     CMFD_CHECK(hipGetLastError());
     n_solve_++;

@@ -51,6 +51,18 @@ public:
     void matrix_csr(int g, hipStream_t s, std::vector<int32_t> &row_ptr, std::vector<int32_t> &col,
                     std::vector<double> &val);
     void coarse_flux(hipStream_t s, std::vector<double> &out);
+    // XSMeshHomogenized::update (xs_mesh_homogenized.cpp:176-197) alone: the
+    // pin-cell cross sections from the current fine flux, as the 2-D/3-D
+    // sweeper updates its Sn cross-section mesh before its correction sweep.
+    void homogenize(const CmfdView &v, hipStream_t s);
+    // homogenized transport cross section [n_cell][G] of the last homogenization
+    const double *homog_tr() const { return d_htr_; }
+    // CoarseData::partial_current / partial_current_old [n_surf][G][2] of the
+    // latest and the previous solve (cmfd.cpp:583-599), and whether the old
+    // ones exist (set_has_old_partial(n_solve > 0) at the store)
+    const double *partial() const { return d_part_; }
+    const double *partial_old() const { return d_part_old_; }
+    bool has_old_partial() const { return has_old_partial_; }
 
 private:
     friend struct CmfdAccess;
@@ -68,7 +80,9 @@ private:
     double *d_gscr_ = nullptr;  // [3][nc]: previous coarse fission source, BiCGSTAB right-hand side and inverse diagonal (solves with the rows in LDS)
                                 // (solves that keep the rows in LDS)
     int *d_info_ = nullptr;     // [0] outer iterations, [1] non-positive pin flux seen
+    double *d_part_ = nullptr, *d_part_old_ = nullptr;  // [n_surf][G][2] partial currents
     int n_solve_ = 0;
+    bool has_old_partial_ = false;
 };
 
 // One problem of a batch that runs CMFD: its index in the batch's status

@@ -14,6 +14,7 @@
 
 #include "../../../include/mocgpu.h"
 #include "cmfd.hpp"
+#include "corr.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -124,6 +125,9 @@ struct mocgpu_handle {
     DBuf<uint64_t> d_parity;
     DBuf<int32_t> d_status, d_radial;
     std::unique_ptr<CmfdSolver> cmfd;
+    // rays for update_incoming_flux (CMFD handles) and the 2-D/3-D
+    // correction sweep (options.correction_worker)
+    std::unique_ptr<CorrSolver> corr;
 
     // host copies
     std::vector<float> bc_template;  // one group of incoming values after initialize

@@ -291,6 +291,7 @@ void mocgpu_default_options(mocgpu_options *o)
     o->cmfd_solver = 0;
     o->gs_boundary = 1;
     o->cmfd_negative_fixup = 0;
+    o->correction_worker = 0;
 }
 
 const char *mocgpu_last_error(const mocgpu_handle *h) { return h ? h->err.c_str() : g_create_err.c_str(); }
@@ -481,6 +482,10 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         h->d_radial.alloc(1);
         h->cmfd = std::make_unique<CmfdSolver>();
         h->cmfd->setup(*p, h->opt, s);
+        if (h->opt.use_cmfd || h->opt.correction_worker) {
+            h->corr = std::make_unique<CorrSolver>();
+            h->corr->setup(*p, pl, s, h->opt.correction_worker != 0);
+        }
         h->sync();
         return 0;
     });
@@ -594,7 +599,7 @@ int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
                      h->d_slot_pos.bytes() + h->d_ev_off.bytes() + h->d_ev.bytes() + h->d_surf_ptr.bytes() +
                      h->d_surf_idx.bytes() + h->d_surf_area.bytes() + h->d_fsr_mat.bytes() +
                      h->d_vol.bytes() + h->d_area.bytes() + h->d_pin_off.bytes() + h->d_pin_cell.bytes() +
-                     h->d_pin_area.bytes();
+                     h->d_pin_area.bytes() + (h->corr ? h->corr->bytes() : 0);
     return (int64_t)b;
 }
 
@@ -1044,6 +1049,89 @@ int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *c
         std::copy(cl.begin(), cl.end(), col);
         std::copy(vl.begin(), vl.end(), val);
         return (int)cl.size();
+    });
+}
+
+// MoCSweeper_2D3D::sweep (moc_sweeper_2d3d.cpp:48-99): the inner iterations
+// of group g, the last one with the CurrentCorrections worker. The pin-cell
+// cross sections are homogenized from the flux before that sweep; its band
+// sweep updates the flux and boundary as any sweep, then the correction sweep
+// (from a copy of the same inputs) tallies the factors and the coarse currents.
+int mocgpu_sweep_2d3d(mocgpu_handle *h, int32_t g)
+{
+    return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        if (!h->corr || !h->corr->full())
+            throw std::runtime_error("the handle was created without the correction worker "
+                                     "(options.correction_worker = 0)");
+        if (!h->opt.use_cmfd)
+            throw std::runtime_error("2D3D MoC sweeper needs coarse data to collect calculate correction "
+                                     "factors. Try enabling CMFD.");
+        for (int inner = 0; inner < h->opt.n_inner; inner++) {
+            if (inner == h->opt.n_inner - 1) {
+                CmfdView v = h->cmfd_view();
+                h->cmfd->homogenize(v, h->stream);
+                CorrView cv{};
+                cv.F = h->F;
+                cv.G = h->G;
+                cv.g = g;
+                cv.bcpg = h->plan->bcpg;
+                cv.gs = h->opt.gs_boundary ? 1 : 0;
+                cv.parity = (int)((h->parity >> g) & 1ull) << g;
+                cv.psi = h->d_psi.p;
+                cv.src = h->d_src.p;
+                cv.phi = h->d_phi.p;
+                cv.fsr_mat = h->d_fsr_mat.p;
+                cv.mat_tr = h->d_mat_tr.p;
+                cv.mat_ss = h->d_mat_ss.p;
+                cv.htr = h->cmfd->homog_tr();
+                cv.cur = h->d_cur.p;
+                cv.sfl = h->d_sfl.p;
+                h->corr->before_sweep(cv, h->stream);
+                h->sweep_once(g, false);
+                h->corr->after_sweep(cv, h->stream);
+                h->has_radial = true;
+            } else {
+                h->sweep_once(g, false);
+            }
+        }
+        h->sync();
+        return 0;
+    });
+}
+
+int mocgpu_get_corrections(mocgpu_handle *h, double *alpha, double *beta)
+{
+    return guard(h, [&] {
+        if (!h->corr || !h->corr->full())
+            throw std::runtime_error("the handle was created without the correction worker");
+        h->corr->corrections(h->stream, alpha, beta);
+        return 0;
+    });
+}
+
+int mocgpu_correction_residual(mocgpu_handle *h, int32_t g, double *out)
+{
+    return guard(h, [&] {
+        if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
+        if (!h->corr || !h->corr->full())
+            throw std::runtime_error("the handle was created without the correction worker");
+        h->corr->residual(h->stream, g, out);
+        return 0;
+    });
+}
+
+// MoCSweeper::update_incoming_flux (moc_sweeper.cpp:255-291) from the partial
+// currents the last CMFD solves stored.
+int mocgpu_update_incoming_flux(mocgpu_handle *h)
+{
+    return guard(h, [&] {
+        if (!h->opt.use_cmfd || !h->corr)
+            throw std::runtime_error("update_incoming_flux needs the CMFD partial currents (options.use_cmfd = 0)");
+        h->corr->update_incoming(h->d_psi.p, h->parity, h->G, h->cmfd->partial(), h->cmfd->partial_old(),
+                                 h->cmfd->has_old_partial(), h->stream);
+        h->sync();
+        return 0;
     });
 }
 

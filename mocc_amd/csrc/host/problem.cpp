@@ -22,6 +22,7 @@ void host_default_options(mocgpu_options *o)
     o->cmfd_solver = 0;
     o->gs_boundary = 1;
     o->cmfd_negative_fixup = 0;
+    o->correction_worker = 0;
 }
 
 namespace {

@@ -51,7 +51,7 @@ class MocgpuOptions(C.Structure):
         ("n_inner", i32), ("energy_mode", i32), ("lane_map", i32), ("band_rays", i32),
         ("use_cmfd", i32), ("cmfd_max_iter", i32), ("cmfd_k_tol", f64),
         ("cmfd_psi_tol", f64), ("cmfd_resid_reduction", f64), ("cmfd_solver", i32),
-        ("gs_boundary", i32), ("cmfd_negative_fixup", i32),
+        ("gs_boundary", i32), ("cmfd_negative_fixup", i32), ("correction_worker", i32),
     ]
 
 
@@ -162,6 +162,10 @@ def gpu() -> C.CDLL:
         L.mocgpu_eigen_solve_history.argtypes = [P, f64, f64, i32, i32, C.POINTER(EigenState), P, i32]
         L.mocgpu_eigen_iterate.argtypes = [P, C.POINTER(EigenState)]
         L.mocgpu_eigen_begin.argtypes = [P, C.POINTER(EigenState)]
+        L.mocgpu_update_incoming_flux.argtypes = [P]
+        L.mocgpu_sweep_2d3d.argtypes = [P, i32]
+        L.mocgpu_get_corrections.argtypes = [P, P, P]
+        L.mocgpu_correction_residual.argtypes = [P, i32, P]
         L.mocgpu_plan_check.argtypes = [C.POINTER(MocgpuProblem), i32, C.c_char_p, i32]
         L.mocgpu_attenuation.restype = i32
         L.mocgpu_attenuation.argtypes = [P, P, i32]

@@ -157,6 +157,30 @@ class GpuMoCSweeper:
         s = np.ascontiguousarray(surface_flux, dtype=np.float64)
         self._check(gpu().mocgpu_set_currents(self.h, g, ptr(c), ptr(s)))
 
+    # --- 2-D/3-D correction worker (cmdo::MoCSweeper_2D3D) ------------------
+    def sweep_2d3d(self, g: int):
+        """MoCSweeper_2D3D::sweep(g): the last inner sweep tallies the
+        correction factors and the coarse currents of group g."""
+        self._check(gpu().mocgpu_sweep_2d3d(self.h, g))
+
+    def corrections(self):
+        """(alpha [G][4nd][n_cell][2], beta [G][4nd][n_cell])."""
+        p = self.prob.p
+        nang, n_cell = 2 * p.n_azi * p.n_polar, p.nx * p.ny
+        a = np.zeros((self.G, nang, n_cell, 2))
+        b = np.zeros((self.G, nang, n_cell))
+        self._check(gpu().mocgpu_get_corrections(self.h, ptr(a), ptr(b)))
+        return a, b
+
+    def correction_residual(self, g: int) -> np.ndarray:
+        out = np.zeros(3)
+        self._check(gpu().mocgpu_correction_residual(self.h, g, ptr(out)))
+        return out
+
+    def update_incoming_flux(self):
+        """MoCSweeper::update_incoming_flux from the CMFD partial currents."""
+        self._check(gpu().mocgpu_update_incoming_flux(self.h))
+
     # --- CMFD / EigenSolver -------------------------------------------------
     def cmfd_step(self, k: float) -> float:
         """EigenSolver::do_cmfd: returns the CMFD eigenvalue."""

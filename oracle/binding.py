@@ -68,6 +68,10 @@ def lib() -> C.CDLL:
         L.oracle_exp_table_node.argtypes = [i32, i32, f64, f64]
         L.oracle_time_sweeps.argtypes = [P, i32, P]
         L.oracle_bicgstab.argtypes = [i32, P, P, P, P, P, f64, i32, P, P]
+        L.oracle_sweep_2d3d.argtypes = [P, i32]
+        L.oracle_get_corrections.argtypes = [P, P, P]
+        L.oracle_correction_residual.argtypes = [P, i32, P]
+        L.oracle_update_incoming_flux.argtypes = [P]
         _L = L
     return _L
 
@@ -215,6 +219,29 @@ class Oracle:
         rc = lib().oracle_eigen_solve(self.h, k_tol, psi_tol, max_iter, min_iter, C.byref(st), _p(hist),
                                       max_iter)
         return rc, st, hist[: st.iterations]
+
+    def sweep_2d3d(self, g):
+        """MoCSweeper_2D3D::sweep(g): the last inner sweep tallies the 2-D/3-D
+        correction factors (and the coarse currents) of group g."""
+        if lib().oracle_sweep_2d3d(self.h, g) != 0:
+            raise ValueError("group out of range")
+
+    def corrections(self):
+        """(alpha [G][4nd][n_cell][2], beta [G][4nd][n_cell])."""
+        p = self.prob.p
+        n_cell, nang = p.nx * p.ny, 2 * p.n_azi * p.n_polar
+        a = np.zeros((self.G, nang, n_cell, 2))
+        b = np.zeros((self.G, nang, n_cell))
+        lib().oracle_get_corrections(self.h, _p(a), _p(b))
+        return a, b
+
+    def correction_residual(self, g):
+        out = np.zeros(3)
+        lib().oracle_correction_residual(self.h, g, _p(out))
+        return out
+
+    def update_incoming_flux(self):
+        lib().oracle_update_incoming_flux(self.h)
 
     def time_sweeps(self, n):
         out = np.zeros(1)

@@ -83,6 +83,10 @@ struct Oracle {
     std::vector<std::vector<int>> m_rowptr, m_col;
     std::vector<std::vector<double>> m_val;
     int last_cmfd_iters = 0;
+    // cmdo::CorrectionData (correction_data.hpp:45-66): alpha [G][4nd][cell][2]
+    // (initially 0.5), beta [G][4nd][cell] (initially 1); the last residual
+    // triple of each group's correction sweep
+    std::vector<double> alpha, beta, corr_resid;
 
     int angle_face_size(int iang, int face) const { return fsize[iang * 3 + face]; }
 };
@@ -272,11 +276,31 @@ void source_self_scatter(Oracle &o, int g)
     }
 }
 
+void calculate_corrections(Oracle &o, int iang, int g, const std::vector<double> &surf_sum,
+                           const std::vector<double> &vol_sum, const std::vector<double> &sigt_sum,
+                           double resid[3]);
+void homogenize(Oracle &o);
+
 // ---- the sweep: moc_sweeper_kernel.inc.hpp:36-180 with moc::Current --------
-void sweep1g(Oracle &o, int g, bool with_current)
+// worker 0: NoCurrent, 1: moc::Current, 2: cmdo::CurrentCorrections
+// (correction_worker.hpp:36-292), which also tallies the per-angle cell and
+// surface sums of the 2-D/3-D correction factors.
+void sweep1g(Oracle &o, int g, bool with_current, int worker = -1)
 {
+    if (worker < 0) worker = with_current ? 1 : 0;
+    with_current = worker != 0;
     std::vector<double> flux_1g(o.F, 0.0);
     const int n_surf_plane = o.nx * o.ny + (o.nx + 1) * o.ny + (o.ny + 1) * o.nx;
+    const int n_cell = o.nx * o.ny;
+    // CurrentCorrections' per-angle buffers (correction_worker.hpp:50-56)
+    std::vector<double> surf_sum, vol_sum, vol_norm, sigt_sum;
+    double resid[3] = {0.0, 0.0, 0.0};
+    if (worker == 2) {
+        surf_sum.assign(2 * n_surf_plane, 0.0);
+        vol_sum.assign(2 * n_cell, 0.0);
+        vol_norm.assign(n_cell, 0.0);
+        sigt_sum.assign(2 * n_cell, 0.0);
+    }
 #pragma omp parallel default(shared) num_threads(o.o.n_threads > 0 ? o.o.n_threads : omp_get_max_threads())
     {
         std::vector<double> e_tau(o.max_seg), psi1(o.max_seg + 1), psi2(o.max_seg + 1);
@@ -305,6 +329,16 @@ void sweep1g(Oracle &o, int g, bool with_current)
             const double rstheta = o.ang_rsin[iang];
             const double wt_v_st = o.ang_weight[iang] * spacing * o.p.hz * stheta * PI;
             const int r0 = o.azi_ray_off[a], r1 = o.azi_ray_off[a + 1];
+            if (worker == 2) {
+                // CurrentCorrections::set_angle: zero the per-angle sums
+#pragma omp single
+                {
+                    std::fill(surf_sum.begin(), surf_sum.end(), 0.0);
+                    std::fill(vol_sum.begin(), vol_sum.end(), 0.0);
+                    std::fill(vol_norm.begin(), vol_norm.end(), 0.0);
+                    std::fill(sigt_sum.begin(), sigt_sum.end(), 0.0);
+                }
+            }
 #pragma omp for schedule(static, 1)
             for (int ir = r0; ir < r1; ir++) {
                 const int64_t s0 = o.ray_seg_off[ir];
@@ -331,7 +365,64 @@ void sweep1g(Oracle &o, int g, bool with_current)
                     t_flux[ireg] += psi_diff * wt_v_st;
                 }
                 bc_out_2[bc1] = psi2[0];
-                if (with_current) {
+                if (worker == 2) {
+                    // CurrentCorrections::post_ray (correction_worker.hpp:114-221)
+#pragma omp critical
+                    {
+                        auto C = [&](int s) -> double & { return o.cur[(size_t)s * o.G + g]; };
+                        auto S = [&](int s) -> double & { return o.sfl[(size_t)s * o.G + g]; };
+                        int cell_fw = o.ray_cm_cell[2 * ir], cell_bw = o.ray_cm_cell[2 * ir + 1];
+                        int surf_fw = o.ray_cm_surf[2 * ir], surf_bw = o.ray_cm_surf[2 * ir + 1];
+                        int iseg_fw = 0, iseg_bw = nseg;
+                        int norm_fw = surface_normal(o, surf_fw);
+                        int norm_bw = surface_normal(o, surf_bw);
+                        C(surf_fw) += psi1[iseg_fw] * cw[norm_fw];
+                        C(surf_bw) -= psi2[iseg_bw] * cw[norm_bw];
+                        S(surf_fw) += psi1[iseg_fw] * fw[norm_fw];
+                        S(surf_bw) -= psi2[iseg_bw] * fw[norm_bw];  // as the reference's worker does
+                        surf_sum[surf_fw * 2 + 0] += psi1[iseg_fw];
+                        surf_sum[surf_bw * 2 + 1] += psi2[iseg_bw];
+                        for (int c = o.ray_cm_off[ir]; c < o.ray_cm_off[ir + 1]; c++) {
+                            const int *crd = &o.cm_data[4 * c];
+                            if (crd[0] != INVALID) {
+                                for (int i = 0; i < crd[2]; i++) {
+                                    const int ireg = o.seg_fsr[s0 + iseg_fw];
+                                    const double xs = o.xstr[ireg];
+                                    const double t = rstheta * o.seg_len[s0 + iseg_fw];
+                                    const double fluxvol = t * qbar[ireg] + (psi1[iseg_fw] - psi1[iseg_fw + 1]) / xs;
+                                    vol_sum[cell_fw * 2 + 0] += fluxvol;
+                                    vol_norm[cell_fw] += t;
+                                    sigt_sum[cell_fw * 2 + 0] += xs * fluxvol;
+                                    iseg_fw++;
+                                }
+                                norm_fw = surface_to_normal(crd[0]);
+                                surf_fw = coarse_surf(o, cell_fw, crd[0]);
+                                C(surf_fw) += psi1[iseg_fw] * cw[norm_fw];
+                                S(surf_fw) += psi1[iseg_fw] * fw[norm_fw];
+                                surf_sum[surf_fw * 2 + 0] += psi1[iseg_fw];
+                            }
+                            if (crd[1] != INVALID) {
+                                for (int i = 0; i < crd[3]; i++) {
+                                    iseg_bw--;
+                                    const int ireg = o.seg_fsr[s0 + iseg_bw];
+                                    const double xs = o.xstr[ireg];
+                                    const double t = rstheta * o.seg_len[s0 + iseg_bw];
+                                    const double fluxvol =
+                                        t * qbar[ireg] + e_tau[iseg_bw] * (psi2[iseg_bw + 1] - qbar[ireg]) / xs;
+                                    vol_sum[cell_bw * 2 + 1] += fluxvol;
+                                    sigt_sum[cell_bw * 2 + 1] += xs * fluxvol;
+                                }
+                                norm_bw = surface_to_normal(crd[1]);
+                                surf_bw = coarse_surf(o, cell_bw, crd[1]);
+                                C(surf_bw) -= psi2[iseg_bw] * cw[norm_bw];
+                                S(surf_bw) -= psi2[iseg_bw] * fw[norm_bw];
+                                surf_sum[surf_bw * 2 + 1] += psi2[iseg_bw];
+                            }
+                            cell_fw = coarse_neighbor(o, cell_fw, crd[0]);
+                            cell_bw = coarse_neighbor(o, cell_bw, crd[1]);
+                        }
+                    }
+                } else if (with_current) {
                     // moc::Current::post_ray (moc_current_worker.hpp:202-264)
 #pragma omp critical
                     {
@@ -370,6 +461,19 @@ void sweep1g(Oracle &o, int g, bool with_current)
                     }
                 }
             }
+            if (worker == 2) {
+                // CurrentCorrections::post_angle (correction_worker.hpp:239-262)
+#pragma omp single
+                {
+                    for (int i = 0; i < n_cell; i++) {
+                        sigt_sum[2 * i + 0] /= vol_sum[2 * i + 0];
+                        sigt_sum[2 * i + 1] /= vol_sum[2 * i + 1];
+                        vol_sum[2 * i + 0] /= vol_norm[i];
+                        vol_sum[2 * i + 1] /= vol_norm[i];
+                    }
+                    calculate_corrections(o, iang, g, surf_sum, vol_sum, sigt_sum, resid);
+                }
+            }
             if (o.o.gs_boundary) {
 #pragma omp single
                 {
@@ -404,6 +508,64 @@ void sweep1g(Oracle &o, int g, bool with_current)
             }
         }
     }
+    if (worker == 2)
+        for (int k = 0; k < 3; k++) o.corr_resid[(size_t)g * 3 + k] = std::sqrt(resid[k]);
+}
+
+// CurrentCorrections::calculate_corrections (correction_worker.cpp:33-160)
+// for the angle iang of octants 1-2 (forward) and its reverse (backward):
+// alpha_x / alpha_y = cell-average angular flux over the sum of the two
+// x- (y-) face average fluxes, beta = the flux-weighted Sigma_t of the cell
+// over its homogenized Sigma_t (the pin-cell XS mesh, h_tr).
+void calculate_corrections(Oracle &o, int iang, int g, const std::vector<double> &surf_sum,
+                           const std::vector<double> &vol_sum, const std::vector<double> &sigt_sum,
+                           double resid[3])
+{
+    const int n_cell = o.nx * o.ny, nang = 4 * o.nd;
+    const int iang1 = iang, iang2 = reverse_ang(o, iang);
+    const double ox = o.ang_ox[iang];
+    int surfs[2][4];  // [dir][XL, XR, YL, YR]: the moc angles all run +y
+    surfs[0][2] = SOUTH;
+    surfs[0][3] = NORTH;
+    surfs[1][2] = NORTH;
+    surfs[1][3] = SOUTH;
+    if (ox > 0.0) {
+        surfs[0][0] = WEST;
+        surfs[0][1] = EAST;
+        surfs[1][0] = EAST;
+        surfs[1][1] = WEST;
+    } else {
+        surfs[0][0] = EAST;
+        surfs[0][1] = WEST;
+        surfs[1][0] = WEST;
+        surfs[1][1] = EAST;
+    }
+    const double spacing = o.azi_spacing[iang / o.NP];
+    const double alpha_ang = o.ang_alpha[iang];
+    const double area[2] = {std::abs(spacing / std::cos(alpha_ang)), std::abs(spacing / std::sin(alpha_ang))};
+    for (int ic = 0; ic < n_cell; ic++) {
+        const double area_x = area[0] / dx_of(o, ic % o.nx);
+        const double area_y = area[1] / dy_of(o, ic / o.nx);
+        const double xs = o.h_tr[(size_t)ic * o.G + g];
+        for (int d = 0; d < 2; d++) {
+            const int ia = d == 0 ? iang1 : iang2;
+            auto ss = [&](int k) { return surf_sum[coarse_surf(o, ic, surfs[d][k]) * 2 + d]; };
+            const double psi_xl = ss(0) * area_x, psi_xr = ss(1) * area_x;
+            const double psi_yl = ss(2) * area_y, psi_yr = ss(3) * area_y;
+            const double ax = vol_sum[ic * 2 + d] / (psi_xl + psi_xr);
+            const double ay = vol_sum[ic * 2 + d] / (psi_yl + psi_yr);
+            const double b = sigt_sum[ic * 2 + d] / xs;
+            double &al_x = o.alpha[(((size_t)g * nang + ia) * n_cell + ic) * 2 + 0];
+            double &al_y = o.alpha[(((size_t)g * nang + ia) * n_cell + ic) * 2 + 1];
+            double &be = o.beta[((size_t)g * nang + ia) * n_cell + ic];
+            resid[0] += (ax - al_x) * (ax - al_x);
+            resid[1] += (ay - al_y) * (ay - al_y);
+            resid[2] += (b - be) * (b - be);
+            al_x = ax;
+            al_y = ay;
+            be = b;
+        }
+    }
 }
 
 // MoCSweeper::sweep (moc_sweeper.cpp:189-225)
@@ -425,6 +587,63 @@ void mocsweep(Oracle &o, int g, bool with_current)
             sweep1g(o, g, false);
         }
     }
+}
+
+// MoCSweeper_2D3D::sweep (moc_sweeper_2d3d.cpp:48-99): the inner iterations
+// of MoCSweeper::sweep, the last one with the CurrentCorrections worker after
+// the pin-cell cross sections are homogenized from the current flux.
+void mocsweep_2d3d(Oracle &o, int g)
+{
+    for (int f = 0; f < o.F; f++) o.xstr[f] = o.xs_tr[o.fsr_mat[f] * o.G + g];
+    const int n_surf_plane = o.nx * o.ny + (o.nx + 1) * o.ny + (o.ny + 1) * o.nx;
+    for (int inner = 0; inner < o.o.n_inner; inner++) {
+        source_self_scatter(o, g);
+        if (inner == o.o.n_inner - 1) {
+            for (int s = o.nx * o.ny; s < n_surf_plane; s++) {
+                o.cur[(size_t)s * o.G + g] = 0.0;
+                o.sfl[(size_t)s * o.G + g] = 0.0;
+            }
+            homogenize(o);
+            sweep1g(o, g, true, 2);
+            o.has_radial = true;
+        } else {
+            sweep1g(o, g, false, 0);
+        }
+    }
+}
+
+// MoCSweeper::update_incoming_flux (moc_sweeper.cpp:255-291) through
+// update_incoming_generic (moc_sweeper.hpp:198-233): the incoming flux at
+// each ray's entry point scaled by the ratio of the CMFD partial currents of
+// the latest solve to those of the previous one (or set from the latest
+// ones after the first solve). The reference indexes the partial currents
+// with the ray's first coarse cell (cm_cell_fw / cm_cell_bw), restated as is.
+void update_incoming_flux(Oracle &o)
+{
+    const int G = o.G;
+    auto part = [&](const std::vector<double> &p, int is, int g) {
+        return 2.0 * (p[((size_t)is * G + g) * 2 + 0] + p[((size_t)is * G + g) * 2 + 1]);
+    };
+    auto update = [&](double in, int is, int g) {
+        if (o.has_old_partial) {
+            const double pn = part(o.part, is, g), po = part(o.part_old, is, g);
+            return po > 0.0 ? in * (pn / po) : in;
+        }
+        return (2.0 / FPI) * (o.part[((size_t)is * G + g) * 2 + 0] + o.part[((size_t)is * G + g) * 2 + 1]);
+    };
+    for (int g = 0; g < G; g++)
+        for (int iang = 0; iang < 2 * o.nd; iang++) {
+            const int a = iang / o.NP;
+            const int iang2 = reverse_ang(o, iang);
+            double *bc_fw = &o.bc_in[(size_t)g * o.bcpg + o.off[iang * 3]];
+            double *bc_bw = &o.bc_in[(size_t)g * o.bcpg + o.off[iang2 * 3]];
+            for (int ir = o.azi_ray_off[a]; ir < o.azi_ray_off[a + 1]; ir++) {
+                const int is1 = o.ray_cm_cell[2 * ir], is2 = o.ray_cm_cell[2 * ir + 1];
+                const int bc1 = o.ray_bc[2 * ir], bc2 = o.ray_bc[2 * ir + 1];
+                bc_fw[bc1] = update(bc_fw[bc1], is1, g);
+                bc_bw[bc2] = update(bc_bw[bc2], is2, g);
+            }
+        }
 }
 
 // TransportSweeper::calc_fission_source (transport_sweeper.cpp:119-134)
@@ -1031,6 +1250,9 @@ int oracle_create(const mocgpu_problem *pp, const oracle_options *opt, oracle_ha
         o.sfl.assign((size_t)o.n_surf * o.G, 0.0);
         o.part.assign((size_t)o.n_surf * o.G * 2, 0.0);
         o.part_old = o.part;
+        o.alpha.assign((size_t)o.G * 4 * o.nd * o.nx * o.ny * 2, 0.5);
+        o.beta.assign((size_t)o.G * 4 * o.nd * o.nx * o.ny, 1.0);
+        o.corr_resid.assign((size_t)o.G * 3, 0.0);
         o.cflux.assign((size_t)o.n_cell * o.G, 1.0);
         h->es.n_fissile = count_fissile(o);
         *out = h;
@@ -1288,6 +1510,33 @@ int oracle_bicgstab(int32_t n, const int32_t *row_ptr, const int32_t *col, const
     std::vector<double> vl(val, val + row_ptr[n]);
     int it = bicgstab(n, rp, cl, vl, b, x, tol, max_iter, err);
     if (iters) *iters = it;
+    return 0;
+}
+
+int oracle_sweep_2d3d(oracle_handle *h, int32_t g)
+{
+    if (g < 0 || g >= h->o.G) return -1;
+    mocsweep_2d3d(h->o, g);
+    return 0;
+}
+
+int oracle_get_corrections(oracle_handle *h, double *alpha, double *beta)
+{
+    if (alpha) std::copy(h->o.alpha.begin(), h->o.alpha.end(), alpha);
+    if (beta) std::copy(h->o.beta.begin(), h->o.beta.end(), beta);
+    return 0;
+}
+
+int oracle_correction_residual(oracle_handle *h, int32_t g, double *out)
+{
+    if (g < 0 || g >= h->o.G) return -1;
+    for (int k = 0; k < 3; k++) out[k] = h->o.corr_resid[(size_t)g * 3 + k];
+    return 0;
+}
+
+int oracle_update_incoming_flux(oracle_handle *h)
+{
+    update_incoming_flux(h->o);
     return 0;
 }
 

@@ -63,6 +63,17 @@ int oracle_eigen_solve(oracle_handle *h, double k_tol, double psi_tol, int32_t m
                        int32_t min_iter, mocgpu_eigen_state *st, double *k_hist, int32_t hist_cap);
 int oracle_eigen_iterate(oracle_handle *h, mocgpu_eigen_state *st);
 
+/* MoCSweeper_2D3D::sweep(g) (moc_sweeper_2d3d.cpp:48-99): the last inner
+ * sweep runs the cmdo::CurrentCorrections worker (correction_worker.hpp,
+ * correction_worker.cpp:33-160), storing alpha [G][4nd][cell][2] and beta
+ * [G][4nd][cell] (CorrectionData order, correction_data.hpp:45-100) and the
+ * group's residual triple (alpha_x, alpha_y, beta). */
+int oracle_sweep_2d3d(oracle_handle *h, int32_t g);
+int oracle_get_corrections(oracle_handle *h, double *alpha, double *beta);
+int oracle_correction_residual(oracle_handle *h, int32_t g, double *out);
+/* MoCSweeper::update_incoming_flux (moc_sweeper.cpp:255-291). */
+int oracle_update_incoming_flux(oracle_handle *h);
+
 /* Exponential_Linear<N> on [xmin, xmax] (src/core/exponential.hpp:56-112). */
 double oracle_exp_table(double v, int32_t n, double xmin, double xmax);
 double oracle_exp_table_node(int32_t i, int32_t n, double xmin, double xmax);

@@ -424,3 +424,114 @@ def test_cmfd_matrix_structure(pins3x3):
         assert (np.diag(A) > 0).all()
         off = A - np.diag(np.diag(A))
         assert (off <= 0).all()
+
+
+# --- 2-D/3-D correction worker and incoming-flux update -------------------------
+
+def _ihm_flat(n_inner=1):
+    hp = HostProblem.from_string(IHM_XML.format(n_inner=n_inner), DECKS)
+    G = hp.n_group
+    tr = hp.array("xs_tr", G)
+    sc = hp.array("xs_sc", G * G).reshape(G, G)
+    ch = hp.array("xs_ch", G)
+    nf = hp.array("xs_nf", G)
+    phi = np.linalg.solve(np.diag(tr) - sc, ch)
+    return hp, phi, float(nf @ phi)
+
+
+def _flat_state(o, hp, phi, k):
+    o.initialize()
+    o.set_flux(np.tile(phi, (hp.n_fsr, 1)))
+    o.calc_fission_source(k)
+    bc = o.boundary()
+    o.set_boundary(np.repeat(phi / (4 * np.pi), bc.size // hp.n_group))
+
+
+def test_corrections_flat_flux_closed_form():
+    """cmdo::CurrentCorrections on the infinite medium at its exact solution
+    (isotropic, flat angular flux q): every cell-average and face-average
+    angular flux equals q, so alpha_x = alpha_y = 1/2 and beta = 1 for every
+    cell, angle and group (correction_worker.cpp:93-153) -- the diamond-
+    difference values CorrectionData starts from. The modular rays make the
+    face areas exact: each pin face is crossed by a whole number of rays."""
+    hp, phi, k = _ihm_flat()
+    o = Oracle(hp, n_inner=1)
+    _flat_state(o, hp, phi, k)
+    for g in range(hp.n_group):
+        o.build_source(g)
+        o.sweep_2d3d(g)
+    a, b = o.corrections()
+    assert a.shape == (hp.n_group, 2 * hp.p.n_azi * hp.p.n_polar, hp.p.nx * hp.p.ny, 2)
+    np.testing.assert_allclose(a, 0.5, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(b, 1.0, rtol=0, atol=1e-13)
+    # the flux was already the fixed point: the sweep left it in place
+    np.testing.assert_allclose(o.flux(), np.tile(phi, (hp.n_fsr, 1)), rtol=1e-12, atol=0)
+
+
+def test_corrections_heterogeneous(deck_dir):
+    """On a 4x4 lattice with moderator-only pins and vacuum east/south faces:
+    beta is exactly 1 in the single-material cells (the flux-weighted
+    Sigma_t of a uniform cell is its Sigma_t, which is also its homogenized
+    value) and differs from 1 in fuel cells; the coarse currents of the 2D3D
+    sweep equal those of the plain sweep with currents from the same state
+    (the same angular fluxes at the same crossings); a second sweep moves the
+    factors less than the first (residuals shrink)."""
+    from mocc_amd import lattice_gen as lg
+    d = lg.pin_lattice(4, 2, 8, 0.05, "corr4x4", bc={"east": "vacuum", "south": "vacuum"})
+    rows = d.lattices[1][2]
+    rows[1][2] = rows[2][1] = 2  # moderator-only pins
+    hp = HostProblem.load(lg.write_deck(d, deck_dir, 7))
+    G = hp.n_group
+    o = Oracle(hp, n_inner=3)
+    o.initialize()
+    o.calc_fission_source(1.0)
+    ref = Oracle(hp, n_inner=3)
+    ref.initialize()
+    ref.calc_fission_source(1.0)
+    for g in range(G):
+        o.build_source(g)
+        o.sweep_2d3d(g)
+        ref.build_source(g)
+        ref.sweep(g, with_current=True)
+    for g in range(G):
+        c2, _ = o.currents(g)
+        c1, _ = ref.currents(g)
+        np.testing.assert_allclose(c2, c1, rtol=1e-12, atol=1e-14 * np.abs(c1).max())
+    a, b = o.corrections()
+    mat = hp.array("fsr_mat", hp.n_fsr, np.int64)
+    off = hp.array("pin_fsr_off", 17, np.int64)
+    cell = hp.array("pin_cell", 16, np.int64)
+    uniform = [int(cell[ip]) for ip in range(16) if len(set(mat[off[ip]:off[ip + 1]])) == 1]
+    mixed = [c for c in range(16) if c not in uniform]
+    assert len(uniform) == 2
+    np.testing.assert_allclose(b[:, :, uniform], 1.0, rtol=0, atol=1e-12)
+    assert np.abs(b[:, :, mixed] - 1.0).max() > 1e-3
+    # alpha grows past 1 in cells on the vacuum faces, where the entering face flux is zero
+    assert np.all(np.isfinite(a)) and a.min() > 0.0
+    r1 = [o.correction_residual(g) for g in range(G)]
+    for g in range(G):
+        o.build_source(g)
+        o.sweep_2d3d(g)
+    r2 = [o.correction_residual(g) for g in range(G)]
+    assert all((r2[g] < r1[g]).all() for g in range(G)), (r1, r2)
+
+
+def test_update_incoming_flux_flat(deck_dir):
+    """MoCSweeper::update_incoming_flux (moc_sweeper.cpp:255-291) after CMFD
+    solves on the infinite medium at its exact solution: the CMFD surface
+    flux is the scalar flux and the net current zero, so the incoming flux
+    set from the first solve's partial currents, 2/(4 pi) (J+ + J-) =
+    phi/(4 pi), and the ratio of a later solve's to the earlier one's (1)
+    both leave the exact angular flux in place."""
+    hp, phi, k = _ihm_flat()
+    o = Oracle(hp, n_inner=1, use_cmfd=True, cmfd_k_tol=1e-12, cmfd_psi_tol=1e-12,
+               cmfd_resid_reduction=1e-10, cmfd_max_iter=200)
+    _flat_state(o, hp, phi, k)
+    for g in range(hp.n_group):
+        o.build_source(g)
+        o.sweep(g, with_current=True)
+    bc0 = o.boundary()
+    for _ in range(2):
+        o.cmfd_step(k)
+        o.update_incoming_flux()
+        np.testing.assert_allclose(o.boundary(), bc0, rtol=1e-10, atol=0)
