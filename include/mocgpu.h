@@ -255,7 +255,8 @@ int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current,
 /* CMFD::solve (src/core/cmfd.cpp:171-279) on the device in FP64, with the
  * coarse flux taken from the current fine flux (EigenSolver::do_cmfd,
  * eigen_solver.cpp:263-291: get_pin_flux, solve, set_pin_flux). The coarse
- * mesh may hold at most 3072 pin cells. A non-positive coarse flux returns
+ * mesh may hold at most 6144 pin cells (3072 in a batch, mocgpu_batch_*).
+ * A non-positive coarse flux returns
  * MOCGPU_ERR_NONPOS_FLUX (MoCSweeper::set_pin_flux_1g throws there). */
 int mocgpu_cmfd_step(mocgpu_handle *h, double *k);
 

@@ -490,6 +490,11 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
             b->err = "batched problems sweep Jacobi in energy: create the handle with energy_mode = 1";
             return (int)MOCGPU_ERR_UNSUPPORTED;
         }
+        if (h->opt.use_cmfd && h->cmfd && h->cmfd->n_cell() > CmfdSolver::kMaxBatchCells) {
+            b->err = "a batched CMFD problem holds at most " + std::to_string(CmfdSolver::kMaxBatchCells) +
+                     " coarse cells (this one has " + std::to_string(h->cmfd->n_cell()) + ")";
+            return (int)MOCGPU_ERR_UNSUPPORTED;
+        }
         if (!b->cases.empty() && h->NP != b->NP) {
             b->err = "all problems of a batch must have the same number of polar angles";
             return (int)MOCGPU_ERR_UNSUPPORTED;

@@ -9,12 +9,14 @@
 //     (cell, group) assembles the 5-point row;
 //   * the coarse power iteration -- fission source, Gauss-Seidel sweep over
 //     groups, one BiCGSTAB solve per group, the k update and the convergence
-//     test -- runs in ONE workgroup of kCmfdThreads lanes. Each lane owns
-//     K cells (c = lane + j * kCmfdThreads) and keeps their BiCGSTAB vectors
-//     in registers; a matrix-vector product stages the operand in LDS, and
-//     dot products are wave butterflies plus a fixed-order sum over the
-//     waves, so every lane sees the same scalars and the iteration is
-//     deterministic. No host round trip happens inside a CMFD solve.
+//     test -- runs in ONE workgroup: up to 512 lanes for meshes of up to 1024
+//     cells (and for every problem of a batch), and the wide solve (512 or
+//     1024 lanes) for a single problem's larger mesh. Each lane owns K cells
+//     (c = lane + j * lanes) and keeps their BiCGSTAB vectors in registers; a
+//     matrix-vector product stages the operand in LDS, and dot products are
+//     wave butterflies plus a fixed-order sum over the waves, so every lane
+//     sees the same scalars and the iteration is deterministic. No host round
+//     trip happens inside a CMFD solve.
 #include "cmfd.hpp"
 
 #include <algorithm>
@@ -56,6 +58,20 @@ inline size_t cmfd_lds_bytes(int nc)
 {
     if (cmfd_rows_in_lds(nc)) return (size_t)kCmfdLdsVectorsRL * nc * sizeof(double);
     return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * cmfd_threads(nc) * sizeof(double);
+}
+// Meshes of more than 1024 cells run the wide solve (k_cmfd_solve_wide): on
+// 512 lanes with the rows in registers up to 3072 cells (3 to 6 cells per
+// lane), on 1024 lanes re-reading the rows from cache beyond (4 to 6). Its LDS
+// holds the two operand banks with a pad of nx entries at each end; its
+// global scratch holds two vectors of K * lanes.
+constexpr int kWideCells = 6144;
+static_assert(kWideCells == CmfdSolver::kMaxCells, "the wide solve covers every single-problem mesh");
+inline bool cmfd_wide(int nc) { return nc > 1024; }
+inline int cmfd_wide_lanes(int nc) { return nc <= 3072 ? 512 : 1024; }
+inline int cmfd_wide_k(int nc) { return (nc + cmfd_wide_lanes(nc) - 1) / cmfd_wide_lanes(nc); }
+inline size_t cmfd_wide_lds_bytes(int nc, int nx)
+{
+    return (2 * (size_t)cmfd_wide_k(nc) * cmfd_wide_lanes(nc) + 2 * (size_t)nx) * sizeof(double);
 }
 enum { EAST = 0, NORTH = 1, WEST = 2, SOUTH = 3, TOP = 4, BOTTOM = 5 };
 
@@ -864,6 +880,431 @@ template <int K, int NT, bool RL = false> __global__ __launch_bounds__(NT) void 
     cmfd_solve<K, NT, RL>(A, lds, red);
 }
 
+// --- the coarse solve of meshes over 1024 cells -------------------------------
+//
+// One workgroup of NT lanes (NT / 64 waves), K cells per lane: lane l owns
+// cells l + k * NT. What a lane keeps where:
+//   * registers: the BiCGSTAB vectors x, r, p, v, t, the inverse diagonal and
+//     the shadow residual of its cells, their coarse fission source and, on
+//     512 lanes, their five row entries;
+//   * LDS: [nx pad | operand bank 0 | operand bank 1 | nx pad];
+//   * global scratch, touched once per group solve or power iteration: the
+//     right-hand side and the previous coarse fission source.
+// What bounds an iteration here is LDS traffic and the chain of barriers
+// (three workgroup sums and two products), so a product reads only the
+// north and south neighbours from LDS; the west and east neighbours (lanes
+// l - 1 and l + 1) move across the wave by DPP, and only a wave's first and
+// last lanes read theirs from LDS. Slots past the mesh are dummy cells with
+// zero rows, zero vector entries and inverse diagonal 1, so no step branches on
+// them. The pads hold zeros, so every neighbour read stays inside the
+// allocation without clamps: a missing neighbour has a zero coefficient
+// (assemble_item writes it so), and what it multiplies -- a pad zero or a
+// finite entry of the other bank or of another cell -- adds an exact zero.
+// The five products of a row are summed in the order south, west, diagonal,
+// east, north, as in the 512-lane solve above, so every row's product is the
+// same there and here; only the partition of the dot products differs.
+// Where a lane keeps its cells' rows: in registers with 512 lanes (up to 6
+// cells per lane: the 256-VGPR budget of two waves per SIMD holds them beside
+// the vectors), or nowhere with 1024 lanes (every product reads them from
+// cache).
+template <int NT, int K> constexpr bool wide_rows_in_regs() { return NT <= 512 && K <= 6; }
+
+template <int NT, int K> struct WideRows {
+    double a[wide_rows_in_regs<NT, K>() ? 5 : 1][wide_rows_in_regs<NT, K>() ? K : 1];  // south, west, diagonal, east, north
+    const double *cf;  // coefficients of group g: entry j of cell c at cf[j * G * nc + c]
+};
+
+// A DPP move whose pattern gives every lane a source lane (quad_perm,
+// row_half_mirror, row_mirror): no old value to initialize.
+template <int CTRL> __device__ __forceinline__ double dpp_all(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave shifts by one lane (lane i gets lane i - 1, or i + 1); the lane left
+// without a source (0, or 63) gets an undefined value that the caller replaces.
+__device__ __forceinline__ double dpp_wave_shr1(double v) { return dpp_all<0x138>(v); }
+__device__ __forceinline__ double dpp_wave_shl1(double v) { return dpp_all<0x130>(v); }
+
+// wave_sum() above, term for term (the same value), with the four in-row
+// steps moved without an old value.
+__device__ __forceinline__ double wide_wave_sum(double v)
+{
+    v += dpp_all<0xb1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_all<0x4e>(v);        // quad_perm [2,3,0,1]
+    v += dpp_all<0x141>(v);       // row_half_mirror
+    v += dpp_all<0x140>(v);       // row_mirror
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// The sum of a row of 16 lanes in every lane of the row.
+__device__ __forceinline__ double row_sum16(double v)
+{
+    v += dpp_all<0xb1>(v);
+    v += dpp_all<0x4e>(v);
+    v += dpp_all<0x141>(v);
+    v += dpp_all<0x140>(v);
+    return v;
+}
+__device__ __forceinline__ double read_lane_f64(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Workgroup sums. Lane 0 of each wave publishes its wave's sum (in one of two
+// banks of 32 doubles, alternating, so each sum needs one barrier); every
+// wave then reads the partials into lanes 0-15 (the second sum of a pair into
+// lanes 16-31) and adds them by the same DPP tree, so every lane of every
+// wave gets the same value, independent of timing.
+constexpr int kWideRedBank = 32;
+template <int NT> struct WideScratch {
+    double *red;  // [2][kWideRedBank]
+    double *buf;  // operand bank 0 (bank 1 follows it)
+    int rb, bb;
+};
+template <int NT> __device__ __forceinline__ double wide_block_sum(double v, WideScratch<NT> &S)
+{
+    static_assert(NT / 64 <= 16, "one row of 16 lanes holds the wave partials");
+    v = wide_wave_sum(v);
+    double *red = S.red + S.rb * kWideRedBank;
+    S.rb ^= 1;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const int l = threadIdx.x & 63;
+    double t = l < NT / 64 ? red[l] : 0.0;
+    t = row_sum16(t);
+    return read_lane_f64(t, 0);
+}
+template <int NT>
+__device__ __forceinline__ void wide_block_sum2(double a, double b, WideScratch<NT> &S, double &sa, double &sb)
+{
+    a = wide_wave_sum(a);
+    b = wide_wave_sum(b);
+    double *red = S.red + S.rb * kWideRedBank;
+    S.rb ^= 1;
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = a;
+        red[16 + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63;
+    double t = (l & 15) < NT / 64 && l < 32 ? red[l] : 0.0;
+    t = row_sum16(t);
+    sa = read_lane_f64(t, 0);
+    sb = read_lane_f64(t, 16);
+}
+
+// The lane index through an opaque move: addresses formed from it are formed
+// where they are used, not hoisted out of the solve loop (where they would hold
+// registers that the BiCGSTAB vectors need).
+__device__ __forceinline__ int opaque_lane()
+{
+    int l = threadIdx.x;
+    __asm__ volatile("" : "+v"(l));
+    return l;
+}
+
+// The group's rows for this lane's cells, and their inverse diagonal (1 where
+// the diagonal is 0, as at the dummies).
+template <int NT, int K>
+__device__ __forceinline__ void wide_load_rows(const SolveArgs &A, int g, const Lane<K> &L, WideRows<NT, K> &R,
+                                               double (&inv)[K])
+{
+    const size_t pl = (size_t)A.G * A.nc;
+    R.cf = A.coef + (size_t)g * A.nc;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int cl = L.on[k] ? L.c[k] : 0;
+        const double dg = L.on[k] ? R.cf[2 * pl + cl] : 0.0;
+        inv[k] = (dg != 0.0) ? 1.0 / dg : 1.0;
+        if constexpr (wide_rows_in_regs<NT, K>()) {
+            R.a[0][k] = L.on[k] ? R.cf[0 * pl + cl] : 0.0;
+            R.a[1][k] = L.on[k] ? R.cf[1 * pl + cl] : 0.0;
+            R.a[2][k] = dg;
+            R.a[3][k] = L.on[k] ? R.cf[3 * pl + cl] : 0.0;
+            R.a[4][k] = L.on[k] ? R.cf[4 * pl + cl] : 0.0;
+        }
+    }
+}
+
+// v = A y for group g (y and v may be the same array); returns the operand
+// bank, where y stays readable until the second product after this one.
+template <int NT, int K>
+__device__ __forceinline__ const double *wide_spmv(const SolveArgs &A, const WideRows<NT, K> &R, const Lane<K> &L,
+                                                   const double (&y)[K], double (&v)[K], WideScratch<NT> &S)
+{
+    constexpr int KN = K * NT;
+    double *buf = S.buf + S.bb * KN;
+    S.bb ^= 1;
+    const int l = opaque_lane();
+    double *own = buf + l;
+#pragma unroll
+    for (int k = 0; k < K; k++) own[k * NT] = y[k];
+    __syncthreads();
+    const double *south = own - A.nx, *north = own + A.nx;
+    const size_t pl = (size_t)A.G * A.nc;
+    const int wl = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        // west and east neighbours (cells c - 1 and c + 1 are lanes l - 1 and
+        // l + 1): moved across the wave by DPP, read from LDS only by the
+        // wave's first and last lanes, whose neighbours sit in other waves
+        double yw = dpp_wave_shr1(y[k]), ye = dpp_wave_shl1(y[k]);
+        if (wl == 0) {
+            __asm__ volatile("" ::: "memory");  // keeps the read in the branch
+            yw = own[k * NT - 1];
+        }
+        if (wl == 63) {
+            __asm__ volatile("" ::: "memory");
+            ye = own[k * NT + 1];
+        }
+        double a0, a1, a2, a3, a4;
+        if constexpr (wide_rows_in_regs<NT, K>()) {
+            a0 = R.a[0][k];
+            a1 = R.a[1][k];
+            a2 = R.a[2][k];
+            a3 = R.a[3][k];
+            a4 = R.a[4][k];
+        } else {
+            const int c = l + k * NT;
+            const bool on = c < A.nc;
+            const double *cf = R.cf + (on ? c : 0);
+            a0 = on ? cf[0 * pl] : 0.0;
+            a1 = on ? cf[1 * pl] : 0.0;
+            a2 = on ? cf[2 * pl] : 0.0;
+            a3 = on ? cf[3 * pl] : 0.0;
+            a4 = on ? cf[4 * pl] : 0.0;
+        }
+        double s = 0.0;
+        s = fma(a0, south[k * NT], s);
+        s = fma(a1, yw, s);
+        s = fma(a2, y[k], s);
+        s = fma(a3, ye, s);
+        s = fma(a4, north[k * NT], s);
+        v[k] = s;
+    }
+    return buf;
+}
+
+// The Jacobi-preconditioned BiCGSTAB of bicgstab() above, step for step, with
+// the inverse diagonal and the shadow residual in registers and the
+// right-hand side sb in global scratch (read at the start and at a restart).
+template <int NT, int K>
+__device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const Lane<K> &L, const double (&inv)[K],
+                             double (&x)[K], double tol, int max_iter, WideScratch<NT> &S, const double *sb)
+{
+    double r[K], p[K], v[K], t[K], r0[K];
+    wide_spmv<NT, K>(A, R, L, x, t, S);
+    double lr0 = 0.0, lb = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const double bk = sb[L.c[k]];
+        r[k] = bk - t[k];
+        r0[k] = r[k];
+        p[k] = 0.0;
+        v[k] = 0.0;
+        lr0 = fma(r[k], r[k], lr0);
+        lb = fma(bk, bk, lb);
+    }
+    double r0_sq, rhs_sq;
+    wide_block_sum2(lr0, lb, S, r0_sq, rhs_sq);
+    if (rhs_sq == 0.0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) x[k] = 0.0;
+        return 0;
+    }
+    double rho = 1.0, alpha = 1.0, w = 1.0;
+    const double tol2 = tol * tol * rhs_sq;
+    const double eps = 2.220446049250313e-16;
+    const double eps2 = eps * eps;
+    int i = 0, restarts = 0;
+    while (true) {
+        double lrr = 0.0, lrho = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            lrr = fma(r[k], r[k], lrr);
+            lrho = fma(r0[k], r[k], lrho);
+        }
+        double rr, rho_new;
+        wide_block_sum2(lrr, lrho, S, rr, rho_new);
+        if (!(rr > tol2 && i < max_iter)) break;
+        const double rho_old = rho;
+        rho = rho_new;
+        if (fabs(rho) < eps2 * r0_sq) {
+            wide_spmv<NT, K>(A, R, L, x, t, S);
+            const double *bl = sb + opaque_lane();
+            double l = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                r[k] = bl[k * NT] - t[k];
+                r0[k] = r[k];
+                l = fma(r[k], r[k], l);
+            }
+            rho = r0_sq = wide_block_sum(l, S);
+            if (restarts++ == 0) i = 0;
+        }
+        const double beta = (rho / rho_old) * (alpha / w);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            p[k] = fma(beta, fma(-w, v[k], p[k]), r[k]);
+            t[k] = inv[k] * p[k];  // y
+        }
+        wide_spmv<NT, K>(A, R, L, t, v, S);  // v = A y; t keeps y
+        double l = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) l = fma(r0[k], v[k], l);
+        alpha = rho / wide_block_sum(l, S);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            x[k] = fma(alpha, t[k], x[k]);
+            r[k] = fma(-alpha, v[k], r[k]);  // s
+            t[k] = inv[k] * r[k];            // z
+        }
+        const double *zb = wide_spmv<NT, K>(A, R, L, t, t, S) + opaque_lane();  // t = A z
+        double ltt = 0.0, lts = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            ltt = fma(t[k], t[k], ltt);
+            lts = fma(t[k], r[k], lts);
+        }
+        double tt, ts;
+        wide_block_sum2(ltt, lts, S, tt, ts);
+        w = (tt > 0.0) ? ts / tt : 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            x[k] = fma(w, zb[k * NT], x[k]);
+            r[k] = fma(-w, t[k], r[k]);
+        }
+        ++i;
+    }
+    return i;
+}
+
+// CMFD::solve (cmfd.cpp:171-279) as cmfd_solve() above, by NT lanes. lds
+// holds nx + 2 K NT + nx doubles (the padded operand banks); A.gscr holds the
+// previous coarse fission source and the right-hand side, K NT entries each.
+template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const SolveArgs &A, double *lds, double *red)
+{
+    constexpr int KN = K * NT;
+    WideScratch<NT> S{red, lds + A.nx, 0, 0};
+    // the pads: zeros, written once
+    for (int i = threadIdx.x; i < A.nx; i += NT) {
+        lds[i] = 0.0;
+        lds[A.nx + 2 * KN + i] = 0.0;
+    }
+    WideRows<NT, K> R;
+    double *fs_old = A.gscr;
+    double *sb = A.gscr + KN;
+    Lane<K> L;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        L.c[k] = threadIdx.x + k * NT;
+        L.ix[k] = 0;  // unused: the rows carry zeros for missing neighbours
+        L.on[k] = L.c[k] < A.nc;
+    }
+    const int G = A.G;
+    double k_eff = A.k[0];
+    double fsc[K], b[K], x[K], t[K], inv[K];
+    double tfis = wide_block_sum(fission_source<K>(A, L, k_eff, fsc), S);
+    double r0 = 0.0;
+    for (int g = 0; g < G; g++) {
+        cmfd_source<K>(A, g, L, fsc, b);
+        wide_load_rows<NT, K>(A, g, L, R, inv);
+#pragma unroll
+        for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * G + g] : 0.0;
+        wide_spmv<NT, K>(A, R, L, x, t, S);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const double d = t[k] - b[k];
+            s = fma(d, d, s);
+        }
+        r0 += wide_block_sum(s, S);
+    }
+    r0 = sqrt(r0) / (G * A.nc);
+    const double tol = A.rr * r0;
+    int iter = 0, n_bicg = 0;
+    while (true) {
+        iter++;
+#pragma unroll
+        for (int k = 0; k < K; k++) fs_old[L.c[k]] = fsc[k];
+        fission_source<K>(A, L, k_eff, fsc);
+        const double tfis_old = tfis;
+        double ri = 0.0;
+        for (int g = 0; g < G; g++) {
+            cmfd_source<K>(A, g, L, fsc, b);
+            wide_load_rows<NT, K>(A, g, L, R, inv);
+#pragma unroll
+            for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * G + g] : 0.0;
+            wide_spmv<NT, K>(A, R, L, x, t, S);
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double d = t[k] - b[k];
+                s = fma(d, d, s);
+                sb[L.c[k]] = b[k];  // 0 at dummies
+            }
+            ri += wide_block_sum(s, S);
+            n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, tol, 150, S, sb);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
+        }
+        ri = sqrt(ri) / (A.nc * G);
+        double lf = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (L.on[k])
+                for (int g = 0; g < G; g++) lf += A.hnf[(size_t)L.c[k] * G + g] * A.cflux[(size_t)L.c[k] * G + g];
+        tfis = wide_block_sum(lf, S);
+        const double k_old = k_eff;
+        k_eff = k_eff * tfis / tfis_old;
+        double le = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (L.on[k]) le += (fsc[k] - fs_old[L.c[k]]) * (fsc[k] - fs_old[L.c[k]]);
+        const double psi_err = sqrt(wide_block_sum(le, S));
+        if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
+            break;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (L.on[k])
+            for (int g = 0; g < G; g++) {
+                double &f = A.cflux[(size_t)L.c[k] * G + g];
+                if (f < 0.0) f = -f;
+            }
+    if (threadIdx.x == 0) {
+        A.k[0] = k_eff;
+        A.info[0] = iter;
+        A.info[2] = n_bicg;
+    }
+}
+
+template <int NT, int K> __global__ __launch_bounds__(NT) void k_cmfd_solve_wide(SolveArgs A)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[2 * kWideRedBank];
+    cmfd_solve_wide<NT, K>(A, lds, red);
+}
+
+// Launch the wide solve at k cells per lane (K <= k <= KMAX).
+template <int NT, int K, int KMAX> void launch_wide(int k, const SolveArgs &a, size_t lds, hipStream_t s)
+{
+    if (k == K)
+        hipLaunchKernelGGL((k_cmfd_solve_wide<NT, K>), dim3(1), dim3(NT), lds, s, a);
+    else if constexpr (K < KMAX)
+        launch_wide<NT, K + 1, KMAX>(k, a, lds, s);
+}
+
 inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 template <typename T> void dalloc(T *&p, size_t n)
@@ -908,11 +1349,12 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
         if (bc_[i] != MOCGPU_BC_VACUUM && bc_[i] != MOCGPU_BC_REFLECT)
             throw std::runtime_error("CMFD supports vacuum and reflective boundaries only");
     {
-        // the solve keeps five cell vectors in LDS (k_cmfd_solve)
+        // the solve keeps its operand banks (and rows or vectors) in LDS
         int dev = 0, lds_max = 0;
         CMFD_CHECK(hipGetDevice(&dev));
         CMFD_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t need = cmfd_lds_bytes(nc_) + 2 * red_bank<kCmfdThreads>() * sizeof(double);
+        const size_t need = cmfd_wide(nc_) ? cmfd_wide_lds_bytes(nc_, nx_) + 2 * 32 * sizeof(double)
+                                           : cmfd_lds_bytes(nc_) + 2 * red_bank<kCmfdThreads>() * sizeof(double);
         if (need > (size_t)lds_max)
             throw std::runtime_error("CMFD mesh of " + std::to_string(nc_) + " cells needs " + std::to_string(need) +
                                      " bytes of LDS per workgroup; the device allows " + std::to_string(lds_max));
@@ -940,7 +1382,8 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     dalloc(d_dhat_, (size_t)n_surf_ * G_);
     dalloc(d_shat_, (size_t)n_surf_ * G_);
     dalloc(d_coef_, 5 * ncg);
-    dalloc(d_gscr_, 3 * (size_t)nc_);
+    // [3][nc] for the solve of batches, [2][K * lanes] for the wide solve
+    dalloc(d_gscr_, std::max(3 * (size_t)nc_, 2 * (size_t)cmfd_wide_k(nc_) * cmfd_wide_lanes(nc_)));
     dalloc(d_part_, 2 * (size_t)n_surf_ * G_);
     dalloc(d_part_old_, 2 * (size_t)n_surf_ * G_);
     dalloc(d_info_, 3);
@@ -1040,35 +1483,43 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
     hipLaunchKernelGGL(k_assemble, dim3(nblk((size_t)nc_ * G_)), dim3(256), 0, s, ca.m, G_, d_hrm_, d_dtil_,
                        d_dhat_, d_coef_);
     const SolveArgs a = CmfdAccess::solve(*this, v);
-    const size_t lds = cmfd_lds_bytes(nc_);  // operands, fs_old, BiCGSTAB vectors
-    switch (cmfd_cells_per_lane(nc_)) {
-    case 1:
-        switch (cmfd_threads(nc_)) {
-        case 64: hipLaunchKernelGGL((k_cmfd_solve<1, 64>), dim3(1), dim3(64), lds, s, a); break;
-        case 128: hipLaunchKernelGGL((k_cmfd_solve<1, 128>), dim3(1), dim3(128), lds, s, a); break;
-        case 192: hipLaunchKernelGGL((k_cmfd_solve<1, 192>), dim3(1), dim3(192), lds, s, a); break;
-        case 256: hipLaunchKernelGGL((k_cmfd_solve<1, 256>), dim3(1), dim3(256), lds, s, a); break;
-        case 320: hipLaunchKernelGGL((k_cmfd_solve<1, 320>), dim3(1), dim3(320), lds, s, a); break;
-        case 384: hipLaunchKernelGGL((k_cmfd_solve<1, 384>), dim3(1), dim3(384), lds, s, a); break;
-        case 448: hipLaunchKernelGGL((k_cmfd_solve<1, 448>), dim3(1), dim3(448), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_cmfd_solve<1, 512>), dim3(1), dim3(512), lds, s, a); break;
+    if (cmfd_wide(nc_)) {
+        const size_t lds = cmfd_wide_lds_bytes(nc_, nx_);
+        if (cmfd_wide_lanes(nc_) == 512)
+            launch_wide<512, 3, 6>(cmfd_wide_k(nc_), a, lds, s);
+        else
+            launch_wide<1024, 4, 6>(cmfd_wide_k(nc_), a, lds, s);
+    } else {
+        const size_t lds = cmfd_lds_bytes(nc_);  // operands, fs_old, BiCGSTAB vectors
+        switch (cmfd_cells_per_lane(nc_)) {
+        case 1:
+            switch (cmfd_threads(nc_)) {
+            case 64: hipLaunchKernelGGL((k_cmfd_solve<1, 64>), dim3(1), dim3(64), lds, s, a); break;
+            case 128: hipLaunchKernelGGL((k_cmfd_solve<1, 128>), dim3(1), dim3(128), lds, s, a); break;
+            case 192: hipLaunchKernelGGL((k_cmfd_solve<1, 192>), dim3(1), dim3(192), lds, s, a); break;
+            case 256: hipLaunchKernelGGL((k_cmfd_solve<1, 256>), dim3(1), dim3(256), lds, s, a); break;
+            case 320: hipLaunchKernelGGL((k_cmfd_solve<1, 320>), dim3(1), dim3(320), lds, s, a); break;
+            case 384: hipLaunchKernelGGL((k_cmfd_solve<1, 384>), dim3(1), dim3(384), lds, s, a); break;
+            case 448: hipLaunchKernelGGL((k_cmfd_solve<1, 448>), dim3(1), dim3(448), lds, s, a); break;
+            default: hipLaunchKernelGGL((k_cmfd_solve<1, 512>), dim3(1), dim3(512), lds, s, a); break;
+            }
+            break;
+        case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+        case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+        case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
+        case 5:
+            if (cmfd_rows_in_lds(nc_))
+                hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+            else
+                hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+            break;
+        default:
+            if (cmfd_rows_in_lds(nc_))
+                hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+            else
+                hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
+            break;
         }
-        break;
-    case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-    case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-    case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-    case 5:
-        if (cmfd_rows_in_lds(nc_))
-            hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-        break;
-    default:
-        if (cmfd_rows_in_lds(nc_))
-            hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-        break;
     }
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
     has_old_partial_ = n_solve_ > 0;  // cmfd.cpp:566, before n_solve_++ (:274)
@@ -1255,6 +1706,9 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
         CMFD_CHECK(hipMemcpyAsync(impl_->bm[k], maps[k].data(), maps[k].size() * sizeof(CaseBlock),
                                   hipMemcpyHostToDevice, s));
     }
+    if (ncmax > CmfdSolver::kMaxBatchCells)
+        throw std::runtime_error("a batched CMFD problem holds at most " + std::to_string(CmfdSolver::kMaxBatchCells) +
+                                 " coarse cells; this batch has one of " + std::to_string(ncmax));
     impl_->kmax = cmfd_cells_per_lane(ncmax);
     impl_->nt = cmfd_threads(ncmax);
     impl_->rl = cmfd_rows_in_lds(ncmax);

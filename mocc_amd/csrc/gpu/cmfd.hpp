@@ -26,9 +26,10 @@ struct CmfdView {
 
 class CmfdSolver {
 public:
-    // Largest coarse mesh the single-workgroup solve holds in registers
-    // (512 lanes x 6 cells).
-    static constexpr int kMaxCells = 3072;
+    // Largest coarse mesh of a single problem's solve (1024 lanes x 6 cells),
+    // and of a problem in a batch (512 lanes x 6 cells).
+    static constexpr int kMaxCells = 6144;
+    static constexpr int kMaxBatchCells = 3072;
 
     ~CmfdSolver();
     void setup(const mocgpu_problem &p, const mocgpu_options &opt, hipStream_t s);

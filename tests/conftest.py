@@ -37,12 +37,14 @@ def cmfd_decks(deck_dir):
     """Decks for the CMFD and eigenvalue parity tests: the 3x3-pin deck
     itself, one 17x17 assembly (289 cells) and the 3x3-assembly core geometry
     (51x51 cells, vacuum east/south) with coarser rays so the FP64 oracle
-    finishes in seconds; a 48x48-pin lattice (2304 cells, vacuum east/south)
-    is the size whose coarse solve runs 5 cells per lane with its rows in
-    LDS (the 51x51 core runs 6). Square lattices of 24, 34 and 40 pins (576,
-    1156 and 1600 cells) run 2, 3 and 4 cells per lane with the rows in
-    registers, and 55x55 (3025 cells) 6 per lane re-reading the rows from
-    cache; all with vacuum east/south faces."""
+    finishes in seconds, and square lattices with vacuum east/south faces.
+    A single problem's coarse solve runs on up to 512 lanes, 1 or 2 cells per
+    lane, up to 1024 cells (pins3x3, asm17, lat24: 576 cells), and on 1024
+    lanes beyond: 2 cells per lane (lat34, lat40: 1156 and 1600 cells) and 3
+    (lat48, core51, lat55: 2304, 2601 and 3025 cells) with the rows in LDS,
+    4, 5 and 6 (lat64, lat70, lat78: 4096, 4900 and 6084 cells) re-reading
+    the rows from cache. Batches run every mesh on 512 lanes, up to 6 cells
+    per lane (3072 cells)."""
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {
@@ -52,7 +54,7 @@ def cmfd_decks(deck_dir):
         "lat48": lg.write_deck(lg.pin_lattice(48, 2, 4, 0.1, "lat48_cmfd", cmfd=True,
                                               bc={"east": "vacuum", "south": "vacuum"}), deck_dir, 48),
     }
-    for n in (24, 34, 40, 55):
+    for n in (24, 34, 40, 55, 64, 70, 78):
         paths[f"lat{n}"] = lg.write_deck(lg.pin_lattice(n, 2, 4, 0.1, f"lat{n}_cmfd", cmfd=True,
                                                         bc={"east": "vacuum", "south": "vacuum"}), deck_dir, n)
     return {k: HostProblem.load(v) for k, v in paths.items()}

@@ -348,3 +348,15 @@ def test_autotuned_branch_batch(tmp_path):
     for a, b in zip(res[True], res[False]):
         assert a.converged and b.converged
         assert abs(a.k - b.k) < 1e-9, (a.k, b.k)
+
+
+def test_batch_rejects_oversized_cmfd_mesh(cmfd_decks):
+    """A single problem's CMFD takes up to 6144 coarse cells; a batched one up
+    to 3072 (the batch's 512-lane solve): adding a 4096-cell problem fails
+    with a message and leaves the batch usable."""
+    from mocc_amd.sweeper import DeviceError, GpuBatch, GpuMoCSweeper
+    big = GpuMoCSweeper(cmfd_decks["lat64"], energy_mode=1, n_inner=1, use_cmfd=True)
+    b = GpuBatch()
+    with pytest.raises(DeviceError, match="at most 3072 coarse cells"):
+        b.add(big, 1e-6, 1e-5, 10)
+    assert b.size() == 0

@@ -48,10 +48,12 @@ def _check_pair(hp, tight=True):
 
 
 # coarse-solve instantiations: 1 cell per lane (pins3x3 on one wave, asm17 on
-# 320 lanes), 2, 3 and 4 per lane with the rows in registers (lat24, lat34,
-# lat40), 5 and 6 with the rows in LDS (lat48, core51), 6 re-reading the rows
-# from cache (lat55)
-@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "lat24", "lat34", "lat40", "lat48", "core51", "lat55"])
+# 320 lanes) and 2 (lat24) on up to 512 lanes; on 1024 lanes, 2 and 3 per lane
+# with the rows in LDS (lat34, lat40; lat48, core51, lat55) and 4, 5, 6
+# re-reading them from cache (lat64, lat70, lat78: past the 512-lane solve's
+# 3072 cells)
+@pytest.mark.parametrize("deck", ["pins3x3", "asm17", "lat24", "lat34", "lat40", "lat48", "core51", "lat55",
+                                  "lat64", "lat70", "lat78"])
 def test_cmfd_step_parity(deck, cmfd_decks):
     hp = cmfd_decks[deck]
     dev, ora, kd, ko = _check_pair(hp, tight=True)
