@@ -127,12 +127,17 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     from mocc_amd.sweeper import GpuBatch
     n_inner = 5 if use_cmfd else 1
     factory = BranchFactory(os.path.join(work_dir, "batch_cmfd" if use_cmfd else "batch"), n_inner=n_inner,
-                            use_cmfd=use_cmfd, autotune=True)
+                            use_cmfd=use_cmfd)
     cases = c4_cases(n_cases)
+    # the band layout of the whole batch, timed on a sample spread over the mix
+    t0 = time.perf_counter()
+    factory.tune(cases[::max(1, n_cases // 32)])
+    t_tune = time.perf_counter() - t0
+    tuned = {"band_rays": factory.band_rays, "lane_map": factory.lane_map,
+             "candidates_s": {f"{w}/{m}": round(t, 4) for w, m, t in factory.tuning}, "tune_s": t_tune}
     t0 = time.perf_counter()
     handles = [factory.make(c) for c in cases]
     t_setup = time.perf_counter() - t0
-    tuned = {f"{k[0]}@{k[1]:g}cm": {"band_rays": v[0], "lane_map": v[1]} for k, v in factory.tuned().items()}
     batch = GpuBatch(use_graph=True)
     ids = [batch.add(h, K_TOL, PSI_TOL, MAX_IT) for h in handles]
     t0 = time.perf_counter()
@@ -172,7 +177,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     return {
         "workload": f"{n_cases} branch cases: 1/4 single pin, 1/2 3x3, 1/4 17x17 (25 guide tubes); "
                     "2/7 groups alternating; CY 16x3; spacing from {0.02, 0.035, 0.05} cm; XS x U[0.9,1.1]; "
-                    "band width and lane map autotuned per geometry",
+                    "band width and lane map autotuned for the batch",
         "batched_solves_per_s": bat, "serial_solves_per_s": ser, "speedup": bat / ser,
         "serial_sample": len(sub), "converged": int(n_conv), "outer_iterations_max": int(max(iters)),
         "batch_s": t_batch, "setup_s": t_setup,
@@ -195,16 +200,19 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     from mocc_amd.branch import BranchFactory, c5_request
     from mocc_amd.serve import Server, poisson_arrivals, serve_serial, summarize
     factory = BranchFactory(os.path.join(work_dir, "serve_cmfd" if use_cmfd else "serve"), n_inner=5 if use_cmfd else 1,
-                            use_cmfd=use_cmfd, autotune=True)
+                            use_cmfd=use_cmfd)
     rng = np.random.default_rng(seed)
     cases = [c5_request(rng, i) for i in range(n_req)]
     for c in cases:  # trace every geometry before timing
         factory.host_problem(c)
+    factory.tune(cases[:32])
     srv = Server(factory, chunk=1)
     burst = summarize(srv.serve(np.zeros(n_req), cases))
     out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
                        "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]; "
-                       + ("CMFD, 5 inner iterations" if use_cmfd else "no CMFD, 1 inner iteration"),
+                       + ("CMFD, 5 inner iterations" if use_cmfd else "no CMFD, 1 inner iteration")
+                       + "; band width and lane map autotuned for the batch on the first 32 requests",
+           "tuned": {"band_rays": factory.band_rays, "lane_map": factory.lane_map},
            "burst": burst}
     progress(f"serving burst: {burst['solves_per_s']:.3g} solves/s")
     last = None

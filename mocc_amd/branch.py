@@ -54,20 +54,18 @@ class BranchFactory:
 
     BASE_SEED = 4
 
-    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1, use_cmfd: bool = False,
-                 autotune: bool = False):
+    def __init__(self, work_dir: str, energy_mode: int = 1, n_inner: int = 1, use_cmfd: bool = False):
         """use_cmfd: CMFD-accelerated solves (mocc's CMFD decks use 5 inner
         iterations per outer; with 1 the coarse system can lose its
-        diagonal dominance). autotune: each geometry's prototype handle picks
-        its band width and lane map by timing the candidates on the device
-        (mocgpu_create with lane_map = -1); every case of that geometry then
-        shares the tuned plan, and a batch whose cases all tuned to bands of
-        at most 32 rays runs one-wave queue workgroups."""
+        diagonal dominance). Every geometry's ray plan uses the factory's band
+        width and lane map (the device defaults until tune() picks them)."""
         self.work_dir = work_dir
         self.energy_mode = energy_mode
         self.n_inner = n_inner
         self.use_cmfd = use_cmfd
-        self.autotune = autotune
+        self.band_rays = 0     # 0: the device default (64)
+        self.lane_map = None   # None: the device default
+        self.tuning: list = []  # (band rays, lane map, seconds) of the last tune()
         self._hp: dict = {}
         self._proto: dict = {}
         self._lock = threading.Lock()
@@ -113,10 +111,48 @@ class BranchFactory:
             proto = self._proto.get(gkey)
             if proto is None:
                 proto = GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode,
-                                      use_cmfd=self.use_cmfd, lane_map=-1 if self.autotune else None)
+                                      use_cmfd=self.use_cmfd, band_rays=self.band_rays, lane_map=self.lane_map)
                 self._proto[gkey] = proto
             return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=self.use_cmfd,
                                  like=proto)
+
+    def tune(self, sample: list[Case], widths=(64, 48, 32), maps=(1, 0), n_outer: int = 3) -> tuple[int, int]:
+        """Band width x lane map for the batched workload, by timing it: for
+        every candidate, all geometries get that layout, and n_outer outer
+        iterations of one graph-replayed batch of the sample cases are timed
+        on the device (after one untimed iteration that builds the graph).
+        The fastest candidate is kept for the handles made afterwards.
+
+        The layout is chosen for the batch as a whole, not per geometry: the
+        batch runs every problem's bands in workgroups as wide as its widest
+        plan, so a geometry tuned alone to 32-ray bands would run them with
+        half its lanes idle beside 64-ray plans. Returns (band rays, lane map)."""
+        import time
+
+        from .sweeper import GpuBatch
+        results = []
+        for w in widths:
+            for m in maps:
+                self.close()
+                self.band_rays, self.lane_map = w, m
+                hs = [self.make(c) for c in sample]
+                b = GpuBatch(use_graph=True)
+                for h in hs:  # tolerances no problem meets: all run every iteration
+                    b.add(h, 0.0, 0.0, n_outer + 2)
+                b.run(1)
+                b.poll()
+                t0 = time.perf_counter()
+                b.run(n_outer)
+                b.poll()
+                results.append((w, m, time.perf_counter() - t0))
+                b.close()
+                for h in hs:
+                    h.close()
+        self.close()
+        self.tuning = results
+        w, m, _ = min(results, key=lambda r: r[2])
+        self.band_rays, self.lane_map = w, m
+        return w, m
 
     def tuned(self) -> dict:
         """(lattice, spacing) -> (band rays, lane map) of each geometry's plan."""

@@ -1195,11 +1195,12 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
 {
     constexpr int KN = K * NT;
     WideScratch<NT> S{red, lds + A.nx, 0, 0};
-    // the pads: zeros, written once
-    for (int i = threadIdx.x; i < A.nx; i += NT) {
-        lds[i] = 0.0;
-        lds[A.nx + 2 * KN + i] = 0.0;
-    }
+    // The pads and both banks start as zeros: a product reads across the end
+    // of its bank into the other one (the north neighbours of the last row,
+    // the east neighbour of the last lane) before that bank is first written,
+    // and LDS holds whatever the previous kernel left there (a NaN pattern
+    // times a zero coefficient is NaN).
+    for (int i = threadIdx.x; i < 2 * A.nx + 2 * KN; i += NT) lds[i] = 0.0;
     WideRows<NT, K> R;
     double *fs_old = A.gscr;
     double *sb = A.gscr + KN;

@@ -322,21 +322,25 @@ def test_batch_one_wave_queue_bitwise(batch_decks):
 
 
 def test_autotuned_branch_batch(tmp_path):
-    """Branch cases whose geometries were autotuned (band width x lane map
-    timed on the device, shared by every case of the geometry) solve in one
-    batch to the eigenvalues of the same cases without tuning (within the
-    rounding of the fixed-point tally scale, which follows the band layout)."""
+    """The batch-level autotuner (BranchFactory.tune: every band width x
+    lane map timed on a batch of sample cases, the fastest kept for all
+    geometries) picks a valid layout, and the cases solve in one batch to the
+    eigenvalues of the same cases at the default layout (within the rounding
+    of the fixed-point tally scale, which follows the band layout)."""
     from mocc_amd.branch import BranchFactory, c4_cases
     from mocc_amd.sweeper import GpuBatch
     cases = c4_cases(8)
     res = {}
     for tune in (True, False):
-        f = BranchFactory(str(tmp_path / f"t{int(tune)}"), energy_mode=1, n_inner=5, use_cmfd=True, autotune=tune)
+        f = BranchFactory(str(tmp_path / f"t{int(tune)}"), energy_mode=1, n_inner=5, use_cmfd=True)
+        if tune:
+            w, m = f.tune(cases[:4], n_outer=2)
+            print("tuned (band rays, lane map):", (w, m), "candidates:", f.tuning, flush=True)
+            assert w in (32, 48, 64) and m in (0, 1) and len(f.tuning) == 6
+            assert (w, m) == min(f.tuning, key=lambda r: r[2])[:2]
         hs = [f.make(c) for c in cases]
         if tune:
-            tuned = f.tuned()
-            print("tuned (band rays, lane map):", tuned, flush=True)
-            assert all(br in (32, 48, 64) and lm in (0, 1) for br, lm in tuned.values())
+            assert all(v == (w, m) for v in f.tuned().values())
         b = GpuBatch()
         ids = [b.add(h, K_TOL, PSI_TOL, MAX_IT) for h in hs]
         out = b.solve_all(chunk=4)

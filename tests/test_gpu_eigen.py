@@ -77,13 +77,20 @@ def test_pins3x3_keff(pins3x3, use_cmfd, n_inner, count):
     assert np.abs(fd[m] / fo[m] - 1.0).max() < 1e-4
 
 
-def test_core51_keff_cmfd(cmfd_decks):
-    hp = cmfd_decks["core51"]
-    dev, ora, st_d, st_o = _solve_pair(hp, True, 200)
+# Full solves with CMFD through both coarse solves' wide instantiations: the
+# 51x51 core geometry (2601 cells, 512 lanes at 6 cells per lane) and square
+# lattices of 1156 cells (512 lanes, 3 per lane) and 4096 cells (1024 lanes, 4
+# per lane). Every CMFD step runs right after sweeps, whose kernels leave
+# their own data in LDS.
+@pytest.mark.parametrize("deck", ["core51", "lat34", "lat64"])
+def test_keff_cmfd(cmfd_decks, deck):
+    hp = cmfd_decks[deck]
+    dev, ora, st_d, st_o = _solve_pair(hp, True, 200, n_inner=5)
     assert abs(st_d.k - st_o.k) < 1e-5, (st_d.k, st_o.k)
     assert abs(st_d.iterations - st_o.iterations) <= 1, (st_d.iterations, st_o.iterations)
-    # CMFD cuts the outer iterations by an order of magnitude on this deck
-    assert st_d.iterations < 40
+    if deck == "core51":
+        # CMFD cuts the outer iterations by an order of magnitude on this deck
+        assert st_d.iterations < 40
     fd, fo = _fission_shape(hp, dev.flux()), _fission_shape(hp, ora.flux())
     m = fo > 1e-6 * fo.max()
     assert np.abs(fd[m] / fo[m] - 1.0).max() < 1e-4
