@@ -374,10 +374,12 @@ void CorrSolver::setup(const mocgpu_problem &p, const SweepPlan &plan, hipStream
     }
 
     // rays: boundary indices, azimuth, exit flags (as the band plan forms
-    // them), coarse entries and first cells; per-entry segment counts; the
-    // visit and crossing lists of each (azimuth, cell / surface)
-    std::vector<int32_t> info((size_t)R_ * 8), nseg((size_t)2 * n_cm_, 0);
-    std::vector<std::vector<int32_t>> vis((size_t)A_ * n_cell_), crs((size_t)A_ * n_surf_);
+    // them), coarse entries and first cells; with the worker also the
+    // per-entry segment counts and the visit and crossing lists of each
+    // (azimuth, cell / surface). Without it (update_incoming_flux alone) the
+    // ray table is all: every CMFD handle builds it, so it stays O(rays).
+    std::vector<int32_t> info((size_t)R_ * 8), nseg(full ? (size_t)2 * n_cm_ : 0, 0);
+    std::vector<std::vector<int32_t>> vis(full ? (size_t)A_ * n_cell_ : 0), crs(full ? (size_t)A_ * n_surf_ : 0);
     for (int a = 0; a < A_; a++) {
         const int Ny = p.azi_ny[a];
         const int oct = a < A_ / 2 ? 0 : 1;
@@ -395,6 +397,7 @@ void CorrSolver::setup(const mocgpu_problem &p, const SweepPlan &plan, hipStream
             ri[5] = p.ray_cm_off[r + 1];
             ri[6] = p.ray_cm_cell[2 * r];
             ri[7] = p.ray_cm_cell[2 * r + 1];
+            if (!full) continue;
             for (int dir = 0; dir < 2; dir++) {
                 const int s0 = p.ray_cm_surf[2 * r + dir];
                 if (s0 >= 0 && s0 < n_surf_) crs[(size_t)a * n_surf_ + s0].push_back((r << 2) | (dir << 1) | 1);
@@ -422,12 +425,12 @@ void CorrSolver::setup(const mocgpu_problem &p, const SweepPlan &plan, hipStream
         item.reserve(ptr.back());
         for (const auto &l : lists) item.insert(item.end(), l.begin(), l.end());
     };
-    std::vector<int32_t> vp, vi, cp, ci;
-    csr(vis, vp, vi);
-    csr(crs, cp, ci);
     up(d_ray_info_, info);
     up(d_azi_, plan.azi);
     if (!full) return;
+    std::vector<int32_t> vp, vi, cp, ci;
+    csr(vis, vp, vi);
+    csr(crs, cp, ci);
     up(d_ent_nseg_, nseg);
     up(d_vis_ptr_, vp);
     up(d_vis_item_, vi);
