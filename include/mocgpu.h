@@ -161,6 +161,14 @@ int mocgpu_create(const mocgpu_problem *prob, const mocgpu_options *opt,
                   mocgpu_handle **out);
 int mocgpu_destroy(mocgpu_handle *h);
 
+/* Give the handle another problem's cross sections: the same geometry and
+ * group count (MOCGPU_ERR_ARG otherwise), new material data, and the state
+ * of a new handle (mocgpu_initialize). The device buffers are reused, so a
+ * server can take the next cross-section branch case of a geometry on a
+ * finished case's handle without allocating or freeing device memory. No
+ * reference counterpart: mocc reads its cross sections once per run. */
+int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p);
+
 /* A handle for another problem on the SAME geometry as `like` (same deck
  * geometry and ray parameters; cross sections and group count may differ, as
  * for cross-section branch cases): the ray plan, segment streams and FSR/pin

@@ -66,6 +66,7 @@ class BranchFactory:
         self.band_rays = 0     # 0: the device default (64)
         self.lane_map = None   # None: the device default
         self.tuning: list = []  # (band rays, lane map, seconds) of the last tune()
+        self._pool: dict = {}   # (lattice, spacing, groups) -> released handles
         self._hp: dict = {}
         self._proto: dict = {}
         self._lock = threading.Lock()
@@ -104,9 +105,15 @@ class BranchFactory:
 
     def make(self, c: Case, staged: str | None = None) -> GpuMoCSweeper:
         """Device handle of a case (device allocations and copies on the
-        handle's own stream)."""
+        handle's own stream), or a released handle of the same geometry and
+        group count given the case's cross sections (no allocation)."""
         with self._lock:
             hp = self.host_problem(c, staged)
+            pool = self._pool.get((c.kind, c.spacing, c.groups))
+            if pool:
+                s = pool.pop()
+                s.reload_xs(hp)
+                return s
             gkey = (c.kind, c.spacing)
             proto = self._proto.get(gkey)
             if proto is None:
@@ -154,6 +161,11 @@ class BranchFactory:
         self.band_rays, self.lane_map = w, m
         return w, m
 
+    def release(self, c: Case, s: GpuMoCSweeper):
+        """Hand back the handle of a finished case for reuse by make()."""
+        with self._lock:
+            self._pool.setdefault((c.kind, c.spacing, c.groups), []).append(s)
+
     def tuned(self) -> dict:
         """(lattice, spacing) -> (band rays, lane map) of each geometry's plan."""
         out = {}
@@ -163,6 +175,10 @@ class BranchFactory:
         return out
 
     def close(self):
+        for pool in self._pool.values():
+            for s in pool:
+                s.close()
+        self._pool.clear()
         for p in self._proto.values():
             p.close()
         self._proto.clear()

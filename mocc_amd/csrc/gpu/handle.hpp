@@ -60,6 +60,13 @@ template <typename T> struct DBuf {
         alloc(v.size());
         if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     }
+    // upload into the present allocation when it has the size and this
+    // buffer alone owns it (no allocation, no free); otherwise as upload
+    void refill(const std::vector<T> &v, hipStream_t s)
+    {
+        if (!p || n != v.size() || mem.use_count() != 1) return upload(v, s);
+        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
     void free()
     {
         mem.reset();

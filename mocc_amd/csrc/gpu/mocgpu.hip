@@ -302,6 +302,34 @@ namespace {
 // mocgpu_create / mocgpu_create_like: with a template handle, the geometry
 // (plan, segment streams, exp table, FSR and pin arrays) is shared with it
 // and only the cross sections and the state are new.
+// The material cross sections of the problem on the handle's stream, and its
+// number of fissile FSRs (eigen_solver.cpp:97-111).
+void put_materials(mocgpu_handle *h, const mocgpu_problem *p)
+{
+    const int G = p->n_group;
+    const size_t mg = (size_t)p->n_mat * G;
+    hipStream_t s = h->stream;
+    h->d_mat_tr.refill(std::vector<double>(p->xs_tr, p->xs_tr + mg), s);
+    h->d_mat_nf.refill(std::vector<double>(p->xs_nf, p->xs_nf + mg), s);
+    h->d_mat_f.refill(std::vector<double>(p->xs_f, p->xs_f + mg), s);
+    h->d_mat_ch.refill(std::vector<double>(p->xs_ch, p->xs_ch + mg), s);
+    h->d_mat_sc.refill(std::vector<double>(p->xs_sc, p->xs_sc + mg * G), s);
+    std::vector<double> ss(mg);
+    for (int m = 0; m < p->n_mat; m++)
+        for (int g = 0; g < G; g++) ss[(size_t)m * G + g] = p->xs_sc[((size_t)m * G + g) * G + g];
+    h->d_mat_ss.refill(ss, s);
+    int nfis = 0;
+    for (int f = 0; f < p->n_fsr; f++) {
+        const int m = h->h_fsr_mat[f];
+        for (int g = 0; g < G; g++)
+            if (p->xs_nf[(size_t)m * G + g] > 0.0) {
+                nfis++;
+                break;
+            }
+    }
+    h->n_fissile = nfis;
+}
+
 int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu_handle *like,
                 mocgpu_handle **out)
 {
@@ -409,17 +437,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         std::vector<uint16_t>().swap(h->plan->seg_loc);
         std::vector<float>().swap(h->plan->seg_len);
         }
-        const std::vector<int32_t> &fm = h->h_fsr_mat;
-        auto vecd = [](const double *x, size_t n) { return std::vector<double>(x, x + n); };
-        h->d_mat_tr.upload(vecd(p->xs_tr, (size_t)p->n_mat * G), s);
-        h->d_mat_nf.upload(vecd(p->xs_nf, (size_t)p->n_mat * G), s);
-        h->d_mat_f.upload(vecd(p->xs_f, (size_t)p->n_mat * G), s);
-        h->d_mat_ch.upload(vecd(p->xs_ch, (size_t)p->n_mat * G), s);
-        h->d_mat_sc.upload(vecd(p->xs_sc, (size_t)p->n_mat * G * G), s);
-        std::vector<double> ss((size_t)p->n_mat * G);
-        for (int m = 0; m < p->n_mat; m++)
-            for (int g = 0; g < G; g++) ss[m * G + g] = p->xs_sc[((size_t)m * G + g) * G + g];
-        h->d_mat_ss.upload(ss, s);
+        put_materials(h.get(), p);
         // state
         h->d_phi.alloc((size_t)F * G);
         h->d_phi_old.alloc((size_t)F * G);
@@ -442,21 +460,10 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         HIP_CHECK(hipMemsetAsync(h->d_cur.p, 0, sizeof(double) * h->n_surf * G, s));
         HIP_CHECK(hipMemsetAsync(h->d_sfl.p, 0, sizeof(double) * h->n_surf * G, s));
         HIP_CHECK(hipMemsetAsync(h->d_psi.p, 0, sizeof(float) * G * 2 * pl.bcpg, s));
-        // number of fissile FSRs (eigen_solver.cpp:97-111)
-        int nfis = 0;
-        for (int f = 0; f < F; f++) {
-            int m = fm[f];
-            for (int g = 0; g < G; g++)
-                if (p->xs_nf[m * G + g] > 0.0) {
-                    nfis++;
-                    break;
-                }
-        }
-        h->n_fissile = nfis;
         std::vector<double> sc(S_COUNT, 0.0);
         sc[S_K] = 1.0;
         sc[S_KPREV] = 1.0;
-        sc[S_NFISS] = (double)nfis;
+        sc[S_NFISS] = (double)h->n_fissile;
         h->d_scal.upload(sc, s);
         // template of the initial incoming flux: 1/(4 pi) on non-vacuum faces
         // (MoCSweeper::initialize + BoundaryCondition::initialize_scalar)
@@ -607,6 +614,30 @@ int mocgpu_destroy(mocgpu_handle *h)
 {
     delete h;
     return 0;
+}
+
+int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p)
+{
+    if (!p) return MOCGPU_ERR_ARG;
+    return guard(h, [&] {
+        bool same = p->n_fsr == h->F && p->n_group == h->G && p->n_mat == h->n_mat && p->n_seg == h->n_seg &&
+                    p->n_polar == h->NP && p->n_azi == h->A && p->nx == h->nx && p->ny == h->ny;
+        for (int f = 0; same && f < p->n_fsr; f++) same = p->fsr_mat[f] == h->h_fsr_mat[f];
+        if (!same)
+            throw std::runtime_error("mocgpu_reload_xs: the problem's geometry or group count differs from the handle's");
+        put_materials(h, p);
+        // the state a new handle starts from (mocgpu_initialize sets the rest)
+        const size_t fg = (size_t)h->F * h->G;
+        HIP_CHECK(hipMemsetAsync(h->d_src.p, 0, sizeof(double) * fg, h->stream));
+        HIP_CHECK(hipMemsetAsync(h->d_fs.p, 0, sizeof(double) * h->F, h->stream));
+        std::vector<double> sc(S_COUNT, 0.0);
+        sc[S_K] = 1.0;
+        sc[S_KPREV] = 1.0;
+        sc[S_NFISS] = (double)h->n_fissile;
+        HIP_CHECK(hipMemcpyAsync(h->d_scal.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, h->stream));
+        h->sync();
+        return mocgpu_initialize(h);
+    });
 }
 
 int mocgpu_initialize(mocgpu_handle *h)

@@ -127,6 +127,7 @@ def gpu() -> C.CDLL:
         L.mocgpu_default_options.argtypes = [C.POINTER(MocgpuOptions)]
         L.mocgpu_create.argtypes = [C.POINTER(MocgpuProblem), C.POINTER(MocgpuOptions), C.POINTER(P)]
         L.mocgpu_destroy.argtypes = [P]
+        L.mocgpu_reload_xs.argtypes = [P, C.POINTER(MocgpuProblem)]
         L.mocgpu_get_options.argtypes = [P, C.POINTER(MocgpuOptions)]
         for fn in ("mocgpu_initialize", "mocgpu_store_old_flux"):
             getattr(L, fn).argtypes = [P]

@@ -5,8 +5,11 @@ Requests arrive over time. A request is admitted as soon as the batch's
 device-memory budget has room for its state (FIFO); every admitted problem
 keeps its flux, sources and boundary fluxes on the device until it finishes.
 The device advances all admitted problems together a few outer iterations
-at a time; finished problems leave and free their memory, and waiting ones
-join at the next iteration boundary.
+at a time; finished problems leave the batch, and waiting ones join at the
+next iteration boundary. A finished problem's handle goes back to the
+factory, which gives it the next case of the same geometry and group count
+(mocgpu_reload_xs: new cross sections, no device allocation or free); the
+budget counts the problems in the batch.
 
 While the device works on one chunk, the host does only host work for the
 requests that have arrived (their cross-section libraries). Their device
@@ -88,7 +91,7 @@ class Server:
                     if self.keep_flux:
                         r.flux = s.flux()
                     b.remove(cid)
-                    s.close()
+                    self.factory.release(r.case, s)
                 busy = False
             # the device is idle: make the handles of the staged requests
             while staged:
@@ -131,7 +134,7 @@ def serve_serial(factory: BranchFactory, arrivals, cases: list[Case], k_tol=1e-7
         rc, st = s.eigen_solve(k_tol, psi_tol, max_iter)
         r.t_done = time.perf_counter() - t0
         r.k, r.iterations, r.converged = st.k, st.iterations, bool(st.converged)
-        s.close()
+        factory.release(r.case, s)
     return recs
 
 

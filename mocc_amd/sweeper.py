@@ -73,6 +73,12 @@ class GpuMoCSweeper:
         except Exception:
             pass
 
+    def reload_xs(self, prob: HostProblem):
+        """Take prob's cross sections (same geometry and group count) and
+        start over from a new handle's state, reusing the device buffers
+        (mocgpu_reload_xs)."""
+        self._check(gpu().mocgpu_reload_xs(self.h, prob.view))
+
     # --- TransportSweeper -------------------------------------------------
     def initialize(self):
         self._check(gpu().mocgpu_initialize(self.h))

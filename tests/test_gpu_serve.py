@@ -94,3 +94,43 @@ def test_serve_cmfd_matches_alone(cmfd_factory):
         st, flux = _solve_alone(cmfd_factory, r.case)
         assert st.k == r.k and st.iterations == r.iterations
         assert np.array_equal(flux, r.flux)
+
+
+def test_recycled_handle_matches_fresh(tmp_path):
+    """A finished case's handle given the next case of its geometry
+    (mocgpu_reload_xs through BranchFactory.release/make) answers that case
+    bitwise as a newly created handle does, in a batch and alone, with and
+    without CMFD."""
+    from mocc_amd.branch import BranchFactory, Case
+    from mocc_amd.sweeper import GpuBatch
+    for use_cmfd in (False, True):
+        ni = 5 if use_cmfd else 1
+        f_old = BranchFactory(str(tmp_path / f"r{int(use_cmfd)}"), n_inner=ni, use_cmfd=use_cmfd)
+        f_new = BranchFactory(str(tmp_path / f"n{int(use_cmfd)}"), n_inner=ni, use_cmfd=use_cmfd)
+        first, second = Case("3x3", 7, 0.1, 71), Case("3x3", 7, 0.1, 72)
+        s = f_old.make(first)
+        rc, st = s.eigen_solve(1e-7, 1e-6, 3000)
+        assert rc == 0 and st.converged
+        f_old.release(first, s)
+        r = f_old.make(second)
+        assert r is s, "the released handle is reused"
+        fresh = f_new.make(second)
+        print(f"cmfd={use_cmfd}: recycled vs fresh, alone", flush=True)
+        (rc1, st1), (rc2, st2) = r.eigen_solve(1e-7, 1e-6, 3000), fresh.eigen_solve(1e-7, 1e-6, 3000)
+        assert rc1 == rc2 == 0 and st1.k == st2.k and st1.iterations == st2.iterations
+        assert np.array_equal(r.flux(), fresh.flux())
+        print(f"cmfd={use_cmfd}: recycled vs fresh, in a batch", flush=True)
+        f_old.release(second, r)
+        r = f_old.make(second)
+        out = []
+        for h in (r, fresh):
+            b = GpuBatch()
+            b.add(h, 1e-7, 1e-6, 3000)
+            (cid, st), = list(b.solve_all().items())
+            out.append((st.k, st.iterations, h.flux()))
+            b.close()
+        assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and np.array_equal(out[0][2], out[1][2])
+        f_old.release(second, r)
+        fresh.close()
+        f_old.close()
+        f_new.close()
