@@ -138,6 +138,7 @@ struct mocgpu_handle {
 
     // host copies
     std::vector<float> bc_template;  // one group of incoming values after initialize
+    DBuf<float> d_bc_template;        // the same on the device (shared with create_like handles)
     std::vector<int32_t> h_fsr_mat;
 
     ~mocgpu_handle()

@@ -162,6 +162,19 @@ __global__ void k_fill_f(float *a, float v, size_t n)
     if (i < n) a[i] = v;
 }
 
+// MoCSweeper::initialize's state: flat scalar flux 1 (and its old copy), and
+// both incoming-flux buffers of every group set to the one-group template.
+__global__ void k_init_state(double *phi, double *phi_old, size_t fg, float *psi, const float *tmpl, size_t bcpg,
+                             size_t n_psi)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < fg) {
+        phi[i] = 1.0;
+        phi_old[i] = 1.0;
+    }
+    if (i < n_psi) psi[i] = tmpl[i % bcpg];
+}
+
 // MoCSweeper::get_pin_flux_1g, PIN_PLANE (moc_sweeper.cpp:293-326)
 __global__ void k_pin_flux(int npin, int G, int g, const int32_t *pin_off, const int32_t *pin_cell,
                            const double *vol, const double *phi, double *pin)
@@ -405,6 +418,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
             h->d_pin_cell = like->d_pin_cell;
             h->d_pin_area = like->d_pin_area;
             h->bc_template = like->bc_template;
+            h->d_bc_template = like->d_bc_template;
         } else {
         h->d_bands.upload(pl.bands, s);
         h->d_order.upload(pl.band_order, s);
@@ -483,6 +497,7 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
             if (p->bc[sy] == MOCGPU_BC_REFLECT)
                 for (int i = 0; i < nx_; i++) h->bc_template[pl.ang_off[iang] + ny_ + i] = val;
         }
+        h->d_bc_template.upload(h->bc_template, s);
         }
         h->d_parity.alloc(1);
         h->d_status.alloc(2);
@@ -643,13 +658,11 @@ int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p)
 int mocgpu_initialize(mocgpu_handle *h)
 {
     return guard(h, [&] {
-        std::vector<double> ones((size_t)h->F * h->G, 1.0);
-        HIP_CHECK(hipMemcpyAsync(h->d_phi.p, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, h->stream));
-        HIP_CHECK(hipMemcpyAsync(h->d_phi_old.p, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, h->stream));
-        for (int g = 0; g < h->G; g++)
-            for (int b = 0; b < 2; b++)
-                HIP_CHECK(hipMemcpyAsync(h->d_psi.p + ((size_t)g * 2 + b) * h->plan->bcpg, h->bc_template.data(),
-                                         h->bc_template.size() * 4, hipMemcpyHostToDevice, h->stream));
+        const size_t fg = (size_t)h->F * h->G, bcpg = (size_t)h->plan->bcpg, n_psi = (size_t)h->G * 2 * bcpg;
+        const size_t n = std::max(fg, n_psi);
+        hipLaunchKernelGGL(k_init_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_phi.p,
+                           h->d_phi_old.p, fg, h->d_psi.p, h->d_bc_template.p, bcpg, n_psi);
+        HIP_CHECK(hipGetLastError());
         h->parity = 0;
         // no coarse-surface currents until a sweep tallies them: the first
         // CMFD solve after initialize runs with D-hat = 0, as after
