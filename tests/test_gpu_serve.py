@@ -134,3 +134,21 @@ def test_recycled_handle_matches_fresh(tmp_path):
         fresh.close()
         f_old.close()
         f_new.close()
+
+
+def test_reload_xs_refuses_other_geometry(tmp_path):
+    """mocgpu_reload_xs takes only a problem of the handle's geometry and
+    group count: another lattice or group count is refused with a message
+    and leaves the handle usable."""
+    from mocc_amd.branch import BranchFactory, Case
+    from mocc_amd.sweeper import DeviceError
+    f = BranchFactory(str(tmp_path))
+    s = f.make(Case("3x3", 2, 0.1, 81))
+    for other in (Case("pin", 2, 0.1, 82), Case("3x3", 7, 0.1, 83)):
+        with pytest.raises(DeviceError, match="geometry or group count differs"):
+            s.reload_xs(f.host_problem(other))
+    s.reload_xs(f.host_problem(Case("3x3", 2, 0.1, 84)))
+    rc, st = s.eigen_solve(1e-7, 1e-6, 3000)
+    assert rc == 0 and st.converged
+    s.close()
+    f.close()
