@@ -44,7 +44,7 @@ def _run(cmd, cwd=None):
 def build_host():
     os.makedirs(LIB, exist_ok=True)
     src = [os.path.join(PKG, "csrc", "host", s) for s in HOST_SRC]
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-Wall",
           "-o", os.path.join(LIB, "libmocc_host.so.tmp")] + src)
     os.replace(os.path.join(LIB, "libmocc_host.so.tmp"), os.path.join(LIB, "libmocc_host.so"))
 

@@ -1,7 +1,12 @@
 #include "model.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <fstream>
 #include <limits>
 #include <numeric>
@@ -980,21 +985,17 @@ int CoreMesh::coarse_neighbor(int cell, Surface s) const
 }
 
 namespace {
+// The first entry of the ascending v that equals p (fp_equiv) or lies above
+// it: `on` if it equals p, and i is its index, or the index before it when it
+// lies above (v.size() when there is none). The entries equal to p form one
+// run around p, so "equal to p or above it" holds on a suffix of v and a
+// binary search finds its start.
 void locate(const VecF &v, double p, int &i, bool &on)
 {
-    on = false;
-    i = 0;
-    for (double xi : v) {
-        if (fp_equiv(p, xi)) {
-            on = true;
-            break;
-        }
-        if (xi > p) {
-            i--;
-            break;
-        }
-        i++;
-    }
+    const auto it = std::partition_point(v.begin(), v.end(), [p](double xi) { return !(fp_equiv(p, xi) || xi > p); });
+    i = (int)(it - v.begin());
+    on = it != v.end() && fp_equiv(p, *it);
+    if (it != v.end() && !on) i--;
 }
 }  // namespace
 
@@ -1232,6 +1233,45 @@ Ray::Ray(Point2 a, Point2 b, int bc0, int bc1, const CoreMesh &mesh) : p1(a), p2
     if (nsegs_bw.size() > nsegs_fw.size()) cm_data.push_back({INVALID, surfs_bw.back(), 0, 0});
 }
 
+int host_threads()
+{
+    for (const char *name : {"MOCC_HOST_THREADS", "OMP_NUM_THREADS"}) {
+        const char *v = std::getenv(name);
+        if (v && std::atoi(v) > 0) return std::min(64, std::atoi(v));
+    }
+    const int hw = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(64, hw));
+}
+
+void parallel_for(int n, const std::function<void(int)> &f)
+{
+    const int nt = std::min(n, host_threads());
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&]() {
+        for (int i = next++; i < n && !failed.load(); i = next++) {
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!err) err = std::current_exception();
+                failed = true;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
 // util/rational_approximation.hpp: binary search of the Farey sequence
 std::pair<int, int> rational_approximation(double target, double tol, int max_d)
 {
@@ -1312,15 +1352,20 @@ RayData::RayData(const XmlNode &in, const AngularQuadrature &quad, const CoreMes
     // azimuth.
     int n_pol = (ang_quad.type() == QuadType::LS || ang_quad.type() == QuadType::USER) ? 1 : ang_quad.n_polar();
     int n_azi = 2 * nd / n_pol;
+    // The azimuths are independent: each is traced on its own thread into
+    // its own slot, so the rays are the same for any thread count.
     Point2 lo(0.0, 0.0), hi(hx, hy);
-    for (int a = 0; a < n_azi; a++) {
+    rays.resize(n_azi);
+    VecI azi_max(n_azi, 0);
+    parallel_for(n_azi, [&](int a) {
         int iang = a * n_pol;
         const Angle &ang = ang_quad[iang];
         int nx_ = Nx[iang], ny_ = Ny[iang];
         double space = spacing[iang];
         double sx = std::abs(space / std::sin(ang.alpha));
         double sy = std::abs(space / std::cos(ang.alpha));
-        std::vector<Ray> rs;
+        std::vector<Ray> &rs = rays[a];
+        rs.reserve(nx_ + ny_);
         auto exit_bc = [&](const Point2 &p2) -> int {
             if (fp_equiv(p2.x, hx)) return (int)(p2.y / sy);
             if (fp_equiv(p2.y, hy)) return (int)(p2.x / sx) + ny_;
@@ -1335,17 +1380,17 @@ RayData::RayData(const XmlNode &in, const AngularQuadrature &quad, const CoreMes
             Point2 p2 = box_intersect(lo, hi, p1, ang.alpha);
             if (!p2.ok) throw Error("Ray does not exit the core box");
             rs.emplace_back(p1, p2, ir, exit_bc(p2), mesh);
-            max_seg = std::max(max_seg, rs.back().nseg());
+            azi_max[a] = std::max(azi_max[a], rs.back().nseg());
         }
         for (int ir = 0; ir < nx_; ir++) {
             Point2 p1((0.5 + ir) * sx, 0.0);
             Point2 p2 = box_intersect(lo, hi, p1, ang.alpha);
             if (!p2.ok) throw Error("Ray does not exit the core box");
             rs.emplace_back(p1, p2, ir + ny_, exit_bc(p2), mesh);
-            max_seg = std::max(max_seg, rs.back().nseg());
+            azi_max[a] = std::max(azi_max[a], rs.back().nseg());
         }
-        rays.push_back(std::move(rs));
-    }
+    });
+    for (int m : azi_max) max_seg = std::max(max_seg, m);
     correct_volume(mesh);
 }
 
@@ -1356,8 +1401,9 @@ void RayData::correct_volume(const CoreMesh &mesh)
     int nreg = (int)true_vol.size();
     int n_pol = (int)(2 * ang_quad.ndir_oct() / rays.size());
     if (correction == VolumeCorrection::FLAT) {
-        for (size_t a = 0; a < rays.size(); a++) {
-            double space = spacing[a * n_pol];
+        // each azimuth is corrected on its own
+        parallel_for((int)rays.size(), [&](int a) {
+            double space = spacing[(size_t)a * n_pol];
             VecF fsr_vol(nreg, 0.0);
             for (const auto &r : rays[a])
                 for (int k = 0; k < r.nseg(); k++) fsr_vol[r.seg_index[k]] += r.seg_len[k] * space;
@@ -1365,7 +1411,7 @@ void RayData::correct_volume(const CoreMesh &mesh)
             for (int i = 0; i < nreg; i++) cf[i] = true_vol[i] / fsr_vol[i];
             for (auto &r : rays[a])
                 for (int k = 0; k < r.nseg(); k++) r.seg_len[k] = r.seg_len[k] * cf[r.seg_index[k]];
-        }
+        });
     } else if (correction == VolumeCorrection::ANGLE) {
         VecF fsr_vol(nreg, 0.0);
         for (size_t a = 0; a < rays.size(); a++)

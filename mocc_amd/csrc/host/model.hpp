@@ -15,6 +15,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -273,5 +274,15 @@ private:
 };
 
 std::pair<int, int> rational_approximation(double target, double tol, int max_d);
+
+// Host threads for the tracer and the array assembly: MOCC_HOST_THREADS, else
+// OMP_NUM_THREADS, else the hardware's count, within [1, 64].
+int host_threads();
+
+// f(i) for every i in [0, n), spread over host_threads() threads; each i runs
+// exactly once and writes only its own outputs, so the result does not depend
+// on the thread count. The first exception thrown is rethrown here after all
+// threads have stopped.
+void parallel_for(int n, const std::function<void(int)> &f);
 
 }  // namespace mocc_amd

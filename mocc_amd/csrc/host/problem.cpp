@@ -135,6 +135,8 @@ void Problem::fill_view()
         ang_oy_.push_back(a.oy);
         ang_alpha_.push_back(a.alpha);
     }
+    // per-ray records and offsets first, then the segments and coarse data of
+    // each azimuth copied to their places in parallel
     azi_ray_off_.push_back(0);
     ray_seg_off_.push_back(0);
     ray_cm_off_.push_back(0);
@@ -144,27 +146,38 @@ void Problem::fill_view()
         azi_ny_.push_back(rd.Ny[iang]);
         azi_spacing_.push_back(rd.spacing[iang]);
         for (const Ray &r : rd.rays[a]) {
-            for (int k = 0; k < r.nseg(); k++) {
-                seg_fsr_.push_back(r.seg_index[k]);
-                seg_len_.push_back(r.seg_len[k]);
-            }
-            ray_seg_off_.push_back((int64_t)seg_fsr_.size());
+            ray_seg_off_.push_back(ray_seg_off_.back() + r.nseg());
             ray_bc_.push_back(r.bc[0]);
             ray_bc_.push_back(r.bc[1]);
             ray_cm_cell_.push_back(r.cm_cell_fw);
             ray_cm_cell_.push_back(r.cm_cell_bw);
             ray_cm_surf_.push_back(r.cm_surf_fw);
             ray_cm_surf_.push_back(r.cm_surf_bw);
-            for (const auto &c : r.cm_data) {
-                cm_data_.push_back(c.fw);
-                cm_data_.push_back(c.bw);
-                cm_data_.push_back(c.nseg_fw);
-                cm_data_.push_back(c.nseg_bw);
-            }
-            ray_cm_off_.push_back((int32_t)(cm_data_.size() / 4));
+            ray_cm_off_.push_back(ray_cm_off_.back() + (int32_t)r.cm_data.size());
         }
         azi_ray_off_.push_back((int32_t)(ray_seg_off_.size() - 1));
     }
+    seg_fsr_.resize((size_t)ray_seg_off_.back());
+    seg_len_.resize((size_t)ray_seg_off_.back());
+    cm_data_.resize((size_t)ray_cm_off_.back() * 4);
+    parallel_for(A, [&](int a) {
+        int ir = azi_ray_off_[a];
+        for (const Ray &r : rd.rays[a]) {
+            const int64_t s0 = ray_seg_off_[ir];
+            for (int k = 0; k < r.nseg(); k++) {
+                seg_fsr_[s0 + k] = r.seg_index[k];
+                seg_len_[s0 + k] = r.seg_len[k];
+            }
+            size_t c0 = (size_t)ray_cm_off_[ir] * 4;
+            for (const auto &c : r.cm_data) {
+                cm_data_[c0++] = c.fw;
+                cm_data_[c0++] = c.bw;
+                cm_data_[c0++] = c.nseg_fw;
+                cm_data_[c0++] = c.nseg_bw;
+            }
+            ir++;
+        }
+    });
 
     fill_xs();
     VecF areas = m.areas();
