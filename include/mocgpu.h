@@ -390,8 +390,8 @@ int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b);
 /* Build the band plan of a problem on the host (no device) and verify its
  * invariants: every segment once per direction in the streams (backward
  * reversed), padding, band-local FSR slots, slot lists, band orders, current
- * events. Writes a one-line report. No reference counterpart (a test of this
- * build's data layout). */
+ * events. Writes a one-line report that ends with a digest of the plan's
+ * arrays. No reference counterpart (a test of this build's data layout). */
 int mocgpu_plan_check(const mocgpu_problem *p, int32_t band_rays, char *msg, int32_t cap);
 
 /* Number of (2-D segment, polar, direction, group) updates per inner sweep of

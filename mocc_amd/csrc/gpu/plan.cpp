@@ -1,10 +1,15 @@
 #include "plan.hpp"
 
 #include <algorithm>
-#include <cstring>
+#include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace mocc_amd {
 namespace gpu {
@@ -96,6 +101,216 @@ void ray_events(const mocgpu_problem &p, const Coarse &cm, int r, int nseg, std:
         cell_bw = cm.neighbor(cell_bw, crd[1]);
     }
 }
+
+// Per-thread scratch of the band builder: FSR and surface marks stamped with
+// a per-thread band counter, per-FSR and per-surface contribution counts
+// (cleared again after each band) and the current band's event lists.
+struct BandScratch {
+    std::vector<int> stamp, g2l, sstamp, s2l;
+    std::vector<int32_t> fsr_cnt, surf_cnt;
+    std::vector<std::vector<Event>> evf, evb;
+    int band_id = 0;
+    BandScratch(int F, int n_surf)
+        : stamp(F, -1), g2l(F, -1), sstamp(n_surf, -1), s2l(n_surf, -1), fsr_cnt(F, 0), surf_cnt(n_surf, 0)
+    {
+    }
+};
+
+// The bands of one azimuth. Their offsets (loc_base, ray_base, seg_base,
+// lsurf_base and the event offsets) count from this azimuth's own arrays
+// until the azimuths are joined in order.
+struct AziBands {
+    std::vector<BandInfo> bands;
+    std::vector<int32_t> loc_fsr, ray_global, lsurf_glob, ev_off;
+    std::vector<uint16_t> seg_loc;
+    std::vector<float> seg_len;
+    std::vector<RayInfo> rays;
+    std::vector<uint32_t> ev;
+};
+
+// Bands of azimuth a: its rays in spatial order, grown into bands whose
+// touched FSRs fit the band-local index space, each with its chunked forward
+// and backward streams, ray records and coarse-surface events.
+void build_azimuth(const mocgpu_problem &p, const Coarse &cm, int a, int max_rays, BandScratch &sc, AziBands &pt)
+{
+    const int NP = p.n_polar, n_az_oct = p.n_azi / 2;
+    const double hx = p.x_vec[p.nx], hy = p.y_vec[p.ny];
+    const int oct = a / n_az_oct;  // 0: octant 1, 1: octant 2
+    const int r0 = p.azi_ray_off[a], r1 = p.azi_ray_off[a + 1];
+    const int Ny = p.azi_ny[a], Nx = p.azi_nx[a];
+    const double alpha = p.ang_alpha[a * NP];
+    const double sx = hx / Nx, sy = hy / Ny;
+    // spatial order: distance of the entry point along the normal
+    std::vector<std::pair<double, int>> order;
+    for (int r = r0; r < r1; r++) {
+        int b0 = p.ray_bc[2 * r];
+        double x, y;
+        if (b0 < Ny) {
+            x = oct == 0 ? 0.0 : hx;
+            y = (0.5 + b0) * sy;
+        } else {
+            x = (0.5 + (b0 - Ny)) * sx;
+            y = 0.0;
+        }
+        order.push_back({-std::sin(alpha) * x + std::cos(alpha) * y, r});
+    }
+    std::sort(order.begin(), order.end());
+    size_t i = 0;
+    while (i < order.size()) {
+        // grow a band
+        sc.evf.clear();
+        sc.evb.clear();
+        std::vector<int> brays;
+        std::vector<int> locs;
+        int n_local = 0;
+        while (i < order.size() && (int)brays.size() < max_rays) {
+            int r = order[i].second;
+            int new_count = 0;
+            for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                int f = p.seg_fsr[s];
+                if (sc.stamp[f] != sc.band_id && sc.g2l[f] != -2) {
+                    sc.g2l[f] = -2;  // counted for this ray
+                    new_count++;
+                }
+            }
+            for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                int f = p.seg_fsr[s];
+                if (sc.g2l[f] == -2) sc.g2l[f] = -1;
+            }
+            if (!brays.empty() && n_local + new_count > kMaxLocal) break;
+            if (n_local + new_count > kMaxLocal)
+                throw std::runtime_error("a single ray touches more FSRs than a band holds");
+            for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
+                int f = p.seg_fsr[s];
+                if (sc.stamp[f] != sc.band_id) {
+                    sc.stamp[f] = sc.band_id;
+                    locs.push_back(f);
+                    n_local++;
+                }
+            }
+            brays.push_back(r);
+            i++;
+        }
+        std::sort(locs.begin(), locs.end());
+        for (size_t l = 0; l < locs.size(); l++) sc.g2l[locs[l]] = (int)l;
+        BandInfo b{};
+        b.azi = a;
+        b.n_rays = (int)brays.size();
+        b.n_local = n_local;
+        b.loc_base = (int32_t)pt.loc_fsr.size();
+        b.ray_base = (int32_t)pt.rays.size();
+        b.nmax = 0;
+        for (int r : brays) b.nmax = std::max(b.nmax, (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]));
+        if (b.nmax > 65535) throw std::runtime_error("ray longer than 65535 segments");
+        for (int f : locs) pt.loc_fsr.push_back(f);
+        // forward and backward streams, chunked layout
+        const int nchunk = (b.nmax + kChunk - 1) / kChunk;
+        for (int d = 0; d < 2; d++) {
+            b.seg_base[d] = (int64_t)pt.seg_loc.size();
+            for (int c = 0; c < nchunk; c++)
+                for (int j = 0; j < b.n_rays; j++) {
+                    int r = brays[j];
+                    int64_t s0 = p.ray_seg_off[r];
+                    int n = (int)(p.ray_seg_off[r + 1] - s0);
+                    for (int u = 0; u < kChunk; u++) {
+                        int t = c * kChunk + u;
+                        if (t < n) {
+                            int k = d == 0 ? t : n - 1 - t;
+                            pt.seg_loc.push_back((uint16_t)(sc.g2l[p.seg_fsr[s0 + k]] * kSlotBytes));
+                            pt.seg_len.push_back((float)p.seg_len[s0 + k]);
+                        } else {
+                            // padding: a zero-length step into the lane's own dummy slot
+                            pt.seg_loc.push_back((uint16_t)((n_local + j) * kSlotBytes));
+                            pt.seg_len.push_back(0.0f);
+                        }
+                    }
+                }
+        }
+        // contributions per local FSR: every segment, both directions
+        b.max_contrib = 1;
+        for (int r : brays)
+            for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) sc.fsr_cnt[p.seg_fsr[s]] += 2;
+        for (int f : locs) {
+            b.max_contrib = std::max(b.max_contrib, sc.fsr_cnt[f]);
+            sc.fsr_cnt[f] = 0;
+        }
+        // ray records and coarse events
+        b.lsurf_base = (int32_t)pt.lsurf_glob.size();
+        std::vector<int> lsurfs;
+        for (int j = 0; j < b.n_rays; j++) {
+            int r = brays[j];
+            int n = (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]);
+            RayInfo ri{};
+            ri.bc0 = p.ray_bc[2 * r];
+            ri.bc1 = p.ray_bc[2 * r + 1];
+            ri.nseg = n;
+            int fw_y = ri.bc1 >= Ny ? 1 : 0;
+            int bw_y = ri.bc0 >= Ny ? 1 : 0;
+            // exit faces: octant 1 fw leaves E/N, octant 2 fw leaves W/N,
+            // their reverses leave W/S and E/S
+            int fw_face = fw_y ? NORTH : (oct == 0 ? EAST : WEST);
+            int bw_face = bw_y ? SOUTH : (oct == 0 ? WEST : EAST);
+            ri.flags = fw_y | (bw_y << 1) | ((p.bc[fw_face] == MOCGPU_BC_VACUUM) << 2) |
+                       ((p.bc[bw_face] == MOCGPU_BC_VACUUM) << 3);
+            pt.rays.push_back(ri);
+            pt.ray_global.push_back(r);
+            std::vector<Event> ef, eb;
+            ray_events(p, cm, r, n, ef, eb);
+            for (int d = 0; d < 2; d++) {
+                for (const Event &e : (d == 0 ? ef : eb)) {
+                    if (sc.sstamp[e.surf] != sc.band_id) {
+                        sc.sstamp[e.surf] = sc.band_id;
+                        lsurfs.push_back(e.surf);
+                    }
+                    sc.surf_cnt[e.surf]++;
+                }
+            }
+            sc.evf.push_back(std::move(ef));
+            sc.evb.push_back(std::move(eb));
+        }
+        b.max_surf_contrib = 1;
+        for (int s : lsurfs) {
+            b.max_surf_contrib = std::max(b.max_surf_contrib, sc.surf_cnt[s]);
+            sc.surf_cnt[s] = 0;
+        }
+        std::sort(lsurfs.begin(), lsurfs.end());
+        if (lsurfs.size() > (size_t)kMaxLSurf) throw std::runtime_error("too many coarse surfaces in one band");
+        for (size_t l = 0; l < lsurfs.size(); l++) sc.s2l[lsurfs[l]] = (int)l;
+        b.n_lsurf = (int)lsurfs.size();
+        for (int s : lsurfs) pt.lsurf_glob.push_back(s);
+        for (int j = 0; j < b.n_rays; j++) {
+            const size_t gi = (size_t)j;
+            for (int d = 0; d < 2; d++) {
+                for (const Event &e : (d == 0 ? sc.evf[gi] : sc.evb[gi])) {
+                    if (e.step > 65535) throw std::runtime_error("event step out of range");
+                    pt.ev.push_back((uint32_t)e.step | ((uint32_t)sc.s2l[e.surf] << 16) |
+                                    ((uint32_t)e.normal << 31));
+                }
+                pt.ev_off.push_back((int32_t)pt.ev.size());
+            }
+        }
+        pt.bands.push_back(b);
+        sc.band_id++;
+    }
+}
+
+// Threads for the band builder: MOCC_HOST_THREADS, else OMP_NUM_THREADS, else
+// the hardware's count (at most 64); one thread for problems under a million
+// segments, where starting threads costs more than it saves.
+int plan_threads(int64_t n_seg, int n_azi)
+{
+    if (n_seg < 1000000) return 1;
+    int n = 0;
+    for (const char *name : {"MOCC_HOST_THREADS", "OMP_NUM_THREADS"}) {
+        const char *v = std::getenv(name);
+        if (v && std::atoi(v) > 0) {
+            n = std::atoi(v);
+            break;
+        }
+    }
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min({n, 64, n_azi}));
+}
 }  // namespace
 
 SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
@@ -112,8 +327,7 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
     for (int s = 0; s < 4; s++)
         if (p.bc[s] != MOCGPU_BC_VACUUM && p.bc[s] != MOCGPU_BC_REFLECT)
             throw std::runtime_error("only vacuum and reflective radial boundaries are supported");
-    const int nd = pl.nd, NP = pl.NP, A = pl.A, n_az_oct = A / 2;
-    const double hx = p.x_vec[p.nx], hy = p.y_vec[p.ny];
+    const int nd = pl.nd, NP = pl.NP, A = pl.A;
 
     // boundary layout (moc_sweeper.cpp:38-52, boundary_condition.cpp:33-78)
     pl.ang_off.assign(4 * nd, 0);
@@ -183,168 +397,74 @@ SweepPlan build_plan(const mocgpu_problem &p, int max_rays)
         }
     }
 
-    std::vector<int> stamp(pl.F, -1), g2l(pl.F, -1);
-    std::vector<int> sstamp(pl.n_surf, -1), s2l(pl.n_surf, -1);
-    std::vector<int32_t> fsr_cnt(pl.F, 0), surf_cnt(pl.n_surf, 0);
-    std::vector<std::vector<Event>> evf, evb;
+    // The azimuths' bands are built independently (on several threads for a
+    // large problem) and joined in azimuth order, so the plan does not depend
+    // on the thread count. An error is reported for the lowest azimuth that
+    // has one, as a sequential build would.
+    std::vector<AziBands> parts(A);
+    {
+        const int nt = plan_threads(p.n_seg, A);
+        std::vector<std::exception_ptr> errs(A);
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            BandScratch sc(pl.F, pl.n_surf);
+            for (int a = next++; a < A; a = next++) {
+                try {
+                    build_azimuth(p, cm, a, max_rays, sc, parts[a]);
+                } catch (...) {
+                    errs[a] = std::current_exception();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    }
+    {
+        size_t n_seg = 0, n_loc = 0, n_ray = 0, n_ls = 0, n_ev = 0, n_b = 0;
+        for (const AziBands &pt : parts) {
+            n_seg += pt.seg_loc.size();
+            n_loc += pt.loc_fsr.size();
+            n_ray += pt.rays.size();
+            n_ls += pt.lsurf_glob.size();
+            n_ev += pt.ev.size();
+            n_b += pt.bands.size();
+        }
+        pl.seg_loc.reserve(n_seg);
+        pl.seg_len.reserve(n_seg);
+        pl.loc_fsr.reserve(n_loc);
+        pl.rays.reserve(n_ray);
+        pl.ray_global.reserve(n_ray);
+        pl.lsurf_glob.reserve(n_ls);
+        pl.ev.reserve(n_ev);
+        pl.ev_off.reserve(2 * n_ray + 1);
+        pl.bands.reserve(n_b);
+    }
     pl.ev_off.push_back(0);
-    int band_id = 0;
-    for (int a = 0; a < A; a++) {
-        const int oct = a / n_az_oct;  // 0: octant 1, 1: octant 2
-        const int r0 = p.azi_ray_off[a], r1 = p.azi_ray_off[a + 1];
-        const int Ny = p.azi_ny[a], Nx = p.azi_nx[a];
-        const double alpha = p.ang_alpha[a * NP];
-        const double sx = hx / Nx, sy = hy / Ny;
-        // spatial order: distance of the entry point along the normal
-        std::vector<std::pair<double, int>> order;
-        for (int r = r0; r < r1; r++) {
-            int b0 = p.ray_bc[2 * r];
-            double x, y;
-            if (b0 < Ny) {
-                x = oct == 0 ? 0.0 : hx;
-                y = (0.5 + b0) * sy;
-            } else {
-                x = (0.5 + (b0 - Ny)) * sx;
-                y = 0.0;
-            }
-            order.push_back({-std::sin(alpha) * x + std::cos(alpha) * y, r});
-        }
-        std::sort(order.begin(), order.end());
-        size_t i = 0;
-        while (i < order.size()) {
-            // grow a band
-            std::vector<int> brays;
-            std::vector<int> locs;
-            int n_local = 0;
-            while (i < order.size() && (int)brays.size() < max_rays) {
-                int r = order[i].second;
-                int new_count = 0;
-                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
-                    int f = p.seg_fsr[s];
-                    if (stamp[f] != band_id && g2l[f] != -2) {
-                        g2l[f] = -2;  // counted for this ray
-                        new_count++;
-                    }
-                }
-                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
-                    int f = p.seg_fsr[s];
-                    if (g2l[f] == -2) g2l[f] = -1;
-                }
-                if (!brays.empty() && n_local + new_count > kMaxLocal) break;
-                if (n_local + new_count > kMaxLocal)
-                    throw std::runtime_error("a single ray touches more FSRs than a band holds");
-                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) {
-                    int f = p.seg_fsr[s];
-                    if (stamp[f] != band_id) {
-                        stamp[f] = band_id;
-                        locs.push_back(f);
-                        n_local++;
-                    }
-                }
-                brays.push_back(r);
-                i++;
-            }
-            std::sort(locs.begin(), locs.end());
-            for (size_t l = 0; l < locs.size(); l++) g2l[locs[l]] = (int)l;
-            BandInfo b{};
-            b.azi = a;
-            b.n_rays = (int)brays.size();
-            b.n_local = n_local;
-            b.loc_base = (int32_t)pl.loc_fsr.size();
-            b.ray_base = (int32_t)pl.rays.size();
-            b.nmax = 0;
-            for (int r : brays) b.nmax = std::max(b.nmax, (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]));
-            if (b.nmax > 65535) throw std::runtime_error("ray longer than 65535 segments");
-            for (int f : locs) pl.loc_fsr.push_back(f);
-            // forward and backward streams, chunked layout
-            const int nchunk = (b.nmax + kChunk - 1) / kChunk;
-            for (int d = 0; d < 2; d++) {
-                b.seg_base[d] = (int64_t)pl.seg_loc.size();
-                for (int c = 0; c < nchunk; c++)
-                    for (int j = 0; j < b.n_rays; j++) {
-                        int r = brays[j];
-                        int64_t s0 = p.ray_seg_off[r];
-                        int n = (int)(p.ray_seg_off[r + 1] - s0);
-                        for (int u = 0; u < kChunk; u++) {
-                            int t = c * kChunk + u;
-                            if (t < n) {
-                                int k = d == 0 ? t : n - 1 - t;
-                                pl.seg_loc.push_back((uint16_t)(g2l[p.seg_fsr[s0 + k]] * kSlotBytes));
-                                pl.seg_len.push_back((float)p.seg_len[s0 + k]);
-                            } else {
-                                // padding: a zero-length step into the lane's own dummy slot
-                                pl.seg_loc.push_back((uint16_t)((n_local + j) * kSlotBytes));
-                                pl.seg_len.push_back(0.0f);
-                            }
-                        }
-                    }
-            }
-            // contributions per local FSR: every segment, both directions
-            b.max_contrib = 1;
-            for (int r : brays)
-                for (int64_t s = p.ray_seg_off[r]; s < p.ray_seg_off[r + 1]; s++) fsr_cnt[p.seg_fsr[s]] += 2;
-            for (int f : locs) {
-                b.max_contrib = std::max(b.max_contrib, fsr_cnt[f]);
-                fsr_cnt[f] = 0;
-            }
-            // ray records and coarse events
-            b.lsurf_base = (int32_t)pl.lsurf_glob.size();
-            std::vector<int> lsurfs;
-            for (int j = 0; j < b.n_rays; j++) {
-                int r = brays[j];
-                int n = (int)(p.ray_seg_off[r + 1] - p.ray_seg_off[r]);
-                RayInfo ri{};
-                ri.bc0 = p.ray_bc[2 * r];
-                ri.bc1 = p.ray_bc[2 * r + 1];
-                ri.nseg = n;
-                int fw_y = ri.bc1 >= Ny ? 1 : 0;
-                int bw_y = ri.bc0 >= Ny ? 1 : 0;
-                // exit faces: octant 1 fw leaves E/N, octant 2 fw leaves W/N,
-                // their reverses leave W/S and E/S
-                int fw_face = fw_y ? NORTH : (oct == 0 ? EAST : WEST);
-                int bw_face = bw_y ? SOUTH : (oct == 0 ? WEST : EAST);
-                ri.flags = fw_y | (bw_y << 1) | ((p.bc[fw_face] == MOCGPU_BC_VACUUM) << 2) |
-                           ((p.bc[bw_face] == MOCGPU_BC_VACUUM) << 3);
-                pl.rays.push_back(ri);
-                pl.ray_global.push_back(r);
-                std::vector<Event> ef, eb;
-                ray_events(p, cm, r, n, ef, eb);
-                for (int d = 0; d < 2; d++) {
-                    for (const Event &e : (d == 0 ? ef : eb)) {
-                        if (sstamp[e.surf] != band_id) {
-                            sstamp[e.surf] = band_id;
-                            lsurfs.push_back(e.surf);
-                        }
-                        surf_cnt[e.surf]++;
-                    }
-                }
-                evf.push_back(std::move(ef));
-                evb.push_back(std::move(eb));
-            }
-            b.max_surf_contrib = 1;
-            for (int s : lsurfs) {
-                b.max_surf_contrib = std::max(b.max_surf_contrib, surf_cnt[s]);
-                surf_cnt[s] = 0;
-            }
-            std::sort(lsurfs.begin(), lsurfs.end());
-            if (lsurfs.size() > (size_t)kMaxLSurf) throw std::runtime_error("too many coarse surfaces in one band");
-            for (size_t l = 0; l < lsurfs.size(); l++) s2l[lsurfs[l]] = (int)l;
-            b.n_lsurf = (int)lsurfs.size();
-            for (int s : lsurfs) pl.lsurf_glob.push_back(s);
-            for (int j = 0; j < b.n_rays; j++) {
-                size_t gi = evf.size() - b.n_rays + j;
-                for (int d = 0; d < 2; d++) {
-                    for (const Event &e : (d == 0 ? evf[gi] : evb[gi])) {
-                        if (e.step > 65535) throw std::runtime_error("event step out of range");
-                        pl.ev.push_back((uint32_t)e.step | ((uint32_t)s2l[e.surf] << 16) |
-                                        ((uint32_t)e.normal << 31));
-                    }
-                    pl.ev_off.push_back((int32_t)pl.ev.size());
-                }
-            }
+    for (AziBands &pt : parts) {
+        const int32_t loc0 = (int32_t)pl.loc_fsr.size(), ray0 = (int32_t)pl.rays.size();
+        const int32_t ls0 = (int32_t)pl.lsurf_glob.size(), ev0 = (int32_t)pl.ev.size();
+        const int64_t seg0 = (int64_t)pl.seg_loc.size();
+        for (BandInfo b : pt.bands) {
+            b.loc_base += loc0;
+            b.ray_base += ray0;
+            b.lsurf_base += ls0;
+            b.seg_base[0] += seg0;
+            b.seg_base[1] += seg0;
             pl.bands.push_back(b);
-            band_id++;
         }
+        pl.seg_loc.insert(pl.seg_loc.end(), pt.seg_loc.begin(), pt.seg_loc.end());
+        pl.seg_len.insert(pl.seg_len.end(), pt.seg_len.begin(), pt.seg_len.end());
+        pl.loc_fsr.insert(pl.loc_fsr.end(), pt.loc_fsr.begin(), pt.loc_fsr.end());
+        pl.rays.insert(pl.rays.end(), pt.rays.begin(), pt.rays.end());
+        pl.ray_global.insert(pl.ray_global.end(), pt.ray_global.begin(), pt.ray_global.end());
+        pl.lsurf_glob.insert(pl.lsurf_glob.end(), pt.lsurf_glob.begin(), pt.lsurf_glob.end());
+        pl.ev.insert(pl.ev.end(), pt.ev.begin(), pt.ev.end());
+        for (int32_t e : pt.ev_off) pl.ev_off.push_back(e + ev0);
+        pt = AziBands();  // release the azimuth's arrays
     }
     pl.n_seg_padded = (int64_t)pl.seg_loc.size();
     for (const BandInfo &b : pl.bands) {
@@ -484,8 +604,34 @@ std::string check_plan(const mocgpu_problem &p, const SweepPlan &pl, int band_ra
     }
     for (size_t b = 0; b < pl.bands.size(); b++)
         if (o1[b] != 1 || o2[b] != 1) fail("band orders are permutations");
+    // FNV-1a digest of the arrays the device receives, so two builds of one
+    // problem can be compared from the report
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const auto &v) {
+        const unsigned char *b = (const unsigned char *)v.data();
+        const size_t n = v.size() * sizeof(v[0]);
+        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+        h = (h ^ (uint64_t)n) * 1099511628211ull;
+    };
+    mix(pl.bands);
+    mix(pl.band_order);
+    mix(pl.band_order_gs);
+    mix(pl.seg_loc);
+    mix(pl.seg_len);
+    mix(pl.loc_fsr);
+    mix(pl.rays);
+    mix(pl.slot_pos);
+    mix(pl.fsr_slot_ptr);
+    mix(pl.fsr_slot_idx);
+    mix(pl.lsurf_glob);
+    mix(pl.ev_off);
+    mix(pl.ev);
+    mix(pl.surf_slot_ptr);
+    mix(pl.surf_slot_idx);
+    char dig[17];
+    std::snprintf(dig, sizeof dig, "%016llx", (unsigned long long)h);
     return "ok: " + std::to_string(pl.bands.size()) + " bands, " + std::to_string(pl.loc_fsr.size()) +
-           " slots, " + std::to_string(pl.n_seg_padded) + " stream steps";
+           " slots, " + std::to_string(pl.n_seg_padded) + " stream steps, digest " + dig;
 }
 }  // namespace
 

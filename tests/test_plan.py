@@ -35,3 +35,22 @@ def test_plan_assembly17(deck_dir):
     hp = HostProblem.load(lg.write_config("assembly17", deck_dir))
     rc, msg = _check(hp, 64)
     assert rc == 0, msg
+
+
+@pytest.mark.parametrize("width", [64, 32])
+def test_plan_independent_of_thread_count(deck_dir, monkeypatch, width):
+    """A problem of over a million segments builds its azimuths' bands on
+    several threads (MOCC_HOST_THREADS) and joins them in azimuth order: the
+    plan's digest is the same at any thread count."""
+    from mocc_amd import lattice_gen as lg
+    from mocc_amd.native import HostProblem
+    hp = HostProblem.load(lg.write_deck(lg.pin_lattice(24, 2, 8, 0.05, "plan24v", cmfd=True,
+                                                       bc={"east": "vacuum", "south": "vacuum"}), deck_dir, 24))
+    assert hp.p.n_seg > 1_000_000
+    digests = set()
+    for n in ("1", "3", "8"):
+        monkeypatch.setenv("MOCC_HOST_THREADS", n)
+        rc, msg = _check(hp, width)
+        assert rc == 0, msg
+        digests.add(msg.rsplit("digest ", 1)[1])
+    assert len(digests) == 1
