@@ -344,8 +344,13 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
 /* ---------------------------------------------------------------------------
  * Batch engine: many independent problems advanced together. No reference
  * counterpart (mocc solves one problem per run); it serves cross-section
- * branch cases and many-lattice workloads. A batch does not own its handles:
- * each handle must stay alive until it is removed from the batch.
+ * branch cases and many-lattice workloads. A batch does not own its handles,
+ * but from mocgpu_batch_add until mocgpu_batch_remove (or the batch's
+ * destruction) its launches own the handle's device state: single-problem
+ * entry points that launch work on a member or change its state return
+ * MOCGPU_ERR_ARG, reads of its flux, boundary fluxes and currents are
+ * allowed, a handle is in at most one batch, and mocgpu_destroy of a member
+ * takes it out of its batch first.
  *
  * Every unfinished problem advances by whole outer iterations of
  * EigenSolver::step (eigen_solver.cpp:225-253) with Jacobi energy sweeps and

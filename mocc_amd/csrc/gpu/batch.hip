@@ -225,6 +225,9 @@ struct mocgpu_batch {
 
     ~mocgpu_batch()
     {
+        // the members are the caller's again once the batch's work is done
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (const Case &c : cases) c.h->batch = nullptr;
         drop_graph();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -499,8 +502,8 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
             b->err = "all problems of a batch must have the same number of polar angles";
             return (int)MOCGPU_ERR_UNSUPPORTED;
         }
-        for (const auto &c : b->cases)
-            if (c.h == h) throw std::runtime_error("handle already in the batch");
+        if (h->batch == b) throw std::runtime_error("handle already in the batch");
+        if (h->batch) throw std::runtime_error("handle is a member of another batch");
         // the host status mirrors must be current before the composition changes
         b->sync();
         if (!b->dirty) b->pull_status();
@@ -538,6 +541,7 @@ int mocgpu_batch_add(mocgpu_batch *b, mocgpu_handle *h, double k_tol, double psi
         b->used += bytes;
         b->NP = h->NP;
         b->dirty = true;
+        h->batch = b;
         *id = c.id;
         return 0;
     });
@@ -591,6 +595,7 @@ int mocgpu_batch_remove(mocgpu_batch *b, int32_t id)
         b->sync();
         if (!b->dirty) b->pull_status();
         b->used -= b->cases[i].bytes;
+        b->cases[i].h->batch = nullptr;
         b->cases.erase(b->cases.begin() + i);
         b->dirty = true;
         return 0;
@@ -602,3 +607,13 @@ int32_t mocgpu_batch_size(const mocgpu_batch *b) { return b ? (int32_t)b->cases.
 int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b) { return b ? b->used : 0; }
 
 }  // extern "C"
+
+void batch_detach(mocgpu_batch *b, mocgpu_handle *h)
+{
+    for (const auto &c : b->cases)
+        if (c.h == h) {
+            (void)mocgpu_batch_remove(b, c.id);
+            break;
+        }
+    h->batch = nullptr;
+}

@@ -95,9 +95,16 @@ inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 using namespace mocc_amd::gpu;  // the handle is the C ABI's global type; its members use the library's types
 
+// Take h out of batch b (mocgpu_destroy of a member handle); defined with the
+// batch engine.
+void batch_detach(mocgpu_batch *b, struct mocgpu_handle *h);
+
 struct mocgpu_handle {
     std::string err;
     mocgpu_options opt{};
+    // The batch this handle is a member of (mocgpu_batch_add), or null. While
+    // it is set, the batch's launches own the handle's device state.
+    mocgpu_batch *batch = nullptr;
     std::shared_ptr<SweepPlan> plan;  // host plan, shared by the handles of one geometry
     int F = 0, G = 0, NP = 0, A = 0, nd = 0, nx = 0, ny = 0, npin = 0, n_surf = 0, n_mat = 0;
     int n_fissile = 0;

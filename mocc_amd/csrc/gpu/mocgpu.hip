@@ -285,6 +285,16 @@ template <typename F> int guard(mocgpu_handle *h, F &&f)
 
 thread_local std::string g_create_err;
 
+// Entry points that launch work on a handle or change its device state refuse
+// a handle that a batch owns (from mocgpu_batch_add to mocgpu_batch_remove):
+// the batch's launches use the same buffers on the batch's stream.
+void require_free(const mocgpu_handle *h, const char *fn)
+{
+    if (h->batch)
+        throw std::runtime_error(std::string(fn) +
+                                 ": the handle is a member of a batch (take it out with mocgpu_batch_remove first)");
+}
+
 }  // namespace
 
 extern "C" {
@@ -627,6 +637,9 @@ int64_t mocgpu_geometry_bytes(const mocgpu_handle *h)
 
 int mocgpu_destroy(mocgpu_handle *h)
 {
+    // a member handle leaves its batch first, so the batch never launches
+    // on freed buffers
+    if (h && h->batch) batch_detach(h->batch, h);
     delete h;
     return 0;
 }
@@ -635,6 +648,7 @@ int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p)
 {
     if (!p) return MOCGPU_ERR_ARG;
     return guard(h, [&] {
+        require_free(h, "mocgpu_reload_xs");
         bool same = p->n_fsr == h->F && p->n_group == h->G && p->n_mat == h->n_mat && p->n_seg == h->n_seg &&
                     p->n_polar == h->NP && p->n_azi == h->A && p->nx == h->nx && p->ny == h->ny;
         for (int f = 0; same && f < p->n_fsr; f++) same = p->fsr_mat[f] == h->h_fsr_mat[f];
@@ -658,6 +672,7 @@ int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p)
 int mocgpu_initialize(mocgpu_handle *h)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_initialize");
         const size_t fg = (size_t)h->F * h->G, bcpg = (size_t)h->plan->bcpg, n_psi = (size_t)h->G * 2 * bcpg;
         const size_t n = std::max(fg, n_psi);
         hipLaunchKernelGGL(k_init_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_phi.p,
@@ -681,6 +696,7 @@ int mocgpu_initialize(mocgpu_handle *h)
 int mocgpu_set_flux(mocgpu_handle *h, const double *phi)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_set_flux");
         HIP_CHECK(hipMemcpyAsync(h->d_phi.p, phi, sizeof(double) * h->F * h->G, hipMemcpyHostToDevice, h->stream));
         h->sync();
         return 0;
@@ -696,7 +712,7 @@ int mocgpu_get_flux(mocgpu_handle *h, double *phi)
     });
 }
 
-int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return (int64_t)h->G * h->plan->bcpg; }
+int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return h ? (int64_t)h->G * h->plan->bcpg : 0; }
 
 int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
 {
@@ -717,6 +733,7 @@ int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
 int mocgpu_set_boundary(mocgpu_handle *h, const double *bc, int64_t n)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_set_boundary");
         if (n != (int64_t)h->G * h->plan->bcpg) throw std::runtime_error("boundary size mismatch");
         std::vector<float> tmp(h->plan->bcpg);
         for (int g = 0; g < h->G; g++) {
@@ -733,6 +750,7 @@ int mocgpu_set_boundary(mocgpu_handle *h, const double *bc, int64_t n)
 int mocgpu_store_old_flux(mocgpu_handle *h)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_store_old_flux");
         h->copy_flux_to_old();
         h->sync();
         return 0;
@@ -742,6 +760,7 @@ int mocgpu_store_old_flux(mocgpu_handle *h)
 int mocgpu_calc_fission_source(mocgpu_handle *h, double k, double *fs_out)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_calc_fission_source");
         hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G,
                            h->d_fsr_mat.p, h->d_mat_nf.p, h->d_phi.p, (const double *)nullptr, k, h->d_fs.p);
         h->has_fs = true;
@@ -755,6 +774,7 @@ int mocgpu_calc_fission_source(mocgpu_handle *h, double k, double *fs_out)
 int mocgpu_set_fission_source(mocgpu_handle *h, const double *fs)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_set_fission_source");
         HIP_CHECK(hipMemcpyAsync(h->d_fs.p, fs, sizeof(double) * h->F, hipMemcpyHostToDevice, h->stream));
         h->has_fs = true;
         h->sync();
@@ -765,6 +785,7 @@ int mocgpu_set_fission_source(mocgpu_handle *h, const double *fs)
 int mocgpu_build_source(mocgpu_handle *h, int32_t g)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_build_source");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         h->build_source(g, h->opt.energy_mode == 1);
         h->sync();
@@ -775,6 +796,7 @@ int mocgpu_build_source(mocgpu_handle *h, int32_t g)
 int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_sweep");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         h->mocsweep(g, with_current != 0);
         h->sync();
@@ -785,6 +807,7 @@ int mocgpu_sweep(mocgpu_handle *h, int32_t g, int32_t with_current)
 int mocgpu_sweep_all(mocgpu_handle *h, int32_t with_current)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_sweep_all");
         h->sweep_once(-1, with_current != 0);
         return 0;
     });
@@ -801,6 +824,7 @@ int mocgpu_synchronize(mocgpu_handle *h)
 int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_fss_step");
         h->fss_step(with_current != 0);
         h->sync();
         return 0;
@@ -810,6 +834,7 @@ int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current)
 int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_total_fission");
         h->reduce(RED_FISSION, 0);
         double v[4];
         HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -822,6 +847,7 @@ int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out)
 int mocgpu_flux_residual(mocgpu_handle *h, double *out)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_flux_residual");
         h->reduce(RED_RESID, 0);
         double v[4];
         HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -834,6 +860,7 @@ int mocgpu_flux_residual(mocgpu_handle *h, double *out)
 int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_get_pin_flux_1g");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!pin) throw std::runtime_error("null pin flux array");
         hipLaunchKernelGGL(k_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
@@ -847,6 +874,7 @@ int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
 int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin, double *resid)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_set_pin_flux_1g");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!pin) throw std::runtime_error("null pin flux array");
         // moc_sweeper.cpp:367-375 throws on a non-positive input flux
@@ -869,7 +897,7 @@ int mocgpu_set_pin_flux_1g(mocgpu_handle *h, int32_t g, const double *pin, doubl
     });
 }
 
-int32_t mocgpu_n_surf(const mocgpu_handle *h) { return h->n_surf; }
+int32_t mocgpu_n_surf(const mocgpu_handle *h) { return h ? h->n_surf : 0; }
 
 int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *surface_flux)
 {
@@ -890,6 +918,7 @@ int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *su
 int mocgpu_set_currents(mocgpu_handle *h, int32_t g, const double *current, const double *surface_flux)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_set_currents");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         std::vector<double> c((size_t)h->n_surf * h->G), s((size_t)h->n_surf * h->G);
         HIP_CHECK(hipMemcpyAsync(c.data(), h->d_cur.p, c.size() * 8, hipMemcpyDeviceToHost, h->stream));
@@ -923,11 +952,12 @@ int32_t mocgpu_attenuation(const float *tau, float *out, int32_t n)
     return e == hipSuccess ? 0 : MOCGPU_ERR_DEVICE;
 }
 
-int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return 2LL * h->NP * h->n_seg * h->G; }
+int64_t mocgpu_updates_per_sweep(const mocgpu_handle *h) { return h ? 2LL * h->NP * h->n_seg * h->G : 0; }
 
 int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double *kernel_ms)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_time_sweeps");
         if (n <= 0) throw std::runtime_error("n_sweeps must be positive");
         // the launches of sweep_once(-1, false), with events around the sweep kernels
         auto launch = [&](bool timed) {
@@ -961,6 +991,7 @@ int mocgpu_time_sweeps(mocgpu_handle *h, int32_t n, double *ms_per_sweep, double
 int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *st)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_eigen_iterate");
         HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, &st->k, 8, hipMemcpyHostToDevice, h->stream));
         if (h->opt.use_cmfd) h->cmfd->enqueue_step(h->cmfd_view(), h->stream);
         hipLaunchKernelGGL(k_fission_source, dim3(nblk(h->F)), dim3(256), 0, h->stream, h->F, h->G, h->d_fsr_mat.p,
@@ -1055,6 +1086,7 @@ int mocgpu_eigen_solve_history(mocgpu_handle *h, double k_tol, double psi_tol, i
 int mocgpu_cmfd_step(mocgpu_handle *h, double *k)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_cmfd_step");
         if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
         HIP_CHECK(hipMemcpyAsync(h->d_scal.p + S_K, k, 8, hipMemcpyHostToDevice, h->stream));
         int rc = h->cmfd_step_sync(&h->cmfd_iterations);
@@ -1065,13 +1097,14 @@ int mocgpu_cmfd_step(mocgpu_handle *h, double *k)
     });
 }
 
-int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h) { return h->cmfd_iterations; }
+int32_t mocgpu_cmfd_iterations(const mocgpu_handle *h) { return h ? h->cmfd_iterations : 0; }
 
 int32_t mocgpu_cmfd_bicgstab_iterations(const mocgpu_handle *h) { return h ? h->cmfd_bicgstab : 0; }
 
 int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_cmfd_flux");
         if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
         std::vector<double> f;
         h->cmfd->coarse_flux(h->stream, f);
@@ -1083,6 +1116,7 @@ int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
 int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *col, double *val, int32_t cap)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_cmfd_matrix");
         if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         std::vector<int32_t> rp, cl;
@@ -1104,6 +1138,7 @@ int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *c
 int mocgpu_sweep_2d3d(mocgpu_handle *h, int32_t g)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_sweep_2d3d");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!h->corr || !h->corr->full())
             throw std::runtime_error("the handle was created without the correction worker "
@@ -1157,6 +1192,7 @@ int mocgpu_get_corrections(mocgpu_handle *h, double *alpha, double *beta)
 int mocgpu_correction_residual(mocgpu_handle *h, int32_t g, double *out)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_correction_residual");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!h->corr || !h->corr->full())
             throw std::runtime_error("the handle was created without the correction worker");
@@ -1170,6 +1206,7 @@ int mocgpu_correction_residual(mocgpu_handle *h, int32_t g, double *out)
 int mocgpu_update_incoming_flux(mocgpu_handle *h)
 {
     return guard(h, [&] {
+        require_free(h, "mocgpu_update_incoming_flux");
         if (!h->opt.use_cmfd || !h->corr)
             throw std::runtime_error("update_incoming_flux needs the CMFD partial currents (options.use_cmfd = 0)");
         h->corr->update_incoming(h->d_psi.p, h->parity, h->G, h->cmfd->partial(), h->cmfd->partial_old(),

@@ -247,6 +247,37 @@ def test_batch_rejects_gauss_seidel_energy(batch_decks):
     b.close()
 
 
+def test_member_handle_is_owned_by_the_batch(batch_decks):
+    """From mocgpu_batch_add to mocgpu_batch_remove the batch's launches own a
+    handle's device state: single-problem calls that launch work on it are
+    refused, a second batch cannot take it, and destroying it takes it out of
+    the batch first, so the others still solve as alone."""
+    from mocc_amd.sweeper import DeviceError, GpuBatch
+    sw = _sweepers(batch_decks, ["pin1", "pins3x3"])
+    b = GpuBatch()
+    i1 = b.add(sw["pin1"], K_TOL, PSI_TOL, MAX_IT)
+    b.add(sw["pins3x3"], K_TOL, PSI_TOL, MAX_IT)
+    with pytest.raises(DeviceError, match="member of a batch"):
+        sw["pin1"].initialize()
+    with pytest.raises(DeviceError, match="member of a batch"):
+        sw["pin1"].eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    other = GpuBatch()
+    with pytest.raises(DeviceError, match="another batch"):
+        other.add(sw["pin1"], K_TOL, PSI_TOL, MAX_IT)
+    assert other.size() == 0
+    other.close()
+    print("destroy a member handle, then solve the rest", flush=True)
+    sw["pins3x3"].close()
+    assert b.size() == 1
+    res = b.solve_all(chunk=4)
+    (cid, st), = list(res.items())
+    assert cid == i1 and st.converged
+    b.remove(i1)
+    rc, st1 = sw["pin1"].eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    assert rc == 0 and abs(st.k - st1.k) < 1e-9
+    b.close()
+
+
 @pytest.mark.parametrize("big", ["lat24", "lat40"])
 def test_batch_cmfd_register_rows_mixed_bitwise(cmfd_decks, big):
     """Beside a mesh solved at 2 (24x24) or 4 (40x40) cells per lane with the
