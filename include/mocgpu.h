@@ -347,9 +347,10 @@ int mocgpu_eigen_iterate(mocgpu_handle *h, mocgpu_eigen_state *state);
  * branch cases and many-lattice workloads. A batch does not own its handles,
  * but from mocgpu_batch_add until mocgpu_batch_remove (or the batch's
  * destruction) its launches own the handle's device state: single-problem
- * entry points that launch work on a member or change its state return
- * MOCGPU_ERR_ARG, reads of its flux, boundary fluxes and currents are
- * allowed, a handle is in at most one batch, and mocgpu_destroy of a member
+ * entry points that change a member's state return MOCGPU_ERR_ARG, reads
+ * (flux, boundary fluxes, currents, total fission, residual, pin fluxes, the
+ * coarse flux and matrix, corrections) first wait for the work the batch has
+ * queued, a handle is in at most one batch, and mocgpu_destroy of a member
  * takes it out of its batch first.
  *
  * Every unfinished problem advances by whole outer iterations of

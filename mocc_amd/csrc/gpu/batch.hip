@@ -608,6 +608,8 @@ int64_t mocgpu_batch_bytes_used(const mocgpu_batch *b) { return b ? b->used : 0;
 
 }  // extern "C"
 
+void batch_sync(mocgpu_batch *b) { b->sync(); }
+
 void batch_detach(mocgpu_batch *b, mocgpu_handle *h)
 {
     for (const auto &c : b->cases)

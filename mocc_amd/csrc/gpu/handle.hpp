@@ -95,9 +95,10 @@ inline int nblk(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
 using namespace mocc_amd::gpu;  // the handle is the C ABI's global type; its members use the library's types
 
-// Take h out of batch b (mocgpu_destroy of a member handle); defined with the
-// batch engine.
+// Take h out of batch b (mocgpu_destroy of a member handle), and wait for the
+// work batch b has queued; defined with the batch engine.
 void batch_detach(mocgpu_batch *b, struct mocgpu_handle *h);
+void batch_sync(mocgpu_batch *b);
 
 struct mocgpu_handle {
     std::string err;

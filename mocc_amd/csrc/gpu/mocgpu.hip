@@ -295,6 +295,13 @@ void require_free(const mocgpu_handle *h, const char *fn)
                                  ": the handle is a member of a batch (take it out with mocgpu_batch_remove first)");
 }
 
+// Reads of a member handle (its flux, boundary fluxes, currents, reductions
+// over them) first wait for the work its batch has queued.
+void settle(const mocgpu_handle *h)
+{
+    if (h->batch) batch_sync(h->batch);
+}
+
 }  // namespace
 
 extern "C" {
@@ -706,6 +713,7 @@ int mocgpu_set_flux(mocgpu_handle *h, const double *phi)
 int mocgpu_get_flux(mocgpu_handle *h, double *phi)
 {
     return guard(h, [&] {
+        settle(h);
         HIP_CHECK(hipMemcpyAsync(phi, h->d_phi.p, sizeof(double) * h->F * h->G, hipMemcpyDeviceToHost, h->stream));
         h->sync();
         return 0;
@@ -717,6 +725,7 @@ int64_t mocgpu_boundary_size(const mocgpu_handle *h) { return h ? (int64_t)h->G 
 int mocgpu_get_boundary(mocgpu_handle *h, double *bc, int64_t n)
 {
     return guard(h, [&] {
+        settle(h);
         if (n != (int64_t)h->G * h->plan->bcpg) throw std::runtime_error("boundary size mismatch");
         std::vector<float> tmp(h->plan->bcpg);
         for (int g = 0; g < h->G; g++) {
@@ -834,7 +843,7 @@ int mocgpu_fss_step(mocgpu_handle *h, int32_t with_current)
 int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_total_fission");
+        settle(h);
         h->reduce(RED_FISSION, 0);
         double v[4];
         HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -847,7 +856,7 @@ int mocgpu_total_fission(mocgpu_handle *h, int32_t old, double *out)
 int mocgpu_flux_residual(mocgpu_handle *h, double *out)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_flux_residual");
+        settle(h);
         h->reduce(RED_RESID, 0);
         double v[4];
         HIP_CHECK(hipMemcpyAsync(v, h->d_red_out.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -860,7 +869,7 @@ int mocgpu_flux_residual(mocgpu_handle *h, double *out)
 int mocgpu_get_pin_flux_1g(mocgpu_handle *h, int32_t g, double *pin)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_get_pin_flux_1g");
+        settle(h);
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!pin) throw std::runtime_error("null pin flux array");
         hipLaunchKernelGGL(k_pin_flux, dim3(nblk(h->npin)), dim3(256), 0, h->stream, h->npin, h->G, g,
@@ -902,6 +911,7 @@ int32_t mocgpu_n_surf(const mocgpu_handle *h) { return h ? h->n_surf : 0; }
 int mocgpu_get_currents(mocgpu_handle *h, int32_t g, double *current, double *surface_flux)
 {
     return guard(h, [&] {
+        settle(h);
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         std::vector<double> c((size_t)h->n_surf * h->G), s((size_t)h->n_surf * h->G);
         HIP_CHECK(hipMemcpyAsync(c.data(), h->d_cur.p, c.size() * 8, hipMemcpyDeviceToHost, h->stream));
@@ -1104,7 +1114,7 @@ int32_t mocgpu_cmfd_bicgstab_iterations(const mocgpu_handle *h) { return h ? h->
 int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_cmfd_flux");
+        settle(h);
         if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
         std::vector<double> f;
         h->cmfd->coarse_flux(h->stream, f);
@@ -1116,7 +1126,7 @@ int mocgpu_cmfd_flux(mocgpu_handle *h, double *out)
 int mocgpu_cmfd_matrix(mocgpu_handle *h, int32_t g, int32_t *row_ptr, int32_t *col, double *val, int32_t cap)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_cmfd_matrix");
+        settle(h);
         if (!h->opt.use_cmfd) throw std::runtime_error("CMFD is not enabled (options.use_cmfd = 0)");
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         std::vector<int32_t> rp, cl;
@@ -1182,6 +1192,7 @@ int mocgpu_sweep_2d3d(mocgpu_handle *h, int32_t g)
 int mocgpu_get_corrections(mocgpu_handle *h, double *alpha, double *beta)
 {
     return guard(h, [&] {
+        settle(h);
         if (!h->corr || !h->corr->full())
             throw std::runtime_error("the handle was created without the correction worker");
         h->corr->corrections(h->stream, alpha, beta);
@@ -1192,7 +1203,7 @@ int mocgpu_get_corrections(mocgpu_handle *h, double *alpha, double *beta)
 int mocgpu_correction_residual(mocgpu_handle *h, int32_t g, double *out)
 {
     return guard(h, [&] {
-        require_free(h, "mocgpu_correction_residual");
+        settle(h);
         if (g < 0 || g >= h->G) throw std::runtime_error("group out of range");
         if (!h->corr || !h->corr->full())
             throw std::runtime_error("the handle was created without the correction worker");

@@ -308,7 +308,7 @@ class GpuBatch:
         done = {}
         pending = []
         outer = 0
-        total = len(self.members)
+        total = self.size()  # members whose handles were destroyed have left the batch
         while len(done) < total and outer < max_outer:
             self.run(chunk)
             outer += chunk

@@ -249,9 +249,9 @@ def test_batch_rejects_gauss_seidel_energy(batch_decks):
 
 def test_member_handle_is_owned_by_the_batch(batch_decks):
     """From mocgpu_batch_add to mocgpu_batch_remove the batch's launches own a
-    handle's device state: single-problem calls that launch work on it are
-    refused, a second batch cannot take it, and destroying it takes it out of
-    the batch first, so the others still solve as alone."""
+    handle's device state: single-problem calls that change it are refused,
+    reads wait for the batch, a second batch cannot take it, and destroying
+    it takes it out of the batch first, so the others still solve as alone."""
     from mocc_amd.sweeper import DeviceError, GpuBatch
     sw = _sweepers(batch_decks, ["pin1", "pins3x3"])
     b = GpuBatch()
@@ -261,6 +261,8 @@ def test_member_handle_is_owned_by_the_batch(batch_decks):
         sw["pin1"].initialize()
     with pytest.raises(DeviceError, match="member of a batch"):
         sw["pin1"].eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    b.run(2)  # no poll: the read below waits for the batch's queued work
+    assert np.all(np.isfinite(sw["pin1"].flux())) and sw["pin1"].total_fission() > 0
     other = GpuBatch()
     with pytest.raises(DeviceError, match="another batch"):
         other.add(sw["pin1"], K_TOL, PSI_TOL, MAX_IT)
