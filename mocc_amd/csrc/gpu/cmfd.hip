@@ -10,8 +10,8 @@
 //   * the coarse power iteration -- fission source, Gauss-Seidel sweep over
 //     groups, one BiCGSTAB solve per group, the k update and the convergence
 //     test -- runs in ONE workgroup: up to 512 lanes for meshes of up to 1024
-//     cells (and for every problem of a batch), and the wide solve (512 or
-//     1024 lanes) for a single problem's larger mesh. Each lane owns K cells
+//     cells, and the wide solve (512 or 1024 lanes) for larger meshes, for a
+//     single problem and for each problem of a batch. Each lane owns K cells
 //     (c = lane + j * lanes) and keeps their BiCGSTAB vectors in registers; a
 //     matrix-vector product stages the operand in LDS, and dot products are
 //     wave butterflies plus a fixed-order sum over the waves, so every lane
@@ -30,13 +30,11 @@ namespace gpu {
 namespace {
 
 constexpr int kCmfdThreads = 512;
-constexpr int kCmfdWaves = kCmfdThreads / 64;
 constexpr int kCmfdLdsVectors = 6;    // cell vectors of the solve in LDS (see cmfd_solve)
-constexpr int kCmfdLdsVectorsRL = 7;  // with the rows in LDS: nc-long vectors
-constexpr size_t kCmfdLdsMax = 160 * 1024;  // LDS of a gfx950 workgroup
 
-// Cells per lane of a mesh of nc cells (1..6), and the LDS the solve uses:
-// each vector holds K * kCmfdThreads entries, the mesh cells and the dummies.
+// Cells per lane of a mesh of nc cells on kCmfdThreads lanes: 1 or 2 for the
+// meshes of up to 1024 cells this solve takes (larger ones run the wide
+// solve); each LDS vector holds K * lanes entries, the cells and the dummies.
 inline int cmfd_cells_per_lane(int nc) { return std::max(1, (nc + kCmfdThreads - 1) / kCmfdThreads); }
 // Lanes of the solve: meshes of one cell per lane use the fewest whole waves
 // that cover them (a barrier or a reduction then waits for no idle wave);
@@ -45,18 +43,9 @@ inline int cmfd_threads(int nc)
 {
     return nc <= kCmfdThreads ? std::max(64, (nc + 63) / 64 * 64) : kCmfdThreads;
 }
-// Meshes of 5 or 6 cells per lane keep the group's rows in LDS when they fit
-// (up to about 2900 cells): without that, each product reloads them from
-// cache, since the registers hold no more than 4 cells' rows. The two
-// reduction banks take 2 doubles per wave each.
-inline bool cmfd_rows_in_lds(int nc)
-{
-    return cmfd_cells_per_lane(nc) >= 5 &&
-           (size_t)kCmfdLdsVectorsRL * nc * sizeof(double) + 4 * kCmfdWaves * sizeof(double) <= kCmfdLdsMax;
-}
+// LDS of the solve of up to 1024 cells: kCmfdLdsVectors vectors of K * lanes.
 inline size_t cmfd_lds_bytes(int nc)
 {
-    if (cmfd_rows_in_lds(nc)) return (size_t)kCmfdLdsVectorsRL * nc * sizeof(double);
     return (size_t)kCmfdLdsVectors * cmfd_cells_per_lane(nc) * cmfd_threads(nc) * sizeof(double);
 }
 // Meshes of more than 1024 cells run the wide solve (k_cmfd_solve_wide): on
@@ -776,10 +765,12 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Row
 
 // CMFD::solve (cmfd.cpp:171-279) by one workgroup of NT lanes, K cells per
 // lane; red holds 2 * red_bank<NT>() doubles. lds holds kCmfdLdsVectors *
-// K * NT doubles, or with RL (rows in LDS) kCmfdLdsVectorsRL * nc: one
+// K * NT doubles, or with RL (rows in LDS) 7 * nc: one
 // product operand, the shadow residual and the group's five rows, while the
 // previous fission source, the right-hand side and the inverse diagonal go to
-// A.gscr.
+// A.gscr. The launches use K = 1 or 2 without RL (meshes of up to 1024
+// cells; larger ones run the wide solve); the K > 2 and RL forms are the
+// general form of this solve and are not launched.
 template <int K, int NT, bool RL = false>
 __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, double *red)
 {
@@ -1506,20 +1497,7 @@ void CmfdSolver::enqueue_step(const CmfdView &v, hipStream_t s)
             }
             break;
         case 2: hipLaunchKernelGGL((k_cmfd_solve<2, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-        case 3: hipLaunchKernelGGL((k_cmfd_solve<3, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-        case 4: hipLaunchKernelGGL((k_cmfd_solve<4, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a); break;
-        case 5:
-            if (cmfd_rows_in_lds(nc_))
-                hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-            else
-                hipLaunchKernelGGL((k_cmfd_solve<5, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-            break;
-        default:
-            if (cmfd_rows_in_lds(nc_))
-                hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads, true>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-            else
-                hipLaunchKernelGGL((k_cmfd_solve<6, kCmfdThreads>), dim3(1), dim3(kCmfdThreads), lds, s, a);
-            break;
+        default: throw std::runtime_error("the 512-lane coarse solve takes at most 1024 cells");
         }
     }
     hipLaunchKernelGGL(k_store_currents, dim3(nblk((size_t)n_surf_ * G_)), dim3(256), 0, s, ca, v.cur, v.sfl);
@@ -1579,6 +1557,7 @@ namespace {
 struct CmfdCase {
     int32_t bc;             // index of the problem in the batch status array
     int32_t npg, nsg, ncg;  // elements: pins x G, surfaces x G, cells x G
+    int32_t wide_k;         // cells per lane of its wide solve (mesh over 1024 cells), else 0
     CmfdView v;
     CoupleArgs ca;
     SolveArgs a;
@@ -1632,6 +1611,31 @@ __global__ __launch_bounds__(NT) void k_cmfd_b_solve(const CmfdCase *__restrict_
     cmfd_solve<K, NT, RL>(d.a, lds, red);
 }
 
+// A batch that holds meshes of more than 1024 cells: each of those runs the
+// wide solve at its own cells per lane (512 lanes, as a single problem of that
+// size does), the others the 512-lane solve at K cells per lane (the largest
+// of them, up to 1024 cells). Which solve a problem gets depends only on its
+// own mesh, and neither solve's arithmetic depends on K or on the lane count
+// beyond the problem's own, so a problem's result does not depend on the
+// others in the batch.
+template <int K>
+__global__ __launch_bounds__(kCmfdThreads) void k_cmfd_b_solve_mixed(const CmfdCase *__restrict__ cs,
+                                                                     const int32_t *__restrict__ status)
+{
+    static_assert(2 * kWideRedBank >= 2 * red_bank<kCmfdThreads>(), "one reduction array serves both solves");
+    extern __shared__ double lds[];
+    __shared__ double red[2 * kWideRedBank];
+    const CmfdCase &d = cs[blockIdx.x];
+    if (case_done(d, status)) return;
+    switch (__builtin_amdgcn_readfirstlane(d.wide_k)) {
+    case 0: cmfd_solve<K, kCmfdThreads>(d.a, lds, red); break;
+    case 3: cmfd_solve_wide<kCmfdThreads, 3>(d.a, lds, red); break;
+    case 4: cmfd_solve_wide<kCmfdThreads, 4>(d.a, lds, red); break;
+    case 5: cmfd_solve_wide<kCmfdThreads, 5>(d.a, lds, red); break;
+    default: cmfd_solve_wide<kCmfdThreads, 6>(d.a, lds, red); break;
+    }
+}
+
 std::vector<CaseBlock> case_blocks(const std::vector<int> &counts)
 {
     std::vector<CaseBlock> m;
@@ -1646,8 +1650,8 @@ struct CmfdBatch::Impl {
     CmfdCase *cases = nullptr;
     CaseBlock *bm[3] = {nullptr, nullptr, nullptr};  // pins, surfaces, cells
     int nbm[3] = {0, 0, 0};
-    int kmax = 1;          // cells per lane of the largest mesh
-    bool rl = false;       // its rows kept in LDS
+    int kmax = 1;          // cells per lane of the largest mesh (of the narrow ones when mixed)
+    bool mixed = false;    // meshes of more than 1024 cells present: k_cmfd_b_solve_mixed
     int nt = kCmfdThreads; // lanes of the solve workgroups
     size_t lds = 0;
     void release()
@@ -1672,7 +1676,8 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
     if (entries.empty()) return;
     std::vector<CmfdCase> host;
     std::vector<int> np, ns, nc;
-    int ncmax = 0;
+    int ncmax = 0, ncmax_narrow = 0;
+    size_t lds_wide = 0;
     for (const CmfdBatchEntry &e : entries) {
         const CmfdSolver &sv = *e.solver;
         if (!CmfdAccess::enabled(sv)) throw std::runtime_error("CMFD is not enabled for a batch problem");
@@ -1689,11 +1694,19 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
         d.npg = e.view.npin * e.view.G;
         d.nsg = CmfdAccess::n_surf(sv) * e.view.G;
         d.ncg = CmfdAccess::n_cell(sv) * e.view.G;
+        const int ncell = CmfdAccess::n_cell(sv);
+        d.wide_k = 0;
+        if (cmfd_wide(ncell)) {
+            d.wide_k = cmfd_wide_k(ncell);
+            lds_wide = std::max(lds_wide, cmfd_wide_lds_bytes(ncell, d.a.nx));
+        } else {
+            ncmax_narrow = std::max(ncmax_narrow, ncell);
+        }
         host.push_back(d);
         np.push_back(d.npg);
         ns.push_back(d.nsg);
         nc.push_back(d.ncg);
-        ncmax = std::max(ncmax, CmfdAccess::n_cell(sv));
+        ncmax = std::max(ncmax, ncell);
     }
     n_ = (int)host.size();
     dalloc(impl_->cases, host.size());
@@ -1710,10 +1723,18 @@ void CmfdBatch::build(const std::vector<CmfdBatchEntry> &entries, hipStream_t s)
     if (ncmax > CmfdSolver::kMaxBatchCells)
         throw std::runtime_error("a batched CMFD problem holds at most " + std::to_string(CmfdSolver::kMaxBatchCells) +
                                  " coarse cells; this batch has one of " + std::to_string(ncmax));
-    impl_->kmax = cmfd_cells_per_lane(ncmax);
-    impl_->nt = cmfd_threads(ncmax);
-    impl_->rl = cmfd_rows_in_lds(ncmax);
-    impl_->lds = cmfd_lds_bytes(ncmax);
+    impl_->mixed = lds_wide > 0;
+    if (impl_->mixed) {
+        // the narrow meshes (up to 1024 cells: 1 or 2 cells per lane) on
+        // kCmfdThreads lanes, whose LDS vectors are K * kCmfdThreads long
+        impl_->kmax = cmfd_cells_per_lane(std::max(1, ncmax_narrow));
+        impl_->nt = kCmfdThreads;
+        impl_->lds = std::max(lds_wide, (size_t)kCmfdLdsVectors * impl_->kmax * kCmfdThreads * sizeof(double));
+    } else {
+        impl_->kmax = cmfd_cells_per_lane(ncmax);
+        impl_->nt = cmfd_threads(ncmax);
+        impl_->lds = cmfd_lds_bytes(ncmax);
+    }
     CMFD_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1727,34 +1748,28 @@ void CmfdBatch::enqueue(const int32_t *status, hipStream_t s) const
     hipLaunchKernelGGL(k_cmfd_b_elem<2>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<3>, dim3(m.nbm[2]), dim3(256), 0, s, m.cases, m.bm[2], status);
     const dim3 grid(n_), block(kCmfdThreads);
-    switch (m.kmax) {
-    case 1:
-        switch (m.nt) {
-        case 64: hipLaunchKernelGGL((k_cmfd_b_solve<1, 64>), grid, dim3(64), m.lds, s, m.cases, status); break;
-        case 128: hipLaunchKernelGGL((k_cmfd_b_solve<1, 128>), grid, dim3(128), m.lds, s, m.cases, status); break;
-        case 192: hipLaunchKernelGGL((k_cmfd_b_solve<1, 192>), grid, dim3(192), m.lds, s, m.cases, status); break;
-        case 256: hipLaunchKernelGGL((k_cmfd_b_solve<1, 256>), grid, dim3(256), m.lds, s, m.cases, status); break;
-        case 320: hipLaunchKernelGGL((k_cmfd_b_solve<1, 320>), grid, dim3(320), m.lds, s, m.cases, status); break;
-        case 384: hipLaunchKernelGGL((k_cmfd_b_solve<1, 384>), grid, dim3(384), m.lds, s, m.cases, status); break;
-        case 448: hipLaunchKernelGGL((k_cmfd_b_solve<1, 448>), grid, dim3(448), m.lds, s, m.cases, status); break;
-        default: hipLaunchKernelGGL((k_cmfd_b_solve<1, 512>), grid, dim3(512), m.lds, s, m.cases, status); break;
+    if (m.mixed) {
+        if (m.kmax == 1)
+            hipLaunchKernelGGL(k_cmfd_b_solve_mixed<1>, grid, block, m.lds, s, m.cases, status);
+        else
+            hipLaunchKernelGGL(k_cmfd_b_solve_mixed<2>, grid, block, m.lds, s, m.cases, status);
+    } else {
+        switch (m.kmax) {
+        case 1:
+            switch (m.nt) {
+            case 64: hipLaunchKernelGGL((k_cmfd_b_solve<1, 64>), grid, dim3(64), m.lds, s, m.cases, status); break;
+            case 128: hipLaunchKernelGGL((k_cmfd_b_solve<1, 128>), grid, dim3(128), m.lds, s, m.cases, status); break;
+            case 192: hipLaunchKernelGGL((k_cmfd_b_solve<1, 192>), grid, dim3(192), m.lds, s, m.cases, status); break;
+            case 256: hipLaunchKernelGGL((k_cmfd_b_solve<1, 256>), grid, dim3(256), m.lds, s, m.cases, status); break;
+            case 320: hipLaunchKernelGGL((k_cmfd_b_solve<1, 320>), grid, dim3(320), m.lds, s, m.cases, status); break;
+            case 384: hipLaunchKernelGGL((k_cmfd_b_solve<1, 384>), grid, dim3(384), m.lds, s, m.cases, status); break;
+            case 448: hipLaunchKernelGGL((k_cmfd_b_solve<1, 448>), grid, dim3(448), m.lds, s, m.cases, status); break;
+            default: hipLaunchKernelGGL((k_cmfd_b_solve<1, 512>), grid, dim3(512), m.lds, s, m.cases, status); break;
+            }
+            break;
+        case 2: hipLaunchKernelGGL((k_cmfd_b_solve<2, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
+        default: throw std::runtime_error("the 512-lane coarse solve takes at most 1024 cells");
         }
-        break;
-    case 2: hipLaunchKernelGGL((k_cmfd_b_solve<2, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
-    case 3: hipLaunchKernelGGL((k_cmfd_b_solve<3, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
-    case 4: hipLaunchKernelGGL((k_cmfd_b_solve<4, kCmfdThreads>), grid, block, m.lds, s, m.cases, status); break;
-    case 5:
-        if (m.rl)
-            hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads, true>), grid, block, m.lds, s, m.cases, status);
-        else
-            hipLaunchKernelGGL((k_cmfd_b_solve<5, kCmfdThreads>), grid, block, m.lds, s, m.cases, status);
-        break;
-    default:
-        if (m.rl)
-            hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads, true>), grid, block, m.lds, s, m.cases, status);
-        else
-            hipLaunchKernelGGL((k_cmfd_b_solve<6, kCmfdThreads>), grid, block, m.lds, s, m.cases, status);
-        break;
     }
     hipLaunchKernelGGL(k_cmfd_b_elem<4>, dim3(m.nbm[1]), dim3(256), 0, s, m.cases, m.bm[1], status);
     hipLaunchKernelGGL(k_cmfd_b_elem<5>, dim3(m.nbm[0]), dim3(256), 0, s, m.cases, m.bm[0], status);

@@ -38,13 +38,13 @@ def cmfd_decks(deck_dir):
     itself, one 17x17 assembly (289 cells) and the 3x3-assembly core geometry
     (51x51 cells, vacuum east/south) with coarser rays so the FP64 oracle
     finishes in seconds, and square lattices with vacuum east/south faces.
-    A single problem's coarse solve runs on up to 512 lanes, 1 or 2 cells per
-    lane, up to 1024 cells (pins3x3, asm17, lat24: 576 cells), and on 1024
-    lanes beyond: 2 cells per lane (lat34, lat40: 1156 and 1600 cells) and 3
-    (lat48, core51, lat55: 2304, 2601 and 3025 cells) with the rows in LDS,
-    4, 5 and 6 (lat64, lat70, lat78: 4096, 4900 and 6084 cells) re-reading
-    the rows from cache. Batches run every mesh on 512 lanes, up to 6 cells
-    per lane (3072 cells)."""
+    A coarse solve of up to 1024 cells runs on up to 512 lanes, 1 or 2 cells
+    per lane (pins3x3, asm17, lat24: 9, 289, 576 cells). Larger meshes run
+    the wide solve: on 512 lanes with the rows in registers up to 3072 cells
+    (lat34, lat40, lat48, core51, lat55: 1156, 1600, 2304, 2601, 3025 cells at
+    3, 4, 5, 6, 6 cells per lane), on 1024 lanes re-reading the rows from
+    cache beyond (lat64, lat70, lat78: 4096, 4900, 6084 cells at 4, 5, 6). A
+    batch runs the same two solves, up to 3072 cells."""
     from mocc_amd import lattice_gen as lg
     from mocc_amd.native import HostProblem
     paths = {

@@ -280,21 +280,28 @@ def test_member_handle_is_owned_by_the_batch(batch_decks):
     b.close()
 
 
-@pytest.mark.parametrize("big", ["lat24", "lat40"])
-def test_batch_cmfd_register_rows_mixed_bitwise(cmfd_decks, big):
-    """Beside a mesh solved at 2 (24x24) or 4 (40x40) cells per lane with the
-    rows in registers, a 3x3-pin problem's k, iterations and flux are bitwise
-    those of its solve alone (one wave, one cell per lane)."""
+@pytest.mark.parametrize("big", ["lat24", "lat34", "lat40", "lat55"])
+def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks, big):
+    """A 3x3-pin problem beside a larger mesh: 24x24 (576 cells, the 512-lane
+    solve at 2 cells per lane for both) or 34x34, 40x40, 55x55 (1156, 1600,
+    3025 cells: the wide solve at 3, 4 and 6 cells per lane for the large
+    mesh, the 512-lane solve for the 3x3 one, in one launch). Each problem's
+    k, iterations and flux are bitwise those of its batch alone, and the large
+    mesh's k agrees with its single-problem solve."""
     from mocc_amd.sweeper import GpuMoCSweeper
     mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
-    alone = {"pins3x3": mk("pins3x3")}
-    r_alone = _solve_batch(alone, ["pins3x3"])["pins3x3"]
+    alone = {"pins3x3": mk("pins3x3"), big: mk(big)}
+    r_alone = {n: _solve_batch(alone, [n])[n] for n in alone}
     print(f"cmfd 3x3 pins beside {big}", flush=True)
     mixed = {big: mk(big), "pins3x3": mk("pins3x3")}
     r = _solve_batch(mixed, [big, "pins3x3"])
-    assert r["pins3x3"].converged and r[big].converged, (r["pins3x3"].iterations, r[big].iterations)
-    assert r["pins3x3"].k == r_alone.k and r["pins3x3"].iterations == r_alone.iterations
-    assert np.array_equal(mixed["pins3x3"].flux(), alone["pins3x3"].flux())
+    for n in mixed:
+        assert r[n].converged, (n, r[n].iterations)
+        assert r[n].k == r_alone[n].k and r[n].iterations == r_alone[n].iterations, n
+        assert np.array_equal(mixed[n].flux(), alone[n].flux()), n
+    single = mk(big)
+    rc, st = single.eigen_solve(K_TOL, PSI_TOL, MAX_IT)
+    assert rc == 0 and abs(st.k - r[big].k) < 1e-9 and abs(st.iterations - r[big].iterations) <= 1
 
 
 def test_branch_batch_vs_oracle(tmp_path):
