@@ -118,13 +118,13 @@ def oracle_k(factory, case):
     return st.k
 
 
-def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use_cmfd: bool = True):
+def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use_cmfd: bool = True, lanes: int = 2):
     """Batched branch cases (one graph-replayed batch) against the same
     problems solved one at a time with the single-problem device path; keff
     of a sample checked against the FP64 oracle. With CMFD: 5 inner
     iterations per outer, as mocc's CMFD decks use; without: 1."""
     from mocc_amd.branch import BranchFactory, c4_cases
-    from mocc_amd.sweeper import GpuBatch
+    from mocc_amd.sweeper import GpuBatchSet
     n_inner = 5 if use_cmfd else 1
     factory = BranchFactory(os.path.join(work_dir, "batch_cmfd" if use_cmfd else "batch"), n_inner=n_inner,
                             use_cmfd=use_cmfd)
@@ -138,7 +138,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
     t0 = time.perf_counter()
     handles = [factory.make(c) for c in cases]
     t_setup = time.perf_counter() - t0
-    batch = GpuBatch(use_graph=True)
+    batch = GpuBatchSet(lanes, use_graph=True)
     ids = [batch.add(h, K_TOL, PSI_TOL, MAX_IT) for h in handles]
     t0 = time.perf_counter()
     res = batch.solve_all(chunk=8)
@@ -178,6 +178,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
         "workload": f"{n_cases} branch cases: 1/4 single pin, 1/2 3x3, 1/4 17x17 (25 guide tubes); "
                     "2/7 groups alternating; CY 16x3; spacing from {0.02, 0.035, 0.05} cm; XS x U[0.9,1.1]; "
                     "band width and lane map autotuned for the batch",
+        "batch_lanes": lanes,
         "batched_solves_per_s": bat, "serial_solves_per_s": ser, "speedup": bat / ser,
         "serial_sample": len(sub), "converged": int(n_conv), "outer_iterations_max": int(max(iters)),
         "batch_s": t_batch, "setup_s": t_setup,
@@ -191,7 +192,7 @@ def bench_batched(work_dir: str, n_cases: int, n_serial: int, n_oracle: int, use
 
 
 def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, use_cmfd: bool = True,
-                  n_serial: int = 64, n_oracle: int = 8):
+                  n_serial: int = 64, n_oracle: int = 8, lanes: int = 1):
     """Poisson arrivals of branch-case requests served by the continuous-
     batching loop at the given fractions of its burst (saturation)
     throughput, beside the same requests solved one at a time (CMFD with 5
@@ -206,14 +207,14 @@ def bench_serving(work_dir: str, n_req: int, rates=(0.5, 1.0), seed: int = 5, us
     for c in cases:  # trace every geometry before timing
         factory.host_problem(c)
     factory.tune(cases[:32])
-    srv = Server(factory, chunk=1)
+    srv = Server(factory, chunk=1, lanes=lanes)
     burst = summarize(srv.serve(np.zeros(n_req), cases))
     out = {"workload": f"{n_req} requests drawn from {{pin, 3x3, 17x17, 2x2 of 17x17}} x {{2,7}} groups, "
                        "CY 16x3, spacing 0.05 cm, XS x U[0.9,1.1]; "
                        + ("CMFD, 5 inner iterations" if use_cmfd else "no CMFD, 1 inner iteration")
                        + "; band width and lane map autotuned for the batch on the first 32 requests",
            "tuned": {"band_rays": factory.band_rays, "lane_map": factory.lane_map},
-           "burst": burst}
+           "batch_lanes": lanes, "burst": burst}
     progress(f"serving burst: {burst['solves_per_s']:.3g} solves/s")
     last = None
     for frac in rates:
@@ -317,6 +318,10 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--serial-cases", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--batch-lanes", type=int, default=2,
+                    help="batches (each on its own stream) the batched branch cases are spread over")
+    ap.add_argument("--serve-lanes", type=int, default=1,
+                    help="batches (each on its own stream) the serving loop spreads its requests over")
     ap.add_argument("--serve-rates", default="0.5,1.0",
                     help="comma-separated arrival rates as fractions of the burst throughput")
     ap.add_argument("--no-core51", action="store_true",
@@ -395,14 +400,16 @@ def main():
         progress("CPU baseline done")
     dev.close()
     if not args.no_batch:
-        line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 8)
+        line["batched"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases, 0 if args.no_cpu else 8,
+                                        lanes=args.batch_lanes)
         progress(f"batched (CMFD): {line['batched']['batched_solves_per_s']:.4g} solves/s")
         line["batched_no_cmfd"] = bench_batched(deck_dir, args.batch_cases, args.serial_cases,
-                                                0 if args.no_cpu else 2, use_cmfd=False)
+                                                0 if args.no_cpu else 2, use_cmfd=False, lanes=args.batch_lanes)
         progress(f"batched (no CMFD): {line['batched_no_cmfd']['batched_solves_per_s']:.4g} solves/s")
     if not args.no_serve:
         rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
-        line["serving"] = bench_serving(deck_dir, args.serve_requests, rates, n_oracle=0 if args.no_cpu else 8)
+        line["serving"] = bench_serving(deck_dir, args.serve_requests, rates, n_oracle=0 if args.no_cpu else 8,
+                                        lanes=args.serve_lanes)
         progress("serving done")
     if not args.no_core51:
         line["core51"] = bench_core51(deck_dir, args.core51_oracle)

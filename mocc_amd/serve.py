@@ -27,7 +27,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .branch import BranchFactory, Case
-from .sweeper import CapacityError, GpuBatch
+from .sweeper import CapacityError, GpuBatchSet
 
 
 @dataclass
@@ -55,14 +55,20 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
 
 class Server:
     def __init__(self, factory: BranchFactory, budget_bytes: int = 0, k_tol: float = 1e-7,
-                 psi_tol: float = 1e-6, max_iter: int = 3000, chunk: int = 1, keep_flux: bool = False):
+                 psi_tol: float = 1e-6, max_iter: int = 3000, chunk: int = 1, keep_flux: bool = False,
+                 lanes: int = 1):
         """chunk: outer iterations per graph replay between admissions. One
         admits arrivals and releases finished problems soonest. On requests
         from pin, 3x3, 17x17 and 2x2-of-17x17 lattices arriving at half the
         burst rate, it gave p50 464 ms against 688 ms with 4, at a slightly
-        higher throughput."""
+        higher throughput. lanes: batches (each on its own stream) the
+        requests are spread over. Two raised the 2000-request burst from 109
+        to 115 solves/s but doubled the latency at 0.5x load (p50 483 ->
+        905 ms): a coarse solve shares its CU with the other batch's sweeps
+        and the slowest one sets every request's pace, so one is the
+        default."""
         self.factory = factory
-        self.batch = GpuBatch(budget_bytes=budget_bytes, use_graph=True)
+        self.batch = GpuBatchSet(lanes, budget_bytes=budget_bytes, use_graph=True)
         self.k_tol, self.psi_tol, self.max_iter = k_tol, psi_tol, max_iter
         self.chunk = chunk
         self.keep_flux = keep_flux

@@ -331,3 +331,93 @@ class GpuBatch:
             self.close()
         except Exception:
             pass
+
+
+class GpuBatchSet:
+    """Problems spread over several batches, each with its own stream, run
+    together: one batch's coarse solves (a workgroup per problem) and the
+    tails of its launches overlap with the other batches' sweeps. A new
+    problem joins the batch with the fewest members. A problem's result does
+    not depend on which batch holds it or what else is there (bitwise), so
+    the set gives the same answers as one batch. Same methods as GpuBatch;
+    ids are the set's own."""
+
+    def __init__(self, n: int = 2, budget_bytes: int = 0, use_graph: bool = True):
+        if n < 1:
+            raise ValueError("a batch set needs at least one batch")
+        self.batches = [GpuBatch(use_graph=use_graph) for _ in range(n)]
+        self.budget = int(budget_bytes)
+        self._where = {}  # set id -> (batch index, batch id)
+        self._next = 0
+        self._live = [0] * n  # unfinished members of each batch
+        self._done = set()    # set ids reported finished and still members
+
+    def add(self, s: GpuMoCSweeper, k_tol: float, psi_tol: float, max_iter: int, min_iter: int = 0) -> int:
+        if self.budget > 0 and self.bytes_used() + s.state_bytes() > self.budget:
+            raise CapacityError("device memory budget exhausted")
+        j = min(range(len(self.batches)), key=lambda i: self.batches[i].size())
+        cid = self.batches[j].add(s, k_tol, psi_tol, max_iter, min_iter)
+        sid = self._next
+        self._next += 1
+        self._where[sid] = (j, cid)
+        self._live[j] += 1
+        return sid
+
+    def run(self, n_outer: int = 1):
+        """Enqueue n_outer outer iterations on every batch with unfinished
+        members (returns at once; poll waits)."""
+        for j, b in enumerate(self.batches):
+            if self._live[j] > 0:
+                b.run(n_outer)
+
+    def poll(self, cap: int = 4096):
+        back = {w: sid for sid, w in self._where.items()}
+        out = []
+        for j, b in enumerate(self.batches):
+            if b.size() > 0:
+                for cid, st in b.poll(cap):
+                    sid = back[(j, cid)]
+                    if sid not in self._done:
+                        self._done.add(sid)
+                        self._live[j] -= 1
+                    out.append((sid, st))
+        return out
+
+    def remove(self, sid: int):
+        j, cid = self._where.pop(sid)
+        if sid in self._done:
+            self._done.discard(sid)
+        else:
+            self._live[j] -= 1
+        self.batches[j].remove(cid)
+
+    def size(self) -> int:
+        return sum(b.size() for b in self.batches)
+
+    def bytes_used(self) -> int:
+        return sum(b.bytes_used() for b in self.batches)
+
+    def solve_all(self, chunk: int = 4, max_outer: int = 100000, compact: float = 0.25):
+        """GpuBatch.solve_all over the set; returns {id: EigenState}."""
+        done = {}
+        pending = []
+        outer = 0
+        total = self.size()
+        while len(done) < total and outer < max_outer:
+            self.run(chunk)
+            outer += chunk
+            for sid, st in self.poll():
+                done[sid] = st
+                pending.append(sid)
+            if pending and len(done) < total and len(pending) >= compact * self.size():
+                for sid in pending:
+                    self.remove(sid)
+                pending = []
+        return done
+
+    def close(self):
+        for b in self.batches:
+            b.close()
+        self._where.clear()
+        self._done.clear()
+        self._live = [0] * len(self.batches)

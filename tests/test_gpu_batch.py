@@ -103,6 +103,33 @@ def test_graph_replays_match_eager(batch_decks):
             assert sg[n].total_fission() == se[n].total_fission()
 
 
+def test_batch_set_matches_one_batch(batch_decks):
+    """Problems spread over two batches on two streams (GpuBatchSet) give
+    bitwise the k, iteration counts and fluxes of one batch."""
+    from mocc_amd.sweeper import CapacityError, GpuBatchSet
+    names = ["pin1", "pins3x3", "pins3x3_pert", "asm17"]
+    one = _sweepers(batch_decks, names)
+    r_one = _solve_batch(one, names)
+    two = _sweepers(batch_decks, names)
+    bs = GpuBatchSet(2)
+    ids = {bs.add(two[n], K_TOL, PSI_TOL, MAX_IT): n for n in names}
+    assert [b.size() for b in bs.batches] == [2, 2]
+    res = bs.solve_all(chunk=4)
+    assert set(res) == set(ids)
+    for sid, n in ids.items():
+        print(f"batch set case {n}", flush=True)
+        assert res[sid].k == r_one[n].k and res[sid].iterations == r_one[n].iterations, n
+        assert np.array_equal(two[n].flux(), one[n].flux()), n
+    bs.close()
+    need = two["pins3x3"].state_bytes()
+    small = GpuBatchSet(2, budget_bytes=need + need // 2)
+    small.add(two["pins3x3"], K_TOL, PSI_TOL, MAX_IT)
+    with pytest.raises(CapacityError):
+        small.add(two["pins3x3_pert"], K_TOL, PSI_TOL, MAX_IT)
+    assert small.size() == 1
+    small.close()
+
+
 def test_admission_budget(batch_decks):
     from mocc_amd.sweeper import CapacityError, GpuBatch
     sw = _sweepers(batch_decks, ["pins3x3", "pins3x3_pert"])
