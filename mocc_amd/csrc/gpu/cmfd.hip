@@ -155,7 +155,7 @@ __device__ __forceinline__ void pin_flux_item(const CmfdView &v, double *cflux, 
         pf += v.phi[(size_t)r * v.G + g] * v.vol[r];
     }
     const double c = pf / vs;
-    cflux[(size_t)v.pin_cell[ip] * v.G + g] = v.neg_fixup ? fmax(c, 0.0) : c;
+    cflux[(size_t)g * v.nc + v.pin_cell[ip]] = v.neg_fixup ? fmax(c, 0.0) : c;
 }
 
 __global__ void k_pin_flux_all(CmfdView v, double *cflux)
@@ -165,7 +165,11 @@ __global__ void k_pin_flux_all(CmfdView v, double *cflux)
 
 // XSMeshHomogenized::homogenize_region_flux: flux-area weighted Sigma_t,
 // nu Sigma_f, Sigma_f; chi weighted by the fission source; Sigma_s[g <- g']
-// weighted by phi_g'. One thread per (pin, group).
+// weighted by phi_g'. One thread per (pin, group). What the coarse solve
+// reads per cell (nu Sigma_f, chi, Sigma_s, like the coarse flux) is stored
+// cell-fastest, [g][c] and [g][g'][c], so the lanes of a wave, which hold
+// consecutive cells, read consecutive addresses; Sigma_t, Sigma_f and the
+// removal stay [c][g].
 struct HomogArrays {
     double *htr, *hnf, *hf, *hch, *hrm, *hsc;
 };
@@ -194,9 +198,9 @@ __device__ __forceinline__ void homogenize_item(const CmfdView &v, const HomogAr
     }
     const double htr_g = tr / fvs;
     htr[(size_t)cell * G + g] = htr_g;
-    hnf[(size_t)cell * G + g] = nf / fvs;
+    hnf[(size_t)g * v.nc + cell] = nf / fvs;
     hf[(size_t)cell * G + g] = fi / fvs;
-    hch[(size_t)cell * G + g] = (fs_sum > 0.0) ? ch / fs_sum : ch;
+    hch[(size_t)g * v.nc + cell] = (fs_sum > 0.0) ? ch / fs_sum : ch;
     double self = 0.0;
     for (int gg = 0; gg < G; gg++) {
         double sc = 0.0, ss = 0.0;
@@ -208,7 +212,7 @@ __device__ __forceinline__ void homogenize_item(const CmfdView &v, const HomogAr
             sc += v.mat_sc[((size_t)m * G + g) * G + gg] * a * fgg;
         }
         if (sc > 0.0) sc /= ss;
-        hsc[((size_t)cell * G + g) * G + gg] = sc;
+        hsc[((size_t)g * G + gg) * v.nc + cell] = sc;
         if (gg == g) self = sc;
     }
     hrm[(size_t)cell * G + g] = htr_g - self;
@@ -249,8 +253,9 @@ __device__ __forceinline__ void couple_item(const CoupleArgs &a, int t)
     if (norm != 2 && has_radial) {
         const double j = a.cur[(size_t)is * G + g];
         const double sf = a.sfl[(size_t)is * G + g];
-        const double fl = c1 >= 0 ? a.cflux[(size_t)c1 * G + g] : 0.0;
-        const double fr = c2 >= 0 ? a.cflux[(size_t)c2 * G + g] : 0.0;
+        const size_t cg = (size_t)g * a.m.nx * a.m.ny;
+        const double fl = c1 >= 0 ? a.cflux[cg + c1] : 0.0;
+        const double fr = c2 >= 0 ? a.cflux[cg + c2] : 0.0;
         dh = (j + dt * (fr - fl)) / (fl + fr);
         if (!isfinite(dh)) dh = 0.0;
         sh = (c1 >= 0) ? (sf - st * fl - (1.0 - st) * fr) / (fl + fr) : (sf - st * fr) / fr;
@@ -308,8 +313,9 @@ __device__ __forceinline__ void store_currents_item(const CoupleArgs &a, double 
     const int is = t / a.G, g = t % a.G, G = a.G;
     int c1, c2;
     neigh_cells(a.m, is, c1, c2);
-    const double fr = c2 >= 0 ? a.cflux[(size_t)c2 * G + g] : 0.0;
-    const double fl = c1 >= 0 ? a.cflux[(size_t)c1 * G + g] : 0.0;
+    const size_t cg = (size_t)g * a.m.nx * a.m.ny;
+    const double fr = c2 >= 0 ? a.cflux[cg + c2] : 0.0;
+    const double fl = c1 >= 0 ? a.cflux[cg + c1] : 0.0;
     const double dh = a.dhat[(size_t)is * G + g], dt = a.dtil[(size_t)is * G + g];
     const double j = -dt * (fr - fl) + dh * (fr + fl);
     const double sh = a.shat[(size_t)is * G + g], st = a.stil[(size_t)is * G + g];
@@ -338,7 +344,7 @@ __device__ __forceinline__ void set_pin_flux_item(const CmfdView &v, const doubl
 {
     if (t >= v.npin * v.G) return;
     const int ip = t / v.G, g = t % v.G;
-    const double pn = cflux[(size_t)v.pin_cell[ip] * v.G + g];
+    const double pn = cflux[(size_t)g * v.nc + v.pin_cell[ip]];
     if (!(pn > 0.0)) {
         atomicOr(bad, 1);
         return;
@@ -369,7 +375,7 @@ struct SolveArgs {
     double k_tol, psi_tol, rr;
     const double *coef;  // [5][G][nc]
     const double *hnf, *hch, *hsc, *cvol;
-    double *cflux;  // [nc][G]
+    double *cflux;  // [G][nc]
     double *gscr;   // [3][nc] global scratch of the solves that keep the rows in LDS
     double *k;
     int *info;
@@ -708,6 +714,17 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
     return i;
 }
 
+// The per-cell sums below read all K cells of one step together (a dummy
+// slot reads cell 0 and its value is dropped), so a lane has K loads in
+// flight instead of one; each cell's sum keeps its order of terms. The
+// arrays are cell-fastest ([g][c], [g][g'][c]): a wave's lanes, which hold
+// consecutive cells, read consecutive addresses.
+template <int K> __device__ __forceinline__ void lane_cells(const Lane<K> &L, int (&ci)[K])
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) ci[k] = L.on[k] ? L.c[k] : 0;
+}
+
 // CMFD source of group g: V (chi fs + sum_{g' != g} Sigma_s[g <- g'] phi_g')
 // (cmfd.cpp:225-231), from this lane's own cells.
 template <int K>
@@ -715,35 +732,77 @@ __device__ __forceinline__ void cmfd_source(const SolveArgs &A, int g, const Lan
                                             double (&b)[K])
 {
     const int G = A.G;
+    const size_t nc = (size_t)A.nc;
+    int ci[K];
+    lane_cells<K>(L, ci);
+    double s[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        b[k] = 0.0;
-        if (!L.on[k]) continue;
-        const int c = L.c[k];
-        double s = A.hch[(size_t)c * G + g] * fsc[k];
-        const double *row = A.hsc + ((size_t)c * G + g) * G;
-        for (int gg = 0; gg < G; gg++) {
-            if (gg == g) continue;
-            s += row[gg] * A.cflux[(size_t)c * G + gg];
-        }
-        b[k] = s * A.cvol[c];
+    for (int k = 0; k < K; k++) s[k] = A.hch[g * nc + ci[k]] * fsc[k];
+    for (int gg = 0; gg < G; gg++) {
+        if (gg == g) continue;
+        const double *sc = A.hsc + ((size_t)g * G + gg) * nc, *fl = A.cflux + gg * nc;
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += sc[ci[k]] * fl[ci[k]];
     }
+#pragma unroll
+    for (int k = 0; k < K; k++) b[k] = L.on[k] ? s[k] * A.cvol[ci[k]] : 0.0;
 }
 
 template <int K>
 __device__ __forceinline__ double fission_source(const SolveArgs &A, const Lane<K> &L, double k_eff, double (&fsc)[K])
 {
     const double rk = 1.0 / k_eff;
+    const size_t nc = (size_t)A.nc;
+    int ci[K];
+    lane_cells<K>(L, ci);
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int g = 0; g < A.G; g++) {
+        const double *nf = A.hnf + g * nc, *fl = A.cflux + g * nc;
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += nf[ci[k]] * fl[ci[k]];
+    }
     double tot = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double s = 0.0;
-        if (L.on[k])
-            for (int g = 0; g < A.G; g++) s += A.hnf[(size_t)L.c[k] * A.G + g] * A.cflux[(size_t)L.c[k] * A.G + g];
-        tot += s;
-        fsc[k] = s * rk;
+        const double sk = L.on[k] ? s[k] : 0.0;
+        tot += sk;
+        fsc[k] = sk * rk;
     }
     return tot;
+}
+
+// This lane's share of the coarse fission total sum_c sum_g nu-Sigma_f phi,
+// summed term by term in cell order, then group order.
+template <int K> __device__ __forceinline__ double lane_fission_total(const SolveArgs &A, const Lane<K> &L)
+{
+    const size_t nc = (size_t)A.nc;
+    int ci[K];
+    lane_cells<K>(L, ci);
+    double lf = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+#pragma unroll 8
+        for (int g = 0; g < A.G; g++) {
+            const double v = A.hnf[g * nc + ci[k]] * A.cflux[g * nc + ci[k]];
+            lf += L.on[k] ? v : 0.0;
+        }
+    }
+    return lf;
+}
+
+// This lane's share of the squared change of the coarse fission source.
+template <int K>
+__device__ __forceinline__ double lane_source_change(const Lane<K> &L, const double (&fsc)[K], const double *fs_old)
+{
+    double le = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const double d = fsc[k] - fs_old[L.on[k] ? L.c[k] : 0];
+        le += L.on[k] ? d * d : 0.0;
+    }
+    return le;
 }
 
 // ||A x - b||^2 of group g with x the stored coarse flux (cmfd.cpp:640-666)
@@ -752,7 +811,7 @@ __device__ __forceinline__ double residual2(const SolveArgs &A, int g, const Row
                                             const double (&b)[K], double (&x)[K], double (&t)[K], Scratch<NT> &S)
 {
 #pragma unroll
-    for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * A.G + g] : 0.0;
+    for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)g * A.nc + L.c[k]] : 0.0;
     spmv<K, NT, RL>(A, R, L, x, t, S);
     double s = 0.0;
 #pragma unroll
@@ -831,22 +890,13 @@ __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, doub
             n_bicg += bicgstab<K, NT, RL>(A, R, L, x, tol, 150, S, sinv, sr0, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
-                if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
+                if (L.on[k]) A.cflux[(size_t)g * A.nc + L.c[k]] = x[k];
         }
         ri = sqrt(ri) / (A.nc * G);
-        double lf = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (L.on[k])
-                for (int g = 0; g < G; g++) lf += A.hnf[(size_t)L.c[k] * G + g] * A.cflux[(size_t)L.c[k] * G + g];
-        tfis = block_sum(lf, S);
+        tfis = block_sum(lane_fission_total<K>(A, L), S);
         const double k_old = k_eff;
         k_eff = k_eff * tfis / tfis_old;
-        double le = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (L.on[k]) le += (fsc[k] - fs_old[L.c[k]]) * (fsc[k] - fs_old[L.c[k]]);
-        const double psi_err = sqrt(block_sum(le, S));
+        const double psi_err = sqrt(block_sum(lane_source_change<K>(L, fsc, fs_old), S));
         if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
             break;
     }
@@ -854,7 +904,7 @@ __device__ __forceinline__ void cmfd_solve(const SolveArgs &A, double *lds, doub
     for (int k = 0; k < K; k++)
         if (L.on[k])
             for (int g = 0; g < G; g++) {
-                double &f = A.cflux[(size_t)L.c[k] * G + g];
+                double &f = A.cflux[(size_t)g * A.nc + L.c[k]];
                 if (f < 0.0) f = -f;
             }
     if (threadIdx.x == 0) {
@@ -1211,7 +1261,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
         cmfd_source<K>(A, g, L, fsc, b);
         wide_load_rows<NT, K>(A, g, L, R, inv);
 #pragma unroll
-        for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * G + g] : 0.0;
+        for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)g * A.nc + L.c[k]] : 0.0;
         wide_spmv<NT, K>(A, R, L, x, t, S);
         double s = 0.0;
 #pragma unroll
@@ -1235,7 +1285,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
             cmfd_source<K>(A, g, L, fsc, b);
             wide_load_rows<NT, K>(A, g, L, R, inv);
 #pragma unroll
-            for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)L.c[k] * G + g] : 0.0;
+            for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)g * A.nc + L.c[k]] : 0.0;
             wide_spmv<NT, K>(A, R, L, x, t, S);
             double s = 0.0;
 #pragma unroll
@@ -1248,22 +1298,13 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
             n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, tol, 150, S, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
-                if (L.on[k]) A.cflux[(size_t)L.c[k] * G + g] = x[k];
+                if (L.on[k]) A.cflux[(size_t)g * A.nc + L.c[k]] = x[k];
         }
         ri = sqrt(ri) / (A.nc * G);
-        double lf = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (L.on[k])
-                for (int g = 0; g < G; g++) lf += A.hnf[(size_t)L.c[k] * G + g] * A.cflux[(size_t)L.c[k] * G + g];
-        tfis = wide_block_sum(lf, S);
+        tfis = wide_block_sum(lane_fission_total<K>(A, L), S);
         const double k_old = k_eff;
         k_eff = k_eff * tfis / tfis_old;
-        double le = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (L.on[k]) le += (fsc[k] - fs_old[L.c[k]]) * (fsc[k] - fs_old[L.c[k]]);
-        const double psi_err = sqrt(wide_block_sum(le, S));
+        const double psi_err = sqrt(wide_block_sum(lane_source_change<K>(L, fsc, fs_old), S));
         if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
             break;
     }
@@ -1271,7 +1312,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
     for (int k = 0; k < K; k++)
         if (L.on[k])
             for (int g = 0; g < G; g++) {
-                double &f = A.cflux[(size_t)L.c[k] * G + g];
+                double &f = A.cflux[(size_t)g * A.nc + L.c[k]];
                 if (f < 0.0) f = -f;
             }
     if (threadIdx.x == 0) {
@@ -1545,9 +1586,13 @@ void CmfdSolver::matrix_csr(int g, hipStream_t s, std::vector<int32_t> &row_ptr,
 
 void CmfdSolver::coarse_flux(hipStream_t s, std::vector<double> &out)
 {
-    out.assign((size_t)nc_ * G_, 0.0);
-    CMFD_CHECK(hipMemcpyAsync(out.data(), d_cflux_, out.size() * 8, hipMemcpyDeviceToHost, s));
+    // the device keeps [G][nc]; the caller gets [nc][G]
+    std::vector<double> t((size_t)nc_ * G_, 0.0);
+    CMFD_CHECK(hipMemcpyAsync(t.data(), d_cflux_, t.size() * 8, hipMemcpyDeviceToHost, s));
     CMFD_CHECK(hipStreamSynchronize(s));
+    out.assign(t.size(), 0.0);
+    for (int c = 0; c < nc_; c++)
+        for (int g = 0; g < G_; g++) out[(size_t)c * G_ + g] = t[(size_t)g * nc_ + c];
 }
 
 // --- the CMFD step of many problems (batch engine) ------------------------------

@@ -15,6 +15,7 @@ namespace gpu {
 // sweeper handle). Layouts: phi [F][G], cur / sfl [n_surf][G].
 struct CmfdView {
     int F, G, npin, n_mat;
+    int nc;  // coarse cells of the plane (nx * ny)
     const int32_t *fsr_mat, *pin_off, *pin_cell;
     const double *area, *vol, *pin_area;
     const double *mat_tr, *mat_nf, *mat_f, *mat_ch, *mat_sc;  // [m][g], sc [m][to][from]
@@ -73,7 +74,7 @@ private:
     int bc_[6] = {0, 0, 0, 0, 0, 0};
     mocgpu_options opt_{};
     double *d_dx_ = nullptr, *d_dy_ = nullptr, *d_cvol_ = nullptr;
-    double *d_cflux_ = nullptr;                                             // [nc][G]
+    double *d_cflux_ = nullptr;                                             // [G][nc]
     double *d_htr_ = nullptr, *d_hnf_ = nullptr, *d_hf_ = nullptr, *d_hch_ = nullptr, *d_hrm_ = nullptr;  // [nc][G]
     double *d_hsc_ = nullptr;                                               // [nc][G][G]
     double *d_dtil_ = nullptr, *d_stil_ = nullptr, *d_dhat_ = nullptr, *d_shat_ = nullptr;  // [n_surf][G]

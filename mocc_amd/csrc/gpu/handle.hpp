@@ -315,6 +315,7 @@ struct mocgpu_handle {
     {
         CmfdView v{};
         v.F = F;
+        v.nc = nx * ny;
         v.G = G;
         v.npin = npin;
         v.n_mat = n_mat;
