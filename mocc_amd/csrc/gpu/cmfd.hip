@@ -375,6 +375,7 @@ struct SolveArgs {
     double k_tol, psi_tol, rr;
     const double *coef;  // [5][G][nc]
     const double *hnf, *hch, *hsc, *cvol;
+    const uint64_t *smask;  // [G]: the groups g' != g that scatter into g
     double *cflux;  // [G][nc]
     double *gscr;   // [3][nc] global scratch of the solves that keep the rows in LDS
     double *k;
@@ -738,8 +739,10 @@ __device__ __forceinline__ void cmfd_source(const SolveArgs &A, int g, const Lan
     double s[K];
 #pragma unroll
     for (int k = 0; k < K; k++) s[k] = A.hch[g * nc + ci[k]] * fsc[k];
-    for (int gg = 0; gg < G; gg++) {
-        if (gg == g) continue;
+    // the groups g' no material scatters from into g add exact zeros: they
+    // are skipped, the others added in increasing g'
+    for (uint64_t m = A.smask[g]; m != 0; m &= m - 1) {
+        const int gg = __builtin_ctzll(m);
         const double *sc = A.hsc + ((size_t)g * G + gg) * nc, *fl = A.cflux + gg * nc;
 #pragma unroll
         for (int k = 0; k < K; k++) s[k] += sc[ci[k]] * fl[ci[k]];
@@ -1360,6 +1363,21 @@ void CmfdSolver::free_all()
     }
     if (d_info_) (void)hipFree(d_info_);
     d_info_ = nullptr;
+    if (d_smask_) (void)hipFree(d_smask_);
+    d_smask_ = nullptr;
+}
+
+void CmfdSolver::update_xs(const mocgpu_problem &p, hipStream_t s)
+{
+    if (!d_smask_) return;
+    if (p.n_group != G_) throw std::runtime_error("CMFD: cross sections of another group count");
+    std::vector<uint64_t> m(G_, 0);
+    for (int mat = 0; mat < p.n_mat; mat++)
+        for (int g = 0; g < G_; g++)
+            for (int gg = 0; gg < G_; gg++)
+                if (gg != g && p.xs_sc[((size_t)mat * G_ + g) * G_ + gg] != 0.0) m[g] |= 1ull << gg;
+    CMFD_CHECK(hipMemcpyAsync(d_smask_, m.data(), m.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    CMFD_CHECK(hipStreamSynchronize(s));
 }
 
 void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipStream_t s)
@@ -1420,6 +1438,9 @@ void CmfdSolver::setup(const mocgpu_problem &p, const mocgpu_options &opt, hipSt
     dalloc(d_part_, 2 * (size_t)n_surf_ * G_);
     dalloc(d_part_old_, 2 * (size_t)n_surf_ * G_);
     dalloc(d_info_, 3);
+    if (G_ > 64) throw std::runtime_error("CMFD: at most 64 groups");
+    dalloc(d_smask_, G_);
+    update_xs(p, s);
     initialize(s);
 }
 
@@ -1479,6 +1500,7 @@ struct CmfdAccess {
         a.hch = c.d_hch_;
         a.hsc = c.d_hsc_;
         a.cvol = c.d_cvol_;
+        a.smask = c.d_smask_;
         a.cflux = c.d_cflux_;
         a.gscr = c.d_gscr_;
         a.k = v.k;

@@ -35,6 +35,9 @@ public:
     ~CmfdSolver();
     void setup(const mocgpu_problem &p, const mocgpu_options &opt, hipStream_t s);
     void initialize(hipStream_t s);
+    // The groups that scatter into each group (from the material cross
+    // sections); called by setup and again when the cross sections change.
+    void update_xs(const mocgpu_problem &p, hipStream_t s);
     // EigenSolver::do_cmfd (eigen_solver.cpp:263-291) enqueued on s: coarse
     // flux from the fine flux, CMFD::solve(k) with k on the device, then the
     // fine flux rescaled pin by pin. Nothing is read back.
@@ -75,6 +78,7 @@ private:
     mocgpu_options opt_{};
     double *d_dx_ = nullptr, *d_dy_ = nullptr, *d_cvol_ = nullptr;
     double *d_cflux_ = nullptr;                                             // [G][nc]
+    uint64_t *d_smask_ = nullptr;  // [G]: bit g' set when some material scatters from g' into g (g' != g)
     double *d_htr_ = nullptr, *d_hnf_ = nullptr, *d_hf_ = nullptr, *d_hch_ = nullptr, *d_hrm_ = nullptr;  // [nc][G]
     double *d_hsc_ = nullptr;                                               // [nc][G][G]
     double *d_dtil_ = nullptr, *d_stil_ = nullptr, *d_dhat_ = nullptr, *d_shat_ = nullptr;  // [n_surf][G]

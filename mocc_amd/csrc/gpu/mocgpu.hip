@@ -662,6 +662,7 @@ int mocgpu_reload_xs(mocgpu_handle *h, const mocgpu_problem *p)
         if (!same)
             throw std::runtime_error("mocgpu_reload_xs: the problem's geometry or group count differs from the handle's");
         put_materials(h, p);
+        if (h->cmfd) h->cmfd->update_xs(*p, h->stream);
         // the state a new handle starts from (mocgpu_initialize sets the rest)
         const size_t fg = (size_t)h->F * h->G;
         HIP_CHECK(hipMemsetAsync(h->d_src.p, 0, sizeof(double) * fg, h->stream));
