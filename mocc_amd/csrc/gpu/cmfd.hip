@@ -1141,12 +1141,15 @@ __device__ __forceinline__ const double *wide_spmv(const SolveArgs &A, const Wid
 // The Jacobi-preconditioned BiCGSTAB of bicgstab() above, step for step, with
 // the inverse diagonal and the shadow residual in registers and the
 // right-hand side sb in global scratch (read at the start and at a restart).
+// ax is A x of the initial guess, which the caller has already formed.
 template <int NT, int K>
 __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const Lane<K> &L, const double (&inv)[K],
-                             double (&x)[K], double tol, int max_iter, WideScratch<NT> &S, const double *sb)
+                             double (&x)[K], const double (&ax)[K], double tol, int max_iter, WideScratch<NT> &S,
+                             const double *sb)
 {
     double r[K], p[K], v[K], t[K], r0[K];
-    wide_spmv<NT, K>(A, R, L, x, t, S);
+#pragma unroll
+    for (int k = 0; k < K; k++) t[k] = ax[k];
     double lr0 = 0.0, lb = 0.0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -1298,7 +1301,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
                 sb[L.c[k]] = b[k];  // 0 at dummies
             }
             ri += wide_block_sum(s, S);
-            n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, tol, 150, S, sb);
+            n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, t, tol, 150, S, sb);
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)g * A.nc + L.c[k]] = x[k];
