@@ -318,6 +318,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--serial-cases", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--lane-map", type=int, default=-1,
+                    help="lane map of the 17x17 sweep: -1 autotunes band width and lane map at create")
+    ap.add_argument("--band-rays", type=int, default=0, help="band width of the 17x17 sweep with a fixed lane map")
     ap.add_argument("--batch-lanes", type=int, default=2,
                     help="batches (each on its own stream) the batched branch cases are spread over")
     ap.add_argument("--serve-lanes", type=int, default=1,
@@ -347,7 +350,12 @@ def main():
     path = lg.write_config("assembly17", deck_dir)
     hp = HostProblem.load(path)
     p = hp.p
-    dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=1)
+    # band width and lane map picked at create by timing the candidates
+    # (--lane-map -1), or as given
+    t0 = time.perf_counter()
+    dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=1, lane_map=args.lane_map, band_rays=args.band_rays)
+    t_tune = time.perf_counter() - t0
+    tuned = dev.options()
     dev.initialize()
     dev.calc_fission_source(1.0)
     dev.store_old_flux()
@@ -374,9 +382,14 @@ def main():
     B = algorithmic_bytes(p)
     achieved = B / (kernel_ms * 1e-3) / 1e9
     pmc = read_pmc(os.path.join(ROOT, "profiles", "sweep_assembly17_pmc.txt"))
+    # the counters were taken with one band layout; they stand for this run
+    # only when it runs the same one
+    if (pmc.get("band_rays"), pmc.get("lane_map")) != (float(tuned.band_rays), float(tuned.lane_map)):
+        pmc = {}
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": pmc.get("hbm_bytes_per_sweep"),
-                "kernel": "k_sweep<3,false>, both Gauss-Seidel phase launches of one sweep",
+                "kernel": f"k_sweep<3,false>, both Gauss-Seidel phase launches of one sweep "
+                          f"({int(tuned.band_rays)}-ray bands, lane map {int(tuned.lane_map)})",
                 "kernel_ms": kernel_ms, "algorithmic_bytes": B}
     insts = pmc.get("sq_insts_valu_per_sweep")
     if insts:
@@ -391,7 +404,8 @@ def main():
         "config": {"workload": "17x17 assembly (25 guide tubes), 7 groups, CY 32x3, spacing 0.03 cm, "
                                "reflective; one all-group inner sweep per step",
                    "n_fsr": p.n_fsr, "n_seg": p.n_seg, "n_ray": p.n_ray, "updates_per_step": U,
-                   "parallelism": "single GPU"},
+                   "band_rays": int(tuned.band_rays), "lane_map": int(tuned.lane_map),
+                   "autotune_s": round(t_tune, 3), "parallelism": "single GPU"},
         "roofline": roofline,
     }
     progress(f"17x17 7-group sweep: {value:.4g} updates/s")

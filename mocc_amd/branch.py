@@ -123,7 +123,7 @@ class BranchFactory:
             return GpuMoCSweeper(hp, n_inner=self.n_inner, energy_mode=self.energy_mode, use_cmfd=self.use_cmfd,
                                  like=proto)
 
-    def tune(self, sample: list[Case], widths=(64, 48, 32), maps=(1, 0), n_outer: int = 3) -> tuple[int, int]:
+    def tune(self, sample: list[Case], widths=(64, 48, 32), maps=(1, 0, 2), n_outer: int = 3) -> tuple[int, int]:
         """Band width x lane map for the batched workload, by timing it: for
         every candidate, all geometries get that layout, and n_outer outer
         iterations of one graph-replayed batch of the sample cases are timed

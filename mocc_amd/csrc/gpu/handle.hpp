@@ -172,7 +172,7 @@ struct mocgpu_handle {
         a.n_lslots = (int)std::max<size_t>(1, plan->lsurf_glob.size());
         a.parity = parity;
         a.gs = opt.gs_boundary ? 1 : 0;
-        a.interleave = opt.lane_map == 0 ? 0 : 1;
+        a.interleave = opt.lane_map;  // 0, 1 or 2 (checked at create)
         a.wg_rays = plan->max_band_rays <= 32 ? 32 : kRays;
         a.seg_loc = d_seg_loc.p;
         a.seg_len = d_seg_len.p;

@@ -27,7 +27,7 @@ struct SweepArgs {
     uint64_t parity;        // bit g: which boundary buffer of group g is incoming
     const uint64_t *parity_dev;  // if set, the parity word is read from device memory instead
     int32_t gs;             // 1: Gauss-Seidel boundary update (exit fluxes go to the incoming buffer)
-    int32_t interleave;     // 1: each wave takes even rays one way and odd rays the other
+    int32_t interleave;     // lane map: 0 plain, 1 ray-parity interleave, 2 length split
     int32_t wg_rays;        // rays per direction of a workgroup: 64 (two waves) or 32 (one wave); the plan's widest band fits
     const uint16_t *seg_loc;
     const float *seg_len;

@@ -375,6 +375,8 @@ int create_impl(const mocgpu_problem *p, const mocgpu_options *opt, const mocgpu
         if (opt) h->opt = *opt;
         else mocgpu_default_options(&h->opt);
         if (p->n_group > 64) throw std::runtime_error("at most 64 energy groups are supported");
+        if (h->opt.lane_map < 0 || h->opt.lane_map > 2)
+            throw std::runtime_error("lane_map must be 0, 1 or 2 (-1: autotune)");
         if (like) {
             bool same = p->n_fsr == like->F && p->n_seg == like->n_seg && p->n_polar == like->NP &&
                         p->n_azi == like->A && p->nx == like->nx && p->ny == like->ny &&
@@ -543,7 +545,7 @@ int autotune_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_h
 {
     mocgpu_options o = *opt;
     const int widths[3] = {64, 48, 32};
-    const int maps[2] = {1, 0};
+    const int maps[3] = {1, 0, 2};
     mocgpu_handle *best = nullptr;
     double best_ms = 0.0;
     int best_map = 1;
@@ -562,7 +564,9 @@ int autotune_create(const mocgpu_problem *p, const mocgpu_options *opt, mocgpu_h
         for (int m : maps) {
             h->opt.lane_map = m;
             double ms = 0.0, kms = 0.0;
-            rc = mocgpu_time_sweeps(h, 3, &ms, &kms);
+            // a first run settles the clocks; the second is the one kept
+            rc = mocgpu_time_sweeps(h, 5, &ms, &kms);
+            if (rc == 0) rc = mocgpu_time_sweeps(h, 10, &ms, &kms);
             if (rc != 0) {
                 mocgpu_destroy(h);
                 if (best) mocgpu_destroy(best);

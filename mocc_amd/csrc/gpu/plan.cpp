@@ -191,6 +191,12 @@ void build_azimuth(const mocgpu_problem &p, const Coarse &cm, int a, int max_ray
             brays.push_back(r);
             i++;
         }
+        // the band's rays longest first (ties in spatial order): with the
+        // length-split lane map each wave takes 32 consecutive rays in both
+        // directions, so a wave of short rays ends early
+        std::stable_sort(brays.begin(), brays.end(), [&p](int x, int y) {
+            return p.ray_seg_off[x + 1] - p.ray_seg_off[x] > p.ray_seg_off[y + 1] - p.ray_seg_off[y];
+        });
         std::sort(locs.begin(), locs.end());
         for (size_t l = 0; l < locs.size(); l++) sc.g2l[locs[l]] = (int)l;
         BandInfo b{};

@@ -126,13 +126,19 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
         }
     }
 
-    // Lane map. Plain: one wave sweeps the band forward, the other backward.
-    // Interleaved: each wave takes every ray, even rays in one direction and
-    // odd rays in the other (the other wave the reverse), so neighbouring
-    // rays enter a given FSR at different steps and fewer lanes of a wave hit
-    // the same LDS tally address in the same step.
-    const int j = tid % W;
-    const int dir = A.interleave ? ((j & 1) ^ (tid / W)) : (tid / W);
+    // Lane map. Plain (0): one wave sweeps the band forward, the other
+    // backward. Interleaved (1): each wave takes every ray, even rays in one
+    // direction and odd rays in the other (the other wave the reverse), so
+    // neighbouring rays enter a given FSR at different steps and fewer lanes
+    // of a wave hit the same LDS tally address in the same step. Length split
+    // (2): wave w takes rays 32w .. 32w + 31 in both directions (lanes 0-31
+    // forward, 32-63 backward); the plan orders a band's rays longest first,
+    // so the second wave runs only as long as its own longest ray, and not at
+    // all in a band of up to 32 rays. One wave per band (W = 32) is the plain
+    // map whatever the choice.
+    const int j = A.interleave == 2 ? (tid >> 6) * 32 + (tid & 31) : tid % W;
+    const int dir = A.interleave == 2 ? ((tid >> 5) & 1)
+                  : A.interleave ? ((j & 1) ^ (tid / W)) : (tid / W);
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
     // select (not index) the per-direction offsets: indexing a local array

@@ -152,13 +152,45 @@ def test_assembly17_fss_step_parity(assembly17, gs):
     assert _rel(dev.boundary(), ora.boundary()) < 1e-4
 
 
+@pytest.mark.parametrize("gs", [True, False])
+def test_lane_maps_bitwise(pins3x3, assembly17, gs):
+    """The three ray-to-lane maps (plain, ray-parity interleave, length
+    split) give bitwise the same flux, boundary fluxes and coarse currents:
+    the band tallies are integer sums, whose order does not matter. The
+    17x17 deck's 64-ray bands exercise the length split's second wave."""
+    from mocc_amd.sweeper import GpuMoCSweeper
+    for hp in (pins3x3, assembly17):
+        rng = np.random.default_rng(11)
+        phi0 = rng.uniform(0.5, 2.0, (hp.n_fsr, hp.n_group))
+        outs = []
+        for m in (0, 1, 2):
+            print(f"lane map {m}, {hp.n_fsr} FSRs, gs={gs}", flush=True)
+            dev = GpuMoCSweeper(hp, n_inner=1, energy_mode=1, gs_boundary=gs, lane_map=m)
+            assert dev.options().lane_map == m
+            dev.initialize()
+            dev.set_flux(phi0)
+            dev.calc_fission_source(1.0)
+            dev.fss_step(with_current=True)
+            outs.append([dev.flux(), dev.boundary()] + [c for g in range(hp.n_group) for c in dev.currents(g)])
+            dev.close()
+        for o in outs[1:]:
+            for a, b in zip(outs[0], o):
+                assert np.array_equal(a, b)
+
+
+def test_lane_map_checked(pins3x3):
+    from mocc_amd.sweeper import DeviceError, GpuMoCSweeper
+    with pytest.raises(DeviceError, match="lane_map"):
+        GpuMoCSweeper(pins3x3, lane_map=3)
+
+
 def test_autotuned_lane_map(pins3x3):
     """lane_map = -1: the autotuner picks a band width and lane map by timing
     the candidates; the tuned handle sweeps as accurately as the default."""
     from mocc_amd.sweeper import GpuMoCSweeper
     dev = GpuMoCSweeper(pins3x3, n_inner=1, energy_mode=0, gs_boundary=True, lane_map=-1)
     o = dev.options()
-    assert o.band_rays in (32, 48, 64) and o.lane_map in (0, 1)
+    assert o.band_rays in (32, 48, 64) and o.lane_map in (0, 1, 2)
     ora = Oracle(pins3x3, n_inner=1, gs_boundary=True, energy_mode=0)
     rng = np.random.default_rng(2)
     phi0 = rng.uniform(0.5, 2.0, (pins3x3.n_fsr, pins3x3.n_group))
