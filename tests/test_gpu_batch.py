@@ -403,7 +403,7 @@ def test_autotuned_branch_batch(tmp_path):
         if tune:
             w, m = f.tune(cases[:4], n_outer=2)
             print("tuned (band rays, lane map):", (w, m), "candidates:", f.tuning, flush=True)
-            assert w in (32, 48, 64) and m in (0, 1) and len(f.tuning) == 6
+            assert w in (32, 48, 64) and m in (0, 1, 2) and len(f.tuning) == 9
             assert (w, m) == min(f.tuning, key=lambda r: r[2])[:2]
         hs = [f.make(c) for c in cases]
         if tune:
