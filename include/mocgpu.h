@@ -124,11 +124,12 @@ typedef struct mocgpu_options {
     int32_t n_inner;        /* <sweeper n_inner>                           */
     int32_t energy_mode;    /* 0: Gauss-Seidel over groups (FixedSourceSolver::step),
                                1: Jacobi (all groups from the previous outer) */
-    int32_t lane_map;       /* ray/direction -> lane mapping of the sweep: 1 (default)
-                               ray-parity interleave of directions in each wave, 0 one
-                               direction per wave, 2 length split (each wave both
-                               directions of 32 rays, a band's rays longest first),
-                               -1 autotune lane_map x band_rays at create (times
+    int32_t lane_map;       /* ray/direction -> lane mapping of the sweep: 2 (default)
+                               length split (each wave both directions of 32 rays, a
+                               band's rays longest first, directions interleaved by
+                               ray parity), 1 ray-parity interleave of directions in
+                               each wave over all the band's rays, 0 one direction per
+                               wave, -1 autotune lane_map x band_rays at create (times
                                candidate plans on the device)                       */
     int32_t band_rays;      /* rays per band, 1..64 (0 = 64)                */
     int32_t use_cmfd;       /* <solver cmfd="t">                           */

@@ -311,7 +311,7 @@ void mocgpu_default_options(mocgpu_options *o)
     *o = mocgpu_options{};
     o->n_inner = 1;
     o->energy_mode = 0;
-    o->lane_map = 1;
+    o->lane_map = 2;
     o->band_rays = 0;
     o->use_cmfd = 0;
     o->cmfd_max_iter = 100;

@@ -131,13 +131,14 @@ __device__ __forceinline__ void sweep_band(const SweepArgs &A, int band, int g, 
     // direction and odd rays in the other (the other wave the reverse), so
     // neighbouring rays enter a given FSR at different steps and fewer lanes
     // of a wave hit the same LDS tally address in the same step. Length split
-    // (2): wave w takes rays 32w .. 32w + 31 in both directions (lanes 0-31
-    // forward, 32-63 backward); the plan orders a band's rays longest first,
-    // so the second wave runs only as long as its own longest ray, and not at
-    // all in a band of up to 32 rays. One wave per band (W = 32) is the plain
-    // map whatever the choice.
+    // (2): wave w takes rays 32w .. 32w + 31 in both directions, interleaved
+    // the same way (lanes 0-31 even rays forward and odd rays backward, lanes
+    // 32-63 the reverse); the plan orders a band's rays longest first, so the
+    // second wave runs only as long as its own longest ray, and not at all in
+    // a band of up to 32 rays. With one wave per band (W = 32), maps 0 and 1
+    // are the plain map and map 2 its interleaved form.
     const int j = A.interleave == 2 ? (tid >> 6) * 32 + (tid & 31) : tid % W;
-    const int dir = A.interleave == 2 ? ((tid >> 5) & 1)
+    const int dir = A.interleave == 2 ? (((tid >> 5) ^ tid) & 1)
                   : A.interleave ? ((j & 1) ^ (tid / W)) : (tid / W);
     const bool active = j < B.n_rays;
     const AziInfo az = A.azi[B.azi];
