@@ -720,6 +720,14 @@ __device__ int bicgstab(const SolveArgs &A, const Rows<K> &R, const Lane<K> &L, 
 // flight instead of one; each cell's sum keeps its order of terms. The
 // arrays are cell-fastest ([g][c], [g][g'][c]): a wave's lanes, which hold
 // consecutive cells, read consecutive addresses.
+// 1 at a lane's real cells, 0 at its dummies. Values loaded at clamped cells
+// are multiplied by it instead of selected: a select lets the compiler move
+// the load under a branch of its own, where it waits alone, while loads used
+// unconditionally are issued together. Every factor is finite, so a dummy
+// gets a zero (of either sign: it adds nothing to any sum) and a real cell
+// its value unchanged.
+template <int K> __device__ __forceinline__ double on_factor(const Lane<K> &L, int k) { return L.on[k] ? 1.0 : 0.0; }
+
 template <int K> __device__ __forceinline__ void lane_cells(const Lane<K> &L, int (&ci)[K])
 {
 #pragma unroll
@@ -748,7 +756,9 @@ __device__ __forceinline__ void cmfd_source(const SolveArgs &A, int g, const Lan
         for (int k = 0; k < K; k++) s[k] += sc[ci[k]] * fl[ci[k]];
     }
 #pragma unroll
-    for (int k = 0; k < K; k++) b[k] = L.on[k] ? s[k] * A.cvol[ci[k]] : 0.0;
+    for (int k = 0; k < K; k++) {
+        b[k] = s[k] * A.cvol[ci[k]] * on_factor<K>(L, k);
+    }
 }
 
 template <int K>
@@ -972,21 +982,6 @@ template <int CTRL> __device__ __forceinline__ double dpp_all(double v)
 __device__ __forceinline__ double dpp_wave_shr1(double v) { return dpp_all<0x138>(v); }
 __device__ __forceinline__ double dpp_wave_shl1(double v) { return dpp_all<0x130>(v); }
 
-// wave_sum() above, term for term (the same value), with the four in-row
-// steps moved without an old value.
-__device__ __forceinline__ double wide_wave_sum(double v)
-{
-    v += dpp_all<0xb1>(v);        // quad_perm [1,0,3,2]
-    v += dpp_all<0x4e>(v);        // quad_perm [2,3,0,1]
-    v += dpp_all<0x141>(v);       // row_half_mirror
-    v += dpp_all<0x140>(v);       // row_mirror
-    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-    return __hiloint2double(hi, lo);
-}
-
 // The sum of a row of 16 lanes in every lane of the row.
 __device__ __forceinline__ double row_sum16(double v)
 {
@@ -1003,47 +998,65 @@ __device__ __forceinline__ double read_lane_f64(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
-// Workgroup sums. Lane 0 of each wave publishes its wave's sum (in one of two
-// banks of 32 doubles, alternating, so each sum needs one barrier); every
-// wave then reads the partials into lanes 0-15 (the second sum of a pair into
-// lanes 16-31) and adds them by the same DPP tree, so every lane of every
-// wave gets the same value, independent of timing.
-constexpr int kWideRedBank = 32;
+// Workgroup sums of up to five values at once. Each row of 16 lanes sums its
+// lanes by DPP (every lane of the row gets the same sum); lane 0 of each row
+// publishes the row sums, value j of row r at j P + r with P = NT / 16 rows,
+// in one of two banks, alternating, so each sum needs one barrier. Every
+// wave then reads value j into its row j (value 4 into a second register),
+// where each lane adds P / 16 row sums in a fixed order and the row adds its
+// 16 lanes by the same DPP tree: every lane of every wave gets the same
+// values, independent of timing.
+template <int NT> constexpr int wide_red_rows() { return NT / 16; }
+template <int NT> constexpr int wide_red_bank() { return 5 * wide_red_rows<NT>(); }  // doubles per bank
 template <int NT> struct WideScratch {
-    double *red;  // [2][kWideRedBank]
+    double *red;  // [2][wide_red_bank<NT>()]
     double *buf;  // operand bank 0 (bank 1 follows it)
     int rb, bb;
 };
-template <int NT> __device__ __forceinline__ double wide_block_sum(double v, WideScratch<NT> &S)
+template <int NT, int N>
+__device__ __forceinline__ void wide_sum(const double (&v)[N], WideScratch<NT> &S, double (&out)[N])
 {
-    static_assert(NT / 64 <= 16, "one row of 16 lanes holds the wave partials");
-    v = wide_wave_sum(v);
-    double *red = S.red + S.rb * kWideRedBank;
+    constexpr int P = wide_red_rows<NT>();
+    static_assert(N >= 1 && N <= 5, "at most five values per sum");
+    static_assert(P % 16 == 0 && P <= 64, "whole rows of row sums");
+    double *red = S.red + S.rb * wide_red_bank<NT>();
     S.rb ^= 1;
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const int l = threadIdx.x & 63;
-    double t = l < NT / 64 ? red[l] : 0.0;
-    t = row_sum16(t);
-    return read_lane_f64(t, 0);
-}
-template <int NT>
-__device__ __forceinline__ void wide_block_sum2(double a, double b, WideScratch<NT> &S, double &sa, double &sb)
-{
-    a = wide_wave_sum(a);
-    b = wide_wave_sum(b);
-    double *red = S.red + S.rb * kWideRedBank;
-    S.rb ^= 1;
-    if ((threadIdx.x & 63) == 0) {
-        red[threadIdx.x >> 6] = a;
-        red[16 + (threadIdx.x >> 6)] = b;
+    double rs[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) rs[j] = row_sum16(v[j]);
+    if ((threadIdx.x & 15) == 0) {
+        const int r = threadIdx.x >> 4;
+#pragma unroll
+        for (int j = 0; j < N; j++) red[j * P + r] = rs[j];
     }
     __syncthreads();
-    const int l = threadIdx.x & 63;
-    double t = (l & 15) < NT / 64 && l < 32 ? red[l] : 0.0;
-    t = row_sum16(t);
-    sa = read_lane_f64(t, 0);
-    sb = read_lane_f64(t, 16);
+    const int l = threadIdx.x & 63, i = l & 15;
+    constexpr int N4 = N < 4 ? N : 4;
+    {
+        const int j = (l >> 4) < N4 ? (l >> 4) : N4 - 1;  // rows past the last value repeat it
+        const double *q = red + j * P + i;
+        double t = q[0];
+#pragma unroll
+        for (int m = 1; m < P / 16; m++) t += q[16 * m];
+        t = row_sum16(t);
+#pragma unroll
+        for (int j2 = 0; j2 < N4; j2++) out[j2] = read_lane_f64(t, 16 * j2);
+    }
+    if constexpr (N == 5) {
+        const double *q = red + 4 * P + i;
+        double t = q[0];
+#pragma unroll
+        for (int m = 1; m < P / 16; m++) t += q[16 * m];
+        t = row_sum16(t);
+        out[4] = read_lane_f64(t, 0);
+    }
+}
+template <int NT> __device__ __forceinline__ double wide_sum1(double v, WideScratch<NT> &S)
+{
+    const double in[1] = {v};
+    double out[1];
+    wide_sum<NT, 1>(in, S, out);
+    return out[0];
 }
 
 // The lane index through an opaque move: addresses formed from it are formed
@@ -1067,15 +1080,27 @@ __device__ __forceinline__ void wide_load_rows(const SolveArgs &A, int g, const 
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int cl = L.on[k] ? L.c[k] : 0;
-        const double dg = L.on[k] ? R.cf[2 * pl + cl] : 0.0;
+        const double m = on_factor<K>(L, k);
+        const double dg = R.cf[2 * pl + cl] * m;
         inv[k] = (dg != 0.0) ? 1.0 / dg : 1.0;
         if constexpr (wide_rows_in_regs<NT, K>()) {
-            R.a[0][k] = L.on[k] ? R.cf[0 * pl + cl] : 0.0;
-            R.a[1][k] = L.on[k] ? R.cf[1 * pl + cl] : 0.0;
+            R.a[0][k] = R.cf[0 * pl + cl] * m;
+            R.a[1][k] = R.cf[1 * pl + cl] * m;
             R.a[2][k] = dg;
-            R.a[3][k] = L.on[k] ? R.cf[3 * pl + cl] : 0.0;
-            R.a[4][k] = L.on[k] ? R.cf[4 * pl + cl] : 0.0;
+            R.a[3][k] = R.cf[3 * pl + cl] * m;
+            R.a[4][k] = R.cf[4 * pl + cl] * m;
         }
+    }
+}
+
+// The stored coarse flux of group g at this lane's cells (0 at the dummies).
+template <int K>
+__device__ __forceinline__ void wide_load_flux(const SolveArgs &A, int g, const Lane<K> &L, double (&x)[K])
+{
+    const double *fl = A.cflux + (size_t)g * A.nc;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        x[k] = fl[L.on[k] ? L.c[k] : 0] * on_factor<K>(L, k);
     }
 }
 
@@ -1138,31 +1163,45 @@ __device__ __forceinline__ const double *wide_spmv(const SolveArgs &A, const Wid
     return buf;
 }
 
-// The Jacobi-preconditioned BiCGSTAB of bicgstab() above, step for step, with
-// the inverse diagonal and the shadow residual in registers and the
-// right-hand side sb in global scratch (read at the start and at a restart).
-// ax is A x of the initial guess, which the caller has already formed.
+// The Jacobi-preconditioned BiCGSTAB of bicgstab() above, with the inverse
+// diagonal and the shadow residual in registers and the right-hand side b in
+// registers at the start (and in global scratch sb, read at a restart). ax is
+// A x of the initial guess, which the caller has already formed; r0_sq
+// returns ||b - A x||^2 of that guess.
+//
+// An iteration has four workgroup phases instead of five: the products
+// v = A y and t = A z, alpha's sum, and one sum of five values after t, from
+// which the next residual's norm and rho follow without a sum of their own,
+// r = s - w t giving
+//   r.r  = s.s - w t.s   (w = t.s / t.t),   r0.r = r0.s - w r0.t.
+// Where the first loses digits to cancellation (r.r < 1e-4 s.s), where the
+// second does (|r0.r| < 1e-4 |r0.s|), and wherever r.r is within a factor 4
+// of the stopping threshold, both are summed directly instead, so the stopping
+// test always compares a directly summed r.r.
 template <int NT, int K>
 __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const Lane<K> &L, const double (&inv)[K],
-                             double (&x)[K], const double (&ax)[K], double tol, int max_iter, WideScratch<NT> &S,
-                             const double *sb)
+                             double (&x)[K], const double (&ax)[K], const double (&b)[K], double tol, int max_iter,
+                             WideScratch<NT> &S, const double *sb, double &r0_sq)
 {
     double r[K], p[K], v[K], t[K], r0[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) t[k] = ax[k];
-    double lr0 = 0.0, lb = 0.0;
+    double ls[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gs[5];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const double bk = sb[L.c[k]];
-        r[k] = bk - t[k];
+        r[k] = b[k] - ax[k];
         r0[k] = r[k];
         p[k] = 0.0;
         v[k] = 0.0;
-        lr0 = fma(r[k], r[k], lr0);
-        lb = fma(bk, bk, lb);
+        ls[0] = fma(r[k], r[k], ls[0]);
+        ls[1] = fma(b[k], b[k], ls[1]);
     }
-    double r0_sq, rhs_sq;
-    wide_block_sum2(lr0, lb, S, r0_sq, rhs_sq);
+    {
+        const double in[2] = {ls[0], ls[1]};
+        double out[2];
+        wide_sum<NT, 2>(in, S, out);
+        r0_sq = out[0];
+        gs[1] = out[1];
+    }
+    const double rhs_sq = gs[1];
     if (rhs_sq == 0.0) {
 #pragma unroll
         for (int k = 0; k < K; k++) x[k] = 0.0;
@@ -1172,20 +1211,14 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
     const double tol2 = tol * tol * rhs_sq;
     const double eps = 2.220446049250313e-16;
     const double eps2 = eps * eps;
+    double r0r0 = r0_sq;              // r0.r0 (changes at a restart)
+    double rr = r0_sq, rho_new = r0_sq;  // r = r0 at the start
     int i = 0, restarts = 0;
     while (true) {
-        double lrr = 0.0, lrho = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            lrr = fma(r[k], r[k], lrr);
-            lrho = fma(r0[k], r[k], lrho);
-        }
-        double rr, rho_new;
-        wide_block_sum2(lrr, lrho, S, rr, rho_new);
         if (!(rr > tol2 && i < max_iter)) break;
         const double rho_old = rho;
         rho = rho_new;
-        if (fabs(rho) < eps2 * r0_sq) {
+        if (fabs(rho) < eps2 * r0r0) {
             wide_spmv<NT, K>(A, R, L, x, t, S);
             const double *bl = sb + opaque_lane();
             double l = 0.0;
@@ -1195,7 +1228,7 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
                 r0[k] = r[k];
                 l = fma(r[k], r[k], l);
             }
-            rho = r0_sq = wide_block_sum(l, S);
+            rho = r0r0 = wide_sum1<NT>(l, S);
             if (restarts++ == 0) i = 0;
         }
         const double beta = (rho / rho_old) * (alpha / w);
@@ -1208,7 +1241,7 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
         double l = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) l = fma(r0[k], v[k], l);
-        alpha = rho / wide_block_sum(l, S);
+        alpha = rho / wide_sum1<NT>(l, S);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             x[k] = fma(alpha, t[k], x[k]);
@@ -1216,19 +1249,36 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
             t[k] = inv[k] * r[k];            // z
         }
         const double *zb = wide_spmv<NT, K>(A, R, L, t, t, S) + opaque_lane();  // t = A z
-        double ltt = 0.0, lts = 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) ls[j] = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            ltt = fma(t[k], t[k], ltt);
-            lts = fma(t[k], r[k], lts);
+            ls[0] = fma(t[k], t[k], ls[0]);
+            ls[1] = fma(t[k], r[k], ls[1]);
+            ls[2] = fma(r[k], r[k], ls[2]);
+            ls[3] = fma(r0[k], r[k], ls[3]);
+            ls[4] = fma(r0[k], t[k], ls[4]);
         }
-        double tt, ts;
-        wide_block_sum2(ltt, lts, S, tt, ts);
+        wide_sum<NT, 5>(ls, S, gs);
+        const double tt = gs[0], ts = gs[1], ss = gs[2], r0s = gs[3], r0t = gs[4];
         w = (tt > 0.0) ? ts / tt : 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             x[k] = fma(w, zb[k * NT], x[k]);
             r[k] = fma(-w, t[k], r[k]);
+        }
+        rr = fma(-w, ts, ss);
+        rho_new = fma(-w, r0t, r0s);
+        if (!(rr > 4.0 * tol2 && rr > 1e-4 * ss && fabs(rho_new) > 1e-4 * fabs(r0s))) {
+            double in[2] = {0.0, 0.0}, out[2];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                in[0] = fma(r[k], r[k], in[0]);
+                in[1] = fma(r0[k], r[k], in[1]);
+            }
+            wide_sum<NT, 2>(in, S, out);
+            rr = out[0];
+            rho_new = out[1];
         }
         ++i;
     }
@@ -1261,13 +1311,12 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
     const int G = A.G;
     double k_eff = A.k[0];
     double fsc[K], b[K], x[K], t[K], inv[K];
-    double tfis = wide_block_sum(fission_source<K>(A, L, k_eff, fsc), S);
+    double tfis = wide_sum1<NT>(fission_source<K>(A, L, k_eff, fsc), S);
     double r0 = 0.0;
     for (int g = 0; g < G; g++) {
         cmfd_source<K>(A, g, L, fsc, b);
         wide_load_rows<NT, K>(A, g, L, R, inv);
-#pragma unroll
-        for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)g * A.nc + L.c[k]] : 0.0;
+        wide_load_flux<K>(A, g, L, x);
         wide_spmv<NT, K>(A, R, L, x, t, S);
         double s = 0.0;
 #pragma unroll
@@ -1275,7 +1324,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
             const double d = t[k] - b[k];
             s = fma(d, d, s);
         }
-        r0 += wide_block_sum(s, S);
+        r0 += wide_sum1<NT>(s, S);
     }
     r0 = sqrt(r0) / (G * A.nc);
     const double tol = A.rr * r0;
@@ -1290,27 +1339,23 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
         for (int g = 0; g < G; g++) {
             cmfd_source<K>(A, g, L, fsc, b);
             wide_load_rows<NT, K>(A, g, L, R, inv);
-#pragma unroll
-            for (int k = 0; k < K; k++) x[k] = L.on[k] ? A.cflux[(size_t)g * A.nc + L.c[k]] : 0.0;
+            wide_load_flux<K>(A, g, L, x);
             wide_spmv<NT, K>(A, R, L, x, t, S);
-            double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double d = t[k] - b[k];
-                s = fma(d, d, s);
-                sb[L.c[k]] = b[k];  // 0 at dummies
-            }
-            ri += wide_block_sum(s, S);
-            n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, t, tol, 150, S, sb);
+            for (int k = 0; k < K; k++) sb[L.c[k]] = b[k];  // 0 at dummies
+            // ||A x - b||^2 of the stored flux: BiCGSTAB's initial residual
+            double rg;
+            n_bicg += wide_bicgstab<NT, K>(A, R, L, inv, x, t, b, tol, 150, S, sb, rg);
+            ri += rg;
 #pragma unroll
             for (int k = 0; k < K; k++)
                 if (L.on[k]) A.cflux[(size_t)g * A.nc + L.c[k]] = x[k];
         }
         ri = sqrt(ri) / (A.nc * G);
-        tfis = wide_block_sum(lane_fission_total<K>(A, L), S);
+        tfis = wide_sum1<NT>(lane_fission_total<K>(A, L), S);
         const double k_old = k_eff;
         k_eff = k_eff * tfis / tfis_old;
-        const double psi_err = sqrt(wide_block_sum(lane_source_change<K>(L, fsc, fs_old), S));
+        const double psi_err = sqrt(wide_sum1<NT>(lane_source_change<K>(L, fsc, fs_old), S));
         if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
             break;
     }
@@ -1331,7 +1376,7 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
 template <int NT, int K> __global__ __launch_bounds__(NT) void k_cmfd_solve_wide(SolveArgs A)
 {
     extern __shared__ double lds[];
-    __shared__ double red[2 * kWideRedBank];
+    __shared__ double red[2 * wide_red_bank<NT>()];
     cmfd_solve_wide<NT, K>(A, lds, red);
 }
 
@@ -1692,9 +1737,9 @@ template <int K>
 __global__ __launch_bounds__(kCmfdThreads) void k_cmfd_b_solve_mixed(const CmfdCase *__restrict__ cs,
                                                                      const int32_t *__restrict__ status)
 {
-    static_assert(2 * kWideRedBank >= 2 * red_bank<kCmfdThreads>(), "one reduction array serves both solves");
+    static_assert(wide_red_bank<kCmfdThreads>() >= red_bank<kCmfdThreads>(), "one reduction array serves both solves");
     extern __shared__ double lds[];
-    __shared__ double red[2 * kWideRedBank];
+    __shared__ double red[2 * wide_red_bank<kCmfdThreads>()];
     const CmfdCase &d = cs[blockIdx.x];
     if (case_done(d, status)) return;
     switch (__builtin_amdgcn_readfirstlane(d.wide_k)) {

@@ -221,11 +221,11 @@ def test_batch_cmfd_graph_matches_eager(batch_decks):
 
 
 @pytest.mark.parametrize("big", ["lat48", "core51"])
-def test_batch_cmfd_mixed_mesh_sizes_bitwise(cmfd_decks, big):
-    """The coarse solve of a batch runs at the width of its largest mesh (5
-    or 6 cells per lane with the rows in LDS for the 48x48 lattice or the
-    51x51 core, against one wave for 3x3 pins alone); a small problem's k,
-    iterations and flux are bitwise the same either way."""
+def test_batch_cmfd_pins_beside_wide_mesh_bitwise(cmfd_decks, big):
+    """A 3x3-pin problem beside the 48x48 lattice or the 51x51 core (the wide
+    coarse solve at 5 or 6 cells per lane for the large mesh, the 512-lane
+    solve for the 3x3 one, in one launch): the small problem's k, iterations
+    and flux are bitwise those of its batch alone."""
     from mocc_amd.sweeper import GpuMoCSweeper
     mk = lambda n: GpuMoCSweeper(cmfd_decks[n], energy_mode=1, n_inner=5, use_cmfd=True)
     print("cmfd 3x3 pins alone", flush=True)
