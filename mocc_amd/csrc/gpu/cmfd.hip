@@ -1231,7 +1231,9 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
             rho = r0r0 = wide_sum1<NT>(l, S);
             if (restarts++ == 0) i = 0;
         }
-        const double beta = (rho / rho_old) * (alpha / w);
+        // (rho / rho_old) (alpha / w) with one divide: one divide fewer on
+        // the path from the last sum to the next product
+        const double beta = (rho * alpha) / (rho_old * w);
 #pragma unroll
         for (int k = 0; k < K; k++) {
             p[k] = fma(beta, fma(-w, v[k], p[k]), r[k]);
