@@ -1174,10 +1174,12 @@ __device__ __forceinline__ const double *wide_spmv(const SolveArgs &A, const Wid
 // which the next residual's norm and rho follow without a sum of their own,
 // r = s - w t giving
 //   r.r  = s.s - w t.s   (w = t.s / t.t),   r0.r = r0.s - w r0.t.
-// Where the first loses digits to cancellation (r.r < 1e-4 s.s), where the
-// second does (|r0.r| < 1e-4 |r0.s|), and wherever r.r is within a factor 4
-// of the stopping threshold, both are summed directly instead, so the stopping
-// test always compares a directly summed r.r.
+// Where the first would lose more than four digits to cancellation
+// (r.r < 1e-4 s.s), or the second would (|r0.r| < 1e-4 |r0.s|), both are
+// summed directly instead. Otherwise r.r carries a relative error of at most
+// about 1e4 ulps, so the stopping test r.r > tol^2 b.b can differ from one on
+// a direct sum (itself rounded in a different order) only where r.r lies
+// within that error of the threshold.
 template <int NT, int K>
 __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const Lane<K> &L, const double (&inv)[K],
                              double (&x)[K], const double (&ax)[K], const double (&b)[K], double tol, int max_iter,
@@ -1271,7 +1273,7 @@ __device__ int wide_bicgstab(const SolveArgs &A, const WideRows<NT, K> &R, const
         }
         rr = fma(-w, ts, ss);
         rho_new = fma(-w, r0t, r0s);
-        if (!(rr > 4.0 * tol2 && rr > 1e-4 * ss && fabs(rho_new) > 1e-4 * fabs(r0s))) {
+        if (!(rr > 1e-4 * ss && fabs(rho_new) > 1e-4 * fabs(r0s))) {
             double in[2] = {0.0, 0.0}, out[2];
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -1354,10 +1356,15 @@ template <int NT, int K> __device__ __forceinline__ void cmfd_solve_wide(const S
                 if (L.on[k]) A.cflux[(size_t)g * A.nc + L.c[k]] = x[k];
         }
         ri = sqrt(ri) / (A.nc * G);
-        tfis = wide_sum1<NT>(lane_fission_total<K>(A, L), S);
+        // the fission total and the source change in one sum (each value
+        // summed as alone)
+        const double in[2] = {lane_fission_total<K>(A, L), lane_source_change<K>(L, fsc, fs_old)};
+        double out[2];
+        wide_sum<NT, 2>(in, S, out);
+        tfis = out[0];
         const double k_old = k_eff;
         k_eff = k_eff * tfis / tfis_old;
-        const double psi_err = sqrt(wide_sum1<NT>(lane_source_change<K>(L, fsc, fs_old), S));
+        const double psi_err = sqrt(out[1]);
         if (((fabs(k_eff - k_old) < A.k_tol) && (psi_err < A.psi_tol) && (ri / r0 < A.rr)) || (iter > A.max_iter))
             break;
     }
